@@ -1,0 +1,259 @@
+"""FP64 reference evaluation on the node DAG (TEST INFRASTRUCTURE ONLY -- see oracle/__init__.py).
+
+Independent of the compiled layout: every vector of nodes is evaluated from its own
+``edge_ids`` and parameters.  Parameters are read from the DAG (``ns._params``; a tied vector
+uses its source's), so a caller first copies the compiled circuit's parameters back with
+``pc.update_parameters()``.
+
+Maths (reference citations):
+* sum:   ``v[n] = log sum_c W[n, c] exp(v_c)``                       (sum_layer.py:454-568)
+* prod:  ``v[n] = sum_k v[c_k(n)]``                                  (prod_layer.py:398-466)
+* Categorical ``log theta[x]``, Bernoulli ``log(theta or 1 - theta)`` (categorical.py:83-91,
+  bernoulli.py:65-70)
+* flows: ``f_c += sum_n f_n W[n, c] exp(v_c - v_n)``; parameter flows
+  ``pf[n, c] = W[n, c] sum_b f_n exp(v_c - v_n)``                    (sum_layer.py:1569-1730, 2049-2169)
+* EM: ``theta' = (1 - s) theta + s (pf + pc / nch) / (sum pf + pc)`` (par_update.py:214-256,
+  categorical.py:164-201, bernoulli.py:91-101)
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional
+
+import numpy as np
+
+from pyjuice_amd.nodes.nodes import node_iterator
+
+__all__ = ["sum_dense_params", "forward", "backward", "em_update", "sample", "input_params"]
+
+
+def _np(t):
+    try:
+        return t.detach().cpu().numpy()
+    except AttributeError:
+        return np.asarray(t)
+
+
+def _src(ns):
+    return ns.get_source_ns() if ns.is_tied() else ns
+
+
+def sum_dense_params(ns, dtype=np.float64) -> np.ndarray:
+    """Dense ``[num_nodes, num_ch_nodes]`` weight matrix of a sum vector (zeros off the edges)."""
+    blocks = _np(_src(ns)._params).astype(dtype)  # [E, bs, cbs]
+    E = _np(ns.edge_ids)
+    bs, cbs = ns.block_size, ns.ch_block_size
+    W = np.zeros([ns.num_node_blocks, bs, ns.num_ch_node_blocks, cbs], dtype=dtype)
+    W[E[0], :, E[1], :] = blocks
+    return W.reshape(ns.num_nodes, ns.num_ch_nodes)
+
+
+def input_params(ns, dtype=np.float64) -> np.ndarray:
+    p = _np(_src(ns)._params).astype(dtype)
+    k = ns.dist.num_parameters()
+    return p.reshape(ns.num_nodes, k)
+
+
+def _prod_child_index(ns, k: int) -> np.ndarray:
+    """Row of child ``k`` feeding each node of product vector ``ns``."""
+    E = _np(ns.edge_ids)
+    if E.shape[0] == ns.num_node_blocks:
+        bs = ns.block_size
+        return (E[:, k][:, None] * bs + np.arange(bs)[None, :]).reshape(-1)
+    return E[:, k]
+
+
+def _input_ll(ns, data: np.ndarray) -> np.ndarray:
+    v = ns.scope.to_list()
+    assert len(v) == 1
+    x = data[:, v[0]].astype(np.int64)
+    th = input_params(ns)
+    sig = ns.dist.get_signature()
+    with np.errstate(divide="ignore"):
+        if sig == "Categorical":
+            K = th.shape[1]
+            ok = (x >= 0) & (x < K)
+            out = np.full([ns.num_nodes, x.shape[0]], -np.inf)
+            out[:, ok] = np.log(th[:, x[ok]])
+            return out
+        if sig == "Bernoulli":
+            p = th[:, 0][:, None]
+            return np.log(np.where(x[None, :] > 0, p, 1.0 - p))
+    raise NotImplementedError(sig)
+
+
+def _lse_matmul(W: np.ndarray, child: np.ndarray) -> np.ndarray:
+    m = child.max(axis=0, keepdims=True)
+    mm = np.where(np.isfinite(m), m, 0.0)
+    with np.errstate(divide="ignore"):
+        s = W @ np.exp(child - mm)
+        out = np.log(s) + mm
+    out[:, ~np.isfinite(m[0])] = -np.inf
+    return out
+
+
+def forward(root, data) -> Dict:
+    """Log-values of every vector: ``{ns: [num_nodes, B]}``; ``data`` is ``[B, num_vars]``."""
+    data = _np(data)
+    vals = dict()
+    for ns in node_iterator(root):
+        if ns.is_input():
+            vals[ns] = _input_ll(ns, data)
+        elif ns.is_prod():
+            acc = np.zeros([ns.num_nodes, data.shape[0]])
+            for k, cs in enumerate(ns.chs):
+                acc = acc + vals[cs][_prod_child_index(ns, k)]
+            vals[ns] = acc
+        else:
+            child = np.concatenate([vals[c] for c in ns.chs], axis=0)
+            vals[ns] = _lse_matmul(sum_dense_params(ns), child)
+    return vals
+
+
+def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None):
+    """Flows of every vector and parameter flows.
+
+    Returns ``(flows, pflows)``: ``flows[ns]`` is ``[num_nodes, B]``; ``pflows[ns]`` is the dense
+    ``[num_nodes, num_ch_nodes]`` matrix for a sum vector and ``[num_nodes, K]`` (Categorical) or
+    ``[num_nodes, 2]`` (Bernoulli; slot 0 for x = 1) for an input vector."""
+    data = _np(data)
+    B = data.shape[0]
+    flows = {root: np.ones([root.num_nodes, B]) if root_flows is None else np.asarray(root_flows, np.float64)
+             .reshape(root.num_nodes, B)}
+    pflows = dict()
+    for ns in node_iterator(root, reverse=True):
+        f = flows.get(ns)
+        if f is None:
+            f = np.zeros([ns.num_nodes, B])
+        if ns.is_sum():
+            W = sum_dense_params(ns)
+            child = np.concatenate([vals[c] for c in ns.chs], axis=0)
+            v = vals[ns]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                lr = np.where(np.isfinite(v) & (f > 0), np.log(np.maximum(f, 1e-300)) - v, -np.inf)
+            shift = lr.max(axis=0, keepdims=True)
+            sh = np.where(np.isfinite(shift), shift, 0.0)
+            R = np.exp(lr - sh)                                  # [n, B]
+            C = np.exp(np.clip(child + sh, -745.0, 709.0))       # [nch, B]
+            C[:, ~np.isfinite(shift[0])] = 0.0
+            C[~np.isfinite(child)] = 0.0
+            cf = C * (W.T @ R)
+            pflows[ns] = W * (R @ C.T)
+            s = 0
+            for c in ns.chs:
+                flows[c] = flows.get(c, np.zeros([c.num_nodes, B])) + cf[s:s + c.num_nodes]
+                s += c.num_nodes
+        elif ns.is_prod():
+            for k, cs in enumerate(ns.chs):
+                if cs not in flows:
+                    flows[cs] = np.zeros([cs.num_nodes, B])
+                np.add.at(flows[cs], _prod_child_index(ns, k), f)
+        else:
+            v = ns.scope.to_list()[0]
+            x = data[:, v].astype(np.int64)
+            if ns.dist.get_signature() == "Categorical":
+                K = ns.dist.num_cats
+                pf = np.zeros([ns.num_nodes, K])
+                ok = (x >= 0) & (x < K)
+                for k in range(K):
+                    pf[:, k] = f[:, ok & (x == k)].sum(axis=1)
+            else:
+                pf = np.stack([f[:, x > 0].sum(axis=1), f[:, x <= 0].sum(axis=1)], axis=1)
+            pflows[ns] = pf
+        flows[ns] = f
+    return flows, pflows
+
+
+def em_update(root, pflows: Dict, step_size: float, pseudocount: float = 0.0) -> Dict:
+    """New parameters of every source vector after one EM step: sum vectors as dense
+    ``[num_nodes, num_ch_nodes]`` (zeros off the edges), inputs as ``[num_nodes, K]``."""
+    tot = dict()
+    for ns, pf in pflows.items():
+        s = _src(ns)
+        tot[s] = tot.get(s, 0.0) + pf
+    out = dict()
+    for s, pf in tot.items():
+        if s.is_sum():
+            W = sum_dense_params(s)
+            mask = np.zeros_like(W, dtype=bool)
+            E = _np(s.edge_ids)
+            mb = np.zeros([s.num_node_blocks, s.block_size, s.num_ch_node_blocks, s.ch_block_size], dtype=bool)
+            mb[E[0], :, E[1], :] = True
+            mask = mb.reshape(W.shape)
+            nch = mask.sum(axis=1, keepdims=True).astype(np.float64)
+            pfm = np.where(mask, pf, 0.0)
+            new = (pfm + pseudocount / nch) / (pfm.sum(axis=1, keepdims=True) + pseudocount)
+            out[s] = np.where(mask, (1.0 - step_size) * W + step_size * new, 0.0)
+        elif s.dist.get_signature() == "Categorical":
+            th = input_params(s)
+            K = th.shape[1]
+            new = (pf + pseudocount / K) / (pf.sum(axis=1, keepdims=True) + pseudocount)
+            out[s] = (1.0 - step_size) * th + step_size * new
+        else:
+            th = input_params(s)[:, 0]
+            new = (pf[:, 0] + 0.5 * pseudocount) / (pf[:, 0] + pf[:, 1] + pseudocount)
+            out[s] = ((1.0 - step_size) * th + step_size * new)[:, None]
+    return out
+
+
+def sample(root, uniforms: np.ndarray, num_vars: int) -> np.ndarray:
+    """Ancestral samples ``[S, num_vars]`` with the uniform of node ``id`` (its compiled row
+    ``ns._output_ind_range[0] + i``) for sample ``s`` at ``uniforms[id, s]``.
+
+    A sum node's children are ordered as its edge list (flattened per child node), the running
+    sum of their weights is accumulated in FP32 left to right and the pick is the number of
+    running sums <= u (clamped to the last child); a Categorical node picks the first category
+    whose FP32 running sum reaches u (0 if none); Bernoulli gives ``u < theta``."""
+    U = np.asarray(uniforms, dtype=np.float32)
+    S = U.shape[1]
+    act = {root: np.ones([root.num_nodes, S], dtype=bool)}
+    out = np.zeros([S, num_vars], dtype=np.int64)
+    for ns in node_iterator(root, reverse=True):
+        a = act.get(ns)
+        if a is None or not a.any():
+            continue
+        off = ns._output_ind_range[0]
+        if ns.is_sum():
+            blocks = _np(_src(ns)._params).astype(np.float32)   # [E, bs, cbs]
+            E = _np(ns.edge_ids)
+            bs, cbs = ns.block_size, ns.ch_block_size
+            chosen = np.full([ns.num_nodes, S], -1, dtype=np.int64)  # concatenated child node id
+            for blk in range(ns.num_node_blocks):
+                es = np.nonzero(E[0] == blk)[0]
+                ch_nodes = (E[1][es][:, None] * cbs + np.arange(cbs)[None, :]).reshape(-1)
+                for i in range(bs):
+                    n = blk * bs + i
+                    cols = np.nonzero(a[n])[0]
+                    if cols.size == 0:
+                        continue
+                    w = blocks[es][:, i, :].reshape(-1)
+                    cum = np.cumsum(w, dtype=np.float32)
+                    pick = np.searchsorted(cum, U[off + n, cols], side="right")
+                    pick = np.minimum(pick, w.shape[0] - 1)
+                    chosen[n, cols] = ch_nodes[pick]
+            s0 = 0
+            for c in ns.chs:
+                sel = (chosen >= s0) & (chosen < s0 + c.num_nodes)
+                if sel.any():
+                    ca = act.setdefault(c, np.zeros([c.num_nodes, S], dtype=bool))
+                    nn_, ss = np.nonzero(sel)
+                    ca[chosen[nn_, ss] - s0, ss] = True
+                s0 += c.num_nodes
+        elif ns.is_prod():
+            for k, cs in enumerate(ns.chs):
+                ca = act.setdefault(cs, np.zeros([cs.num_nodes, S], dtype=bool))
+                idx = _prod_child_index(ns, k)
+                nn_, ss = np.nonzero(a)
+                ca[idx[nn_], ss] = True
+        else:
+            v = ns.scope.to_list()[0]
+            th = input_params(ns).astype(np.float32)
+            nn_, ss = np.nonzero(a)
+            u = U[off + nn_, ss]
+            if ns.dist.get_signature() == "Categorical":
+                cum = np.cumsum(th, axis=1, dtype=np.float32)
+                hit = cum[nn_] >= u[:, None]
+                pick = np.where(hit.any(axis=1), hit.argmax(axis=1), 0)
+            else:
+                pick = (u < th[nn_, 0]).astype(np.int64)
+            out[ss, v] = pick
+    return out

@@ -1,0 +1,196 @@
+// Buffer fills, data transposition, sum-parameter EM, tied-flow fusing, top-down sampling
+// of sum / product layers, and the library's error strings.
+#include "pjamd_common.h"
+
+using namespace pjamd;
+
+namespace {
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, float v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// [B, V] integer data of 1, 4 or 8 bytes -> [V, B] int32 through a 32 x 32 LDS tile.
+template <typename T>
+__global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const T* __restrict__ src, int64_t B, int64_t V) {
+    __shared__ int tile[32][33];
+    const int64_t v0 = (int64_t)blockIdx.x * 32, b0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t b = b0 + r, v = v0 + tx;
+        tile[r][tx] = (b < B && v < V) ? (int)src[b * V + v] : 0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t v = v0 + r, b = b0 + tx;
+        if (b < B && v < V) out[v * B + b] = tile[tx][r];
+    }
+}
+
+// Mini-batch EM of the sum parameters.  One wavefront handles (node block, up to 64 nodes);
+// record = [par_start, pflow_start, node_start, block_size, n_children] and the parameter of
+// child c of node i sits at par_start + c * block_size + i (same for flows).
+template <typename PF>
+__global__ __launch_bounds__(64) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
+                                                    const int* __restrict__ recs, int n_recs, double step, double pc,
+                                                    bool keep_zero) {
+    const int rec = blockIdx.x;
+    const int* R = recs + (int64_t)rec * 5;
+    const int64_t par0 = R[0], pf0 = R[1];
+    const int bs = R[3], nch = R[4];
+    const int ti = bs < 64 ? bs : 64;       // nodes handled by this wavefront
+    const int G = 64 / ti;                  // lanes cooperating on one node
+    const int lane = threadIdx.x;
+    const int i = blockIdx.y * 64 + (lane % ti);
+    const int g = lane / ti;
+    if (blockIdx.y * 64 >= bs) return;
+    const double npc = pc / (double)nch;
+    double cum = 0.0;
+    for (int c = g; c < nch; c += G) {
+        const int64_t off = (int64_t)c * bs + i;
+        const double f = (double)pflows[pf0 + off];
+        if (keep_zero) cum += (params[par0 + off] < 1e-12f) ? 0.0 : f + npc;
+        else cum += f;
+    }
+    for (int o = ti; o < 64; o <<= 1) cum += __shfl_xor(cum, o, 64);
+    const double denom = keep_zero ? cum : cum + pc;
+    for (int c = g; c < nch; c += G) {
+        const int64_t off = (int64_t)c * bs + i;
+        const double th = (double)params[par0 + off];
+        const double nv = ((double)pflows[pf0 + off] + npc) / denom;
+        double upd = (1.0 - step) * th + step * nv;
+        if (keep_zero && th < 1e-12) upd = 0.0;
+        params[par0 + off] = (float)upd;
+    }
+}
+
+// pf[target + j] = sum over copies c of pf[copy_c + j] (the copies include the target range).
+template <typename PF>
+__global__ __launch_bounds__(256) void fuse_kernel(PF* __restrict__ pf, int64_t target, int64_t n,
+                                                   const int64_t* __restrict__ copies, int ncopies) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    PF s = 0;
+    for (int c = 0; c < ncopies; ++c) s += pf[copies[c] + j];
+    pf[target + j] = s;
+}
+
+// Sum layer top-down step: an active node (nid + i, s) picks child #{k : u >= cumsum_k}.
+__global__ __launch_bounds__(256) void sample_sum_kernel(uint8_t* __restrict__ elem_act, const uint8_t* __restrict__ node_act,
+                                                         const float* __restrict__ uniforms,
+                                                         const float* __restrict__ params, const int* __restrict__ nids,
+                                                         const int* __restrict__ cids, const int* __restrict__ pids,
+                                                         const int* __restrict__ nch, int K, int bs, int64_t S) {
+    const int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int j = blockIdx.x / bs, i = blockIdx.x % bs;
+    const int64_t node = (int64_t)nids[j] + i;
+    if (!node_act[node * S + s]) return;
+    const float u = uniforms[node * S + s];
+    const int n = nch[j];
+    const int* crow = cids + (int64_t)j * K;
+    const int* prow = pids + (int64_t)j * K;
+    float cum = 0.0f;
+    int pick = n - 1;
+    for (int k = 0; k < n; ++k) {
+        cum += params[prow[k] + i];
+        if (u < cum) { pick = k; break; }
+    }
+    elem_act[(int64_t)crow[pick] * S + s] = 1;
+}
+
+// Product layer top-down step: an active element activates all its children.
+__global__ __launch_bounds__(256) void sample_prod_kernel(uint8_t* __restrict__ node_act, const uint8_t* __restrict__ elem_act,
+                                                          const int* __restrict__ nids, const int* __restrict__ cids,
+                                                          int K, int bs, int64_t S) {
+    const int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int j = blockIdx.x / bs, i = blockIdx.x % bs;
+    if (!elem_act[((int64_t)nids[j] + i) * S + s]) return;
+    const int* crow = cids + (int64_t)j * K;
+    for (int k = 0; k < K; ++k) {
+        const int c = crow[k];
+        if (c != 0) node_act[((int64_t)c + i) * S + s] = 1;
+    }
+}
+
+}  // namespace
+
+PJAMD_EXPORT int pjamd_abi_version() { return 1; }
+
+PJAMD_EXPORT const char* pjamd_error_string(int code) {
+    if (code == PJAMD_OK) return "success";
+    if (code == PJAMD_EINVAL) return "invalid argument (size, block size or unsupported combination)";
+    if (code == PJAMD_EUNSUPPORTED) return "unsupported configuration";
+    if (code > 0) return hipGetErrorString((hipError_t)code);
+    return "unknown error";
+}
+
+PJAMD_EXPORT int pjamd_fill_f32(float* p, float v, int64_t n, hipStream_t stream) {
+    PJAMD_REQUIRE(n >= 0);
+    if (n == 0) return PJAMD_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    fill_kernel<<<(unsigned)blocks, 256, 0, stream>>>(p, v, n);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_data_to_vb(int* out, int src_code, const void* src, int64_t B, int64_t V, hipStream_t stream) {
+    PJAMD_REQUIRE(B > 0 && V > 0);
+    dim3 grid(cdiv(V, 32), cdiv(B, 32));
+    switch (src_code) {
+        case 0: to_vb_kernel<uint8_t><<<grid, 256, 0, stream>>>(out, (const uint8_t*)src, B, V); break;
+        case 1: to_vb_kernel<int32_t><<<grid, 256, 0, stream>>>(out, (const int32_t*)src, B, V); break;
+        case 2: to_vb_kernel<int64_t><<<grid, 256, 0, stream>>>(out, (const int64_t*)src, B, V); break;
+        default: return PJAMD_EINVAL;
+    }
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sum_em(float* params, const void* pflows, int pf64, const int* recs, int n_recs, float step,
+                              float pseudocount, int keep_zero, int max_bs, hipStream_t stream) {
+    PJAMD_REQUIRE(n_recs >= 0 && max_bs >= 1);
+    if (n_recs == 0) return PJAMD_OK;
+    dim3 grid((unsigned)n_recs, cdiv(max_bs, 64));
+    if (pf64)
+        sum_em_kernel<double><<<grid, 64, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step, pseudocount,
+                                                       keep_zero != 0);
+    else
+        sum_em_kernel<float><<<grid, 64, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step, pseudocount,
+                                                      keep_zero != 0);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_pflow_fuse(void* pflows, int pf64, int64_t target, int64_t n, const int64_t* copies, int ncopies,
+                                  hipStream_t stream) {
+    PJAMD_REQUIRE(n >= 0 && ncopies >= 1);
+    if (n == 0) return PJAMD_OK;
+    if (pf64) fuse_kernel<double><<<cdiv(n, 256), 256, 0, stream>>>((double*)pflows, target, n, copies, ncopies);
+    else fuse_kernel<float><<<cdiv(n, 256), 256, 0, stream>>>((float*)pflows, target, n, copies, ncopies);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sample_sum(uint8_t* elem_act, const uint8_t* node_act, const float* uniforms, const float* params,
+                                  const int* nids, const int* cids, const int* pids, const int* nch, int n_blocks, int K,
+                                  int bs, int64_t S, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && S > 0);
+    if (n_blocks == 0) return PJAMD_OK;
+    dim3 grid((unsigned)((int64_t)n_blocks * bs), cdiv(S, 256));
+    sample_sum_kernel<<<grid, 256, 0, stream>>>(elem_act, node_act, uniforms, params, nids, cids, pids, nch, K, bs, S);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sample_prod(uint8_t* node_act, const uint8_t* elem_act, const int* nids, const int* cids,
+                                   int n_blocks, int K, int bs, int64_t S, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && S > 0);
+    if (n_blocks == 0) return PJAMD_OK;
+    dim3 grid((unsigned)((int64_t)n_blocks * bs), cdiv(S, 256));
+    sample_prod_kernel<<<grid, 256, 0, stream>>>(node_act, elem_act, nids, cids, K, bs, S);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}

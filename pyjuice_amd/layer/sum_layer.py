@@ -1,0 +1,186 @@
+"""Sum layer (``pyjuice/layer/sum_layer.py``): compile in ``__init__`` (``sum_layer.py:38-198``)
+and the log-sum-exp forward, flow backward and parameter-flow accumulation through the
+HIP library."""
+from __future__ import annotations
+
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from pyjuice_amd import _native as N
+from .compilation import (assign_partitions, get_sum_layer_backward_stats, get_sum_layer_forward_stats,
+                          sum_layer_backward_compilation, sum_layer_forward_compilation)
+from .layer import Layer
+from .partition import next_power_of_2, partition_nodes_by_n_edges
+
+__all__ = ["SumLayer", "PRECISIONS"]
+
+# Arithmetic of the intra-block contractions: 0 = FP32 FMA, 1 = BF16 MFMA, 2 = FP16 MFMA
+# (FP32 accumulation and per-tile max subtraction in every case).
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
+    return torch.unique(torch.from_numpy(next_power_of_2(x)))
+
+
+def _i32(t: torch.Tensor) -> torch.Tensor:
+    if t.numel() > 0:
+        assert int(t.max()) < 2 ** 31, "index exceeds the int32 range of the HIP kernels"
+    return t.to(torch.int32).contiguous()
+
+
+class SumLayer(Layer, nn.Module):
+    BLOCK_SPARSE = 0
+    SPARSE = 1
+    PYTORCH = 2
+    STR2MODE = {"block_sparse": 0, "sparse": 1, "pytorch": 2}
+
+    def __init__(self, nodes: Sequence, global_nid_start: int, global_pid_start: int, global_pfid_start: int,
+                 node2tiednodes: dict, layer_sparsity_tol: Optional[float] = None,
+                 max_num_partitions: Optional[int] = None, max_tied_ns_per_parflow_block: int = 8,
+                 disable_gpu_compilation: bool = False, force_gpu_compilation: bool = False):
+        Layer.__init__(self, nodes)
+        nn.Module.__init__(self)
+        assert len(self.nodes) > 0, "No input node."
+        assert len(self.nodes) == len(set(self.nodes)), "Input node list contains duplicates."
+
+        nid0, pid0, pfid0 = global_nid_start, global_pid_start, global_pfid_start
+        nblk, nedges, n_chs = get_sum_layer_forward_stats(self.nodes, global_nid_start)
+        self.num_nodes = nblk * self.block_size
+        self.num_edges = nedges
+
+        fw_max = _unique_pow2(partition_nodes_by_n_edges(n_chs, sparsity_tolerance=layer_sparsity_tol,
+                                                         max_num_partitions=max_num_partitions))
+        self.num_fw_partitions = len(fw_max)
+        part, loc, cnt = assign_partitions(n_chs, fw_max, start_min=1)
+        nids, cids, pids, pfids, pid_end, pfid_end = sum_layer_forward_compilation(
+            self.nodes, fw_max, part, loc, cnt, n_chs, global_nid_start, global_pid_start, global_pfid_start,
+            node2tiednodes, max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block)
+        self.partitioned_nids, self.partitioned_cids = nids, cids
+        self.partitioned_pids, self.partitioned_pfids = pids, pfids
+        self._fw_n_chs = [n_chs[part == p] for p in range(self.num_fw_partitions)]
+        self._layer_nid_range = (nid0, nid0 + self.num_nodes)
+        self._layer_pid_range = (pid0, pid_end)
+        self._layer_pfid_range = (pfid0, pfid_end)
+
+        ch_gsize2cs, ch_gsize2num_nblocks, ch_gsize2n_pargs, cs2parns = get_sum_layer_backward_stats(self.nodes)
+        chids, parids, parpids, cs_bs, bk_npar = [], [], [], [], []
+        for g, n_pargs in ch_gsize2n_pargs.items():
+            bk_max = _unique_pow2(partition_nodes_by_n_edges(n_pargs, sparsity_tolerance=layer_sparsity_tol,
+                                                             max_num_partitions=max_num_partitions))
+            bpart, bloc, bcnt = assign_partitions(n_pargs, bk_max, start_min=1)
+            c, pa, pp = sum_layer_backward_compilation(list(ch_gsize2cs[g]), cs2parns, bpart, bloc, bcnt, bk_max)
+            chids.extend(c)
+            parids.extend(pa)
+            parpids.extend(pp)
+            cs_bs.extend([g] * len(bk_max))
+            for q in range(len(bk_max)):
+                sel = (bpart == q) & (n_pargs >= 1) & (n_pargs <= int(bk_max[q]))
+                bk_npar.append(n_pargs[sel])
+        self.partitioned_chids, self.partitioned_parids, self.partitioned_parpids = chids, parids, parpids
+        self.cs_block_sizes = cs_bs
+        self._bk_n_pars = bk_npar
+        self.num_bk_partitions = len(chids)
+        self._pf_shared = any(ns.is_tied() for ns in self.nodes)
+        self._dev = None
+
+    @property
+    def num_parameters(self):
+        return self._layer_pid_range[1] - self._layer_pid_range[0]
+
+    @property
+    def num_param_flows(self):
+        return self._layer_pfid_range[1] - self._layer_pfid_range[0]
+
+    @property
+    def nid_range(self):
+        return self._layer_nid_range
+
+    def is_sum(self):
+        return True
+
+    def __repr__(self):
+        return f"SumLayer(nid_range=({self._layer_nid_range[0]}, {self._layer_nid_range[1]}), num_nodes={self.num_nodes}, num_edges={self.num_edges})"
+
+    def to(self, device):
+        nn.Module.to(self, device)
+        device = torch.device(device)
+        for name in ("partitioned_nids", "partitioned_cids", "partitioned_pids", "partitioned_pfids",
+                     "partitioned_chids", "partitioned_parids", "partitioned_parpids"):
+            setattr(self, name, [t.to(device) for t in getattr(self, name)])
+        self.device = device
+        self._dev = None
+        if device.type == "cuda":
+            fw = []
+            for p in range(self.num_fw_partitions):
+                n = self.partitioned_nids[p]
+                if n.numel() == 0:
+                    continue
+                # Real children of block j occupy columns [0, n_chs_j); the rest is padding.
+                fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
+                           _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device))))
+            bk = []
+            for q in range(self.num_bk_partitions):
+                c = self.partitioned_chids[q]
+                if c.numel() == 0:
+                    continue
+                bk.append((_i32(c), _i32(self.partitioned_parids[q]), _i32(self.partitioned_parpids[q]),
+                           _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q])))
+            self._dev = {"fw": fw, "bk": bk}
+        return self
+
+    # ------------------------------------------------------------------ passes
+
+    def forward(self, node_mars: torch.Tensor, element_mars: torch.Tensor, params: torch.Tensor,
+                force_use_bf16: bool = False, force_use_fp32: bool = False, propagation_alg: str = "LL",
+                precision: Optional[str] = None, **kwargs):
+        """``node_mars[n] = log sum_c theta[n, c] exp(element_mars[c])`` for every node of the
+        layer (``sum_layer.py:220-262``), with per-tile max subtraction."""
+        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
+        prec = PRECISIONS[precision] if precision is not None else (1 if force_use_bf16 else 0)
+        if force_use_fp32:
+            prec = 0
+        B = node_mars.size(1)
+        for nids, cids, pids, pfids, nch in self._dev["fw"]:
+            N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
+                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), self.block_size, B, prec,
+                   N.stream())
+
+    def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
+                 element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,
+                 allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
+                 negate_pflows: bool = False, accumulate_ch_flows: bool = False, allow_neg_flows: bool = False,
+                 force_use_fp32: bool = False, precision: Optional[str] = None, **kwargs):
+        """Element flows and (optionally) parameter flows of the layer (``sum_layer.py:264-384``).
+
+        ``allow_modify_flows`` rewrites the layer's node flows in place as ``log(flow) - mar``
+        (``sum_layer.py:1381-1392``); the element / parameter-flow kernels read either form.
+        """
+        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
+        assert not logspace_flows and not allow_neg_flows, "unsupported flow mode for the HIP sum layers"
+        assert not (accumulate_ch_flows and allow_modify_flows)
+        prec = PRECISIONS[precision] if precision is not None else 0
+        if force_use_fp32:
+            prec = 0
+        B = node_flows.size(1)
+        s = N.stream()
+        if allow_modify_flows:
+            for nids, cids, pids, pfids, nch in self._dev["fw"]:
+                N.call("pjamd_sum_modify_flows", N.ptr(node_flows), N.ptr(node_mars), N.ptr(nids), nids.numel(),
+                       self.block_size, B, s)
+        mod = 1 if allow_modify_flows else 0
+        for chids, parids, parpids, npar, cbs in self._dev["bk"]:
+            N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
+                   N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar),
+                   chids.numel(), parids.size(1), cbs, self.block_size, B, mod | (2 if accumulate_ch_flows else 0),
+                   prec, s)
+        if param_flows is not None:
+            pf64 = 1 if param_flows.dtype == torch.float64 else 0
+            flags = mod | (2 if self._pf_shared else 0) | (4 if negate_pflows else 0) | (8 * pf64)
+            for nids, cids, pids, pfids, nch in self._dev["fw"]:
+                N.call("pjamd_sum_backward_par", N.ptr(param_flows), N.ptr(node_flows), N.ptr(node_mars),
+                       N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),
+                       N.ptr(nch), nids.numel(), cids.size(1), self.block_size, B, flags, prec, 0, s)

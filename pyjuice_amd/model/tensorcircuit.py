@@ -1,0 +1,582 @@
+"""Compiled probabilistic circuit (``pyjuice/model/tensorcircuit.py``).
+
+``TensorCircuit`` keeps the reference's module API (``forward``, ``backward``,
+``mini_batch_em``, ``init_param_flows``, ``update_parameters`` ...) and buffer layout
+(``node_mars`` / ``node_flows`` ``[num_nodes, B]``, ``element_mars`` / ``element_flows``
+``[num_elements, B]``, flat ``params``).  Every pass is a sequence of HIP kernel launches on
+the current stream; ``record_cudagraph=True`` captures the inner layers into a HIP graph.
+
+Precision: ``precision`` selects the arithmetic of the sum-layer contractions
+(``"fp32"``, ``"bf16"`` or ``"fp16"``; the latter two on MFMA with FP32 accumulation).
+Parameter flows are accumulated across mini-batches in ``pflow_dtype`` (FP64 by default).
+"""
+from __future__ import annotations
+
+from functools import partial
+from typing import Callable, Optional, Sequence, Tuple, Union
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from pyjuice_amd import _native as N
+from pyjuice_amd.layer import InputLayer, LayerGroup, ProdLayer, SumLayer
+from pyjuice_amd.nodes import CircuitNodes, multiply, summate
+from pyjuice_amd.nodes.nodes import dfs_postorder
+from pyjuice_amd.utils import BitSet
+from .backend import compile_cum_par_flows_fn, compile_par_update_fn, normalize_parameters
+
+__all__ = ["TensorCircuit", "compile"]
+
+_DTYPE_CODE = {torch.uint8: 0, torch.bool: 0, torch.int8: 0, torch.int32: 1, torch.int64: 2}
+
+
+def _pc_model_backward_hook(grad, pc, inputs, record_cudagraph, apply_cudagraph, propagation_alg, **kwargs):
+    """Autograd hook of ``lls``: runs the flow backward with ``ll_weights = grad / sum(grad) * B``
+    (``tensorcircuit.py:24-37``)."""
+    grad = grad.permute(1, 0)
+    pc.backward(inputs=inputs, ll_weights=grad / grad.sum() * grad.size(1),
+                compute_param_flows=pc._optim_hyperparams["compute_param_flows"],
+                flows_memory=pc._optim_hyperparams["flows_memory"], record_cudagraph=record_cudagraph,
+                apply_cudagraph=apply_cudagraph, propagation_alg=propagation_alg, **kwargs)
+    return None
+
+
+def layer_iterator(pc, reverse=False, ret_layer_groups=False, ignore_input_layers=False):
+    groups = list(pc.inner_layer_groups)
+    if reverse:
+        groups = groups[::-1]
+    seq = []
+    if not reverse and not ignore_input_layers:
+        seq.append(pc.input_layer_group)
+    seq.extend(groups)
+    if reverse and not ignore_input_layers:
+        seq.append(pc.input_layer_group)
+    for g in seq:
+        if ret_layer_groups:
+            yield g
+        else:
+            for layer in g:
+                yield layer
+
+
+class TensorCircuit(nn.Module):
+    def __init__(self, root_ns: CircuitNodes, layer_sparsity_tol: float = 0.5,
+                 max_num_partitions: Optional[int] = None, disable_gpu_compilation: bool = False,
+                 force_gpu_compilation: bool = False, max_tied_ns_per_parflow_block: int = 8,
+                 device: Optional[Union[int, torch.device]] = None, verbose: bool = True,
+                 precision: str = "fp32", pflow_dtype: torch.dtype = torch.float64):
+        super().__init__()
+        assert isinstance(root_ns, CircuitNodes), "`root_ns` should be an instance of `CircuitNodes`."
+        assert precision in ("fp32", "bf16", "fp16")
+        self.root_ns = root_ns
+        self.device = torch.device("cpu")
+        self.num_vars = self._get_num_vars(root_ns)
+        self.precision = precision
+        self.pflow_dtype = pflow_dtype
+
+        self.node_mars = None
+        self.element_mars = None
+        self.node_flows = None
+        self.element_flows = None
+        self.param_flows = None
+
+        self._init_layers(layer_sparsity_tol=layer_sparsity_tol, max_num_partitions=max_num_partitions,
+                          max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block, verbose=verbose)
+
+        self._optim_hyperparams = {"compute_param_flows": True, "flows_memory": 1.0}
+        self._recorded_cuda_graphs = dict()
+        self.default_propagation_alg = "LL"
+        self.propagation_alg_kwargs = dict()
+        self._run_params = dict()
+        self._cum_flow = 0.0
+        self._em_dev = None
+
+    # ------------------------------------------------------------------ device placement
+
+    def to(self, device):
+        if isinstance(device, int):
+            device = torch.device(f"cuda:{device}")
+        device = torch.device(device)
+        if device.type == "cuda" and device.index is None:
+            device = torch.device(f"cuda:{torch.cuda.current_device()}")
+        super().to(device)
+        self.input_layer_group.to(device)
+        for g in self.inner_layer_groups:
+            g.to(device)
+        self.device = device
+        self.node_mars = self.element_mars = self.node_flows = self.element_flows = None
+        self.param_flows = None
+        self._recorded_cuda_graphs = dict()
+        if device.type == "cuda":
+            N.lib()  # fail loudly now if the HIP library is missing
+            meta = self.par_update_kwargs[7]
+            blocks = meta["node_blocks"]
+            assert blocks.numel() == 0 or int(blocks[:, :2].max()) + int((blocks[:, 3] * blocks[:, 4]).max()) < 2 ** 31
+            self._em_dev = {
+                "blocks": blocks.to(torch.int32).to(device).contiguous(),
+                "max_bs": int(blocks[:, 3].max()) if blocks.numel() > 0 else 1,
+                "fuse": [(int(t), int(n), torch.tensor(ch, dtype=torch.int64, device=device))
+                         for t, n, ch in self.parflow_fusing_kwargs],
+            }
+        return self
+
+    def cuda(self, device=None):
+        return self.to(torch.device("cuda" if device is None else f"cuda:{device}" if isinstance(device, int) else device))
+
+    # ------------------------------------------------------------------ buffers
+
+    def _ensure_buffers(self, B: int, need_flows: bool = False):
+        dev = self.device
+        assert dev.type == "cuda", "TensorCircuit passes run on the HIP device; call `pc.to('cuda')` first"
+        if self.node_mars is None or self.node_mars.size(1) != B:
+            self.node_mars = torch.empty([self.num_nodes, B], device=dev, dtype=torch.float32)
+            self.element_mars = torch.empty([self.num_elements, B], device=dev, dtype=torch.float32)
+            self._fill(self.node_mars[:self.num_dummy_nodes], 0.0)
+            self._fill(self.element_mars[:self.num_dummy_eles], float("-inf"))
+            self.node_flows = self.element_flows = None
+        if need_flows and (self.node_flows is None or self.node_flows.size(1) != B):
+            self.node_flows = torch.empty([self.num_nodes, B], device=dev, dtype=torch.float32)
+            self.element_flows = torch.empty([self.num_elements, B], device=dev, dtype=torch.float32)
+            self._fill(self.element_flows, 0.0)
+
+    @staticmethod
+    def _fill(t: torch.Tensor, value: float):
+        if t.numel() > 0:
+            N.call("pjamd_fill_f32", N.ptr(t), float(value), t.numel(), N.stream())
+
+    def _to_vb(self, inputs: torch.Tensor) -> torch.Tensor:
+        """``[B, V]`` integer / bool data -> contiguous ``[V, B]`` int32 on the device."""
+        assert inputs.dim() == 2 and inputs.size(1) == self.num_vars
+        x = inputs if inputs.device == self.device else inputs.to(self.device)
+        x = x.contiguous()
+        code = _DTYPE_CODE.get(x.dtype)
+        assert code is not None, f"unsupported data dtype {x.dtype}"
+        B, V = x.shape
+        out = torch.empty([V, B], device=self.device, dtype=torch.int32)
+        N.call("pjamd_data_to_vb", N.ptr(out), code, N.ptr(x), B, V, N.stream())
+        return out
+
+    # ------------------------------------------------------------------ passes
+
+    def set_propagation_alg(self, propagation_alg: str, **kwargs):
+        if propagation_alg != "LL":
+            raise NotImplementedError(f"Propagation algorithm {propagation_alg} is not implemented by the HIP layers.")
+        self.default_propagation_alg = "LL"
+
+    def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
+        for g in self.inner_layer_groups:
+            if g.is_prod():
+                g(self.node_mars, self.element_mars)
+            else:
+                g(self.node_mars, self.element_mars, self.params, force_use_bf16=force_use_bf16,
+                  force_use_fp32=force_use_fp32, precision=precision)
+
+    def forward(self, inputs: torch.Tensor, input_layer_fn: Optional[Union[str, Callable]] = None,
+                cache: Optional[dict] = None, return_cache: bool = False, record_cudagraph: bool = False,
+                apply_cudagraph: bool = True, force_use_bf16: bool = False, force_use_fp32: bool = False,
+                propagation_alg: Optional[Union[str, Sequence[str]]] = None, precision: Optional[str] = None,
+                **kwargs):
+        """Log-likelihoods ``[B, num_roots]`` of ``inputs`` ``[B, num_vars]`` (``tensorcircuit.py:207-342``)."""
+        if propagation_alg is None:
+            propagation_alg = self.default_propagation_alg
+        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented"
+        prec = precision if precision is not None else ("bf16" if force_use_bf16 else self.precision)
+        if force_use_fp32:
+            prec = "fp32"
+        origin_inputs = inputs
+        with torch.no_grad():
+            B = inputs.size(0)
+            self._ensure_buffers(B)
+            if input_layer_fn is None:
+                data = self._to_vb(inputs)
+                self._data_vb = data
+                for layer in self.input_layer_group:
+                    layer(data, self.node_mars, **kwargs)
+            elif isinstance(input_layer_fn, str):
+                for layer in self.input_layer_group:
+                    getattr(layer, input_layer_fn)(inputs, self.node_mars, **kwargs)
+            else:
+                for layer in self.input_layer_group:
+                    ret = input_layer_fn(layer, inputs, self.node_mars, **kwargs)
+                    if not ret and ret is not None:
+                        layer(self._to_vb(inputs), self.node_mars, **kwargs)
+
+            sig = (0, id(self.node_mars), id(self.element_mars), id(self.params), B, prec)
+            run = lambda: self._run_inner_forward(force_use_bf16, force_use_fp32, prec)
+            if record_cudagraph and sig not in self._recorded_cuda_graphs:
+                run()  # this call's result; the capture below does not execute kernels
+                self._recorded_cuda_graphs[sig] = self._capture(run)
+            elif apply_cudagraph and sig in self._recorded_cuda_graphs:
+                self._recorded_cuda_graphs[sig].replay()
+            else:
+                run()
+            lls = self.node_mars[self._root_node_range[0]:self._root_node_range[1], :].permute(1, 0)
+
+        if return_cache:
+            cache = dict() if cache is None else cache
+            cache["node_mars"] = self.node_mars.clone()
+        if torch.is_grad_enabled():
+            lls.requires_grad = True
+            lls.register_hook(partial(_pc_model_backward_hook, pc=self, inputs=origin_inputs,
+                                      record_cudagraph=record_cudagraph, apply_cudagraph=apply_cudagraph,
+                                      propagation_alg=propagation_alg, precision=prec, **kwargs))
+        out = lls.clone()
+        return (out, cache) if return_cache else out
+
+    def _capture(self, fn):
+        """Record ``fn``'s kernel launches into a HIP graph (nothing is executed)."""
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            fn()
+        torch.cuda.synchronize()
+        return g
+
+    def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision):
+        for lid in range(len(self.inner_layer_groups) - 1, -1, -1):
+            g = self.inner_layer_groups[lid]
+            if g.is_prod():
+                g.backward(self.node_flows, self.element_flows)
+            else:
+                self.inner_layer_groups[lid - 1].forward(self.node_mars, self.element_mars, _for_backward=True)
+                g.backward(self.node_flows, self.element_flows, self.node_mars, self.element_mars, self.params,
+                           param_flows=self.param_flows if compute_param_flows else None,
+                           allow_modify_flows=allow_modify_flows, precision=precision)
+
+    def backward(self, inputs: Optional[torch.Tensor] = None, ll_weights: Optional[torch.Tensor] = None,
+                 compute_param_flows: bool = True, flows_memory: float = 1.0,
+                 input_layer_fn: Optional[Union[str, Callable]] = None, cache: Optional[dict] = None,
+                 sum_layer_pre_backward_callback: Optional[Callable] = None,
+                 sum_layer_post_backward_callback: Optional[Callable] = None, return_cache: bool = False,
+                 record_cudagraph: bool = False, apply_cudagraph: bool = True, allow_modify_flows: bool = True,
+                 propagation_alg: Union[str, Sequence[str]] = "LL", logspace_flows: bool = False,
+                 negate_pflows: bool = False, precision: Optional[str] = None, force_use_fp32: bool = False,
+                 **kwargs):
+        """Node flows and accumulated parameter flows (``tensorcircuit.py:344-538``)."""
+        assert propagation_alg == "LL" and not logspace_flows and not negate_pflows, \
+            "only LL flows in linear space are implemented"
+        assert sum_layer_pre_backward_callback is None and sum_layer_post_backward_callback is None, \
+            "sum-layer callbacks are not supported"
+        assert self.node_mars is not None and self.element_mars is not None, "Should run forward path first."
+        prec = precision if precision is not None else self.precision
+        if force_use_fp32:
+            prec = "fp32"
+        self._run_params.update(allow_modify_flows=allow_modify_flows, propagation_alg=propagation_alg,
+                                logspace_flows=logspace_flows, negate_pflows=negate_pflows, precision=prec)
+        with torch.no_grad():
+            B = self.node_mars.size(1)
+            self._ensure_buffers(B, need_flows=True)
+            self._fill(self.node_flows, 0.0)
+            r0, r1 = self._root_node_range
+            if ll_weights is None:
+                self._fill(self.node_flows[r0:r1], 1.0)
+            else:
+                if ll_weights.dim() == 1:
+                    ll_weights = ll_weights.unsqueeze(0)
+                assert ll_weights.size(0) == self.num_root_nodes
+                self.node_flows[r0:r1, :] = ll_weights.to(self.node_flows.dtype)
+            if compute_param_flows:
+                self._cum_flow += (r1 - r0) * B if ll_weights is None else float(ll_weights.sum().item())
+                self.init_param_flows(flows_memory=flows_memory)
+
+            sig = (1, id(self.node_flows), id(self.element_flows), id(self.node_mars), id(self.element_mars),
+                   id(self.params), id(self.param_flows), B, allow_modify_flows, compute_param_flows, prec)
+            run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec)
+            if record_cudagraph and sig not in self._recorded_cuda_graphs:
+                run()  # this call's result; the capture below does not execute kernels
+                self._recorded_cuda_graphs[sig] = self._capture(run)
+            elif apply_cudagraph and sig in self._recorded_cuda_graphs:
+                self._recorded_cuda_graphs[sig].replay()
+            else:
+                run()
+
+            if input_layer_fn is None:
+                data = self._data_vb if inputs is None else self._to_vb(inputs)
+                for layer in self.input_layer_group:
+                    if compute_param_flows:
+                        layer.backward(data, self.node_flows, self.node_mars, **kwargs)
+            elif isinstance(input_layer_fn, str):
+                for layer in self.input_layer_group:
+                    getattr(layer, input_layer_fn)(inputs, self.node_flows, self.node_mars, **kwargs)
+            else:
+                for layer in self.input_layer_group:
+                    ret = input_layer_fn(layer, inputs, self.node_flows, self.node_mars, **kwargs)
+                    if not ret and ret is not None:
+                        layer.backward(self._to_vb(inputs), self.node_flows, self.node_mars, **kwargs)
+
+        if return_cache:
+            cache = dict() if cache is None else cache
+            cache["node_flows"] = self.node_flows.clone()
+            return cache
+        return None
+
+    def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False,
+                      step_size_rescaling: bool = False, use_cudagraph: bool = True):
+        """EM update from the accumulated parameter flows (``tensorcircuit.py:549-590``):
+        input layers, then tied-flow fusing, then the sum parameters."""
+        assert not step_size_rescaling, "step-size rescaling (top-down probabilities) is not implemented"
+        assert 0.0 < step_size <= 1.0, "`step_size` should be between 0 and 1."
+        if self.param_flows is None:
+            self.init_param_flows(0.0)
+        with torch.no_grad():
+            for layer in self.input_layer_group:
+                layer.mini_batch_em(step_size=step_size, pseudocount=pseudocount, keep_zero_params=keep_zero_params)
+            pf64 = 1 if self.param_flows.dtype == torch.float64 else 0
+            s = N.stream()
+            for t, n, ch in self._em_dev["fuse"]:
+                N.call("pjamd_pflow_fuse", N.ptr(self.param_flows), pf64, t, n, N.ptr(ch), ch.numel(), s)
+            blocks = self._em_dev["blocks"]
+            if blocks.numel() > 0:
+                N.call("pjamd_sum_em", N.ptr(self.params), N.ptr(self.param_flows), pf64, N.ptr(blocks),
+                       blocks.size(0), float(step_size), float(pseudocount), 1 if keep_zero_params else 0,
+                       self._em_dev["max_bs"], s)
+
+    def cumulate_flows(self, inputs: torch.Tensor, params: Optional[torch.Tensor] = None):
+        with torch.no_grad():
+            self.forward(inputs)
+            self.backward(inputs=inputs, compute_param_flows=True, flows_memory=1.0)
+
+    def init_param_flows(self, flows_memory: float = 1.0, batch_size: Optional[int] = None):
+        """Scale (``flows_memory`` in [0, 1]) or allocate the parameter-flow accumulators."""
+        assert 0.0 <= flows_memory <= 1.0, "`flows_memory` should be in [0.0, 1.0]"
+        assert batch_size is None, "per-sample parameter flows are not supported"
+        dev = self.device
+        if self.param_flows is None or self.param_flows.device != dev or self.param_flows.dtype != self.pflow_dtype:
+            self.param_flows = torch.zeros([self.num_param_flows], device=dev, dtype=self.pflow_dtype)
+        elif flows_memory != 1.0:
+            if flows_memory == 0.0:
+                self.param_flows.zero_()
+            else:
+                self.param_flows.mul_(flows_memory)
+        for layer in self.input_layer_group:
+            layer.init_param_flows(flows_memory=flows_memory, dtype=self.pflow_dtype)
+        return None
+
+    def zero_param_flows(self):
+        self.init_param_flows(flows_memory=0.0)
+
+    def update_parameters(self, clone: bool = True):
+        params = self.params.detach().cpu()
+        for ns in self.root_ns:
+            if ns.is_sum() and not ns.is_tied():
+                ns.update_parameters(params, clone=clone)
+        for layer in self.input_layer_group:
+            layer.update_parameters()
+
+    def update_param_flows(self, clone: bool = True, origin_ns_only: bool = True):
+        pf = self.param_flows.detach().cpu()
+        for ns in self.root_ns:
+            if ns.is_sum() and not ns.is_tied():
+                ns.update_param_flows(pf, clone=clone, origin_ns_only=origin_ns_only)
+
+    def print_statistics(self):
+        print(f"> Number of nodes: {self.num_nodes}")
+        print(f"> Number of edges: {self.num_edges}")
+        print(f"> Number of sum parameters: {self.num_sum_params}")
+        print(f"> Number of input parameters: {sum(l.params.numel() for l in self.input_layer_group)}")
+
+    def get_node_mars(self, ns: CircuitNodes):
+        assert self.node_mars is not None
+        a, b = ns._output_ind_range
+        if ns.is_sum() or ns.is_input():
+            return self.node_mars[a:b, :].detach()
+        for g in self.inner_layer_groups:
+            for layer in g:
+                if layer.is_prod() and any(n is ns for n in layer.nodes):
+                    layer(self.node_mars, self.element_mars)
+                    return self.element_mars[a:b, :].detach()
+        raise ValueError("node not found")
+
+    def get_node_flows(self, ns: CircuitNodes, **kwargs):
+        assert self.node_flows is not None
+        a, b = ns._output_ind_range
+        if ns.is_sum() or ns.is_input():
+            return self.node_flows[a:b, :].detach()
+        for lid, g in enumerate(self.inner_layer_groups):
+            for layer in g:
+                if layer.is_prod() and any(n is ns for n in layer.nodes):
+                    g.forward(self.node_mars, self.element_mars, _for_backward=True)
+                    self.inner_layer_groups[lid + 1].backward(
+                        self.node_flows, self.element_flows, self.node_mars, self.element_mars, self.params,
+                        param_flows=None, allow_modify_flows=self._run_params.get("allow_modify_flows", False),
+                        precision=self._run_params.get("precision", self.precision))
+                    return self.element_flows[a:b, :].detach()
+        raise ValueError("node not found")
+
+    def layers(self, reverse: bool = False, ret_layer_groups: bool = False, ignore_input_layers: bool = False):
+        return layer_iterator(self, reverse=reverse, ret_layer_groups=ret_layer_groups,
+                              ignore_input_layers=ignore_input_layers)
+
+    # ------------------------------------------------------------------ compilation
+
+    def _get_num_vars(self, ns):
+        return max(ns.scope.to_list()) + 1 if len(ns.scope) > 0 else 0
+
+    def _init_layers(self, layer_sparsity_tol=None, max_num_partitions=None, max_tied_ns_per_parflow_block=8,
+                     verbose=False):
+        """Depth-layering, global id assignment and per-layer compilation (``tensorcircuit.py:920-1092``)."""
+        if hasattr(self, "input_layer_group") or hasattr(self, "inner_layer_groups"):
+            raise ValueError("Attempting to initialize a TensorCircuit for the second time. "
+                             "Please instead create a new TensorCircuit instance by calling `pc = TensorCircuit(root_ns)`.")
+        self.root_ns._clear_tensor_circuit_hooks()
+        pc_num_vars = len(self.root_ns.scope)
+        depth2nodes, num_layers, max_node_bs, max_ele_bs = self._create_node_layers()
+
+        self.input_layer_group = None
+        self.inner_layer_groups = []
+        self.num_dummy_nodes = max_ele_bs
+        self.num_dummy_eles = max_node_bs
+        self.num_dummy_params = max_node_bs * max_ele_bs
+        num_nodes = self.num_dummy_nodes
+        num_edges = 0
+        num_elements = self.num_dummy_eles
+        num_params = self.num_dummy_params
+        num_pflows = 0
+        node2tiednodes = dict()
+
+        layer_id = 0
+        for depth in range(num_layers):
+            if depth == 0:
+                sig2nodes = dict()
+                for ns in depth2nodes[0]["input"]:
+                    sig2nodes.setdefault(ns.dist.get_signature(), []).append(ns)
+                layers = []
+                for sig, nodes in sig2nodes.items():
+                    layer = InputLayer(nodes=nodes, cum_nodes=num_nodes,
+                                       max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block,
+                                       pc_num_vars=pc_num_vars)
+                    layers.append(layer)
+                    num_nodes += layer.num_nodes
+                self.input_layer_group = LayerGroup(layers)
+                continue
+
+            assert len(depth2nodes[depth]["prod"]) > 0 and len(depth2nodes[depth]["sum"]) > 0, \
+                f"Depth {depth}: (# prod nodes: {len(depth2nodes[depth]['prod'])}, # sum nodes: {len(depth2nodes[depth]['sum'])})"
+            g2p = dict()
+            for ns in depth2nodes[depth]["prod"]:
+                g2p.setdefault(ns.block_size, []).append(ns)
+            layer_num_elements = max_node_bs
+            players = []
+            for gsize, nodes in g2p.items():
+                pl = ProdLayer(nodes=nodes, global_nid_start=layer_num_elements, layer_sparsity_tol=layer_sparsity_tol,
+                               max_num_partitions=max_num_partitions)
+                layer_num_elements += pl.num_nodes
+                num_edges += pl.num_edges
+                players.append(pl)
+            pg = LayerGroup(players)
+            self.inner_layer_groups.append(pg)
+            self.add_module(f"prod_layer_{layer_id}", pg)
+            num_elements = max(num_elements, layer_num_elements)
+
+            g2s = dict()
+            for ns in depth2nodes[depth]["sum"]:
+                g2s.setdefault(ns.block_size, []).append(ns)
+            slayers = []
+            for gsize, nodes in g2s.items():
+                sl = SumLayer(nodes=nodes, global_nid_start=num_nodes, global_pid_start=num_params,
+                              global_pfid_start=num_pflows, node2tiednodes=node2tiednodes,
+                              layer_sparsity_tol=layer_sparsity_tol, max_num_partitions=max_num_partitions,
+                              max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block)
+                num_nodes += sl.num_nodes
+                num_edges += sl.num_edges
+                num_params += sl.num_parameters
+                num_pflows += sl.num_param_flows
+                slayers.append(sl)
+            sg = LayerGroup(slayers)
+            self.inner_layer_groups.append(sg)
+            self.add_module(f"sum_layer_{layer_id}", sg)
+            layer_id += 1
+
+        self.num_nodes = num_nodes
+        self.num_edges = num_edges
+        self.num_elements = num_elements
+        self.num_sum_params = num_params
+        self.num_param_flows = num_pflows
+        self.node2tiednodes = node2tiednodes
+        self.parflow_fusing_kwargs = compile_cum_par_flows_fn(node2tiednodes, MAX_NBLOCKS=2048, BLOCK_SIZE=2048)
+        self.par_update_kwargs = compile_par_update_fn(self.root_ns, BLOCK_SIZE=32)
+        self.num_root_nodes = self.root_ns.num_nodes
+        self._root_node_range = (self.num_nodes - self.num_root_nodes, self.num_nodes)
+        self._init_parameters()
+
+    def _init_parameters(self, perturbation: float = 4.0, pseudocount: float = 0.0):
+        """Random / copied initial parameters (``tensorcircuit.py:1094-1116``): same torch CPU RNG
+        draws, in the same order, as the reference."""
+        for ns in self.root_ns:
+            if not ns.is_tied() and (ns.is_sum() or ns.is_input()) and not ns.has_params():
+                ns.init_parameters(perturbation=perturbation, recursive=False)
+        params = torch.exp(torch.rand([self.num_sum_params]) * -perturbation)
+        params[:self.num_dummy_params] = 0.0
+        for ns in self.root_ns:
+            if ns.is_sum() and not ns.is_tied() and ns.has_params():
+                ns.gather_parameters(params)
+        normalize_parameters(params, self.par_update_kwargs, pseudocount=pseudocount)
+        self.params = nn.Parameter(params, requires_grad=False)
+        for layer in self.input_layer_group:
+            layer._init_parameters(perturbation)
+
+    def _create_node_layers(self):
+        """Depth assignment with pass-through insertion (``tensorcircuit.py:1122-1196``).
+
+        Inputs have depth 0, a product is one deeper than its deepest child, a sum sits at the
+        depth of its deepest child.  A sum child (product) that is shallower than its parent
+        gets an identity ``summate`` + ``multiply`` pair inserted, which mutates the DAG."""
+        depth2nodes = dict()
+        nodes2depth = dict()
+        stats = {"num_layers": 1, "max_node_bs": 0, "max_ele_bs": 0}
+
+        def finish(ns):
+            if ns.is_input():
+                nodes2depth[ns] = 0
+                depth2nodes.setdefault(0, {"input": []})["input"].append(ns)
+                return
+            depth = max(nodes2depth[c] for c in ns.chs) + (1 if ns.is_prod() else 0)
+            stats["num_layers"] = max(depth + 1, stats["num_layers"])
+            nodes2depth[ns] = depth
+            if depth not in depth2nodes:
+                depth2nodes[depth] = {"sum": [], "prod": []}
+            if ns.is_sum():
+                for idx, cs in enumerate(ns.chs):
+                    cd = nodes2depth[cs]
+                    if cd < depth:
+                        pass_sum = summate(cs, num_node_blocks=cs.num_node_blocks, block_size=cs.block_size,
+                                           edge_ids=torch.arange(0, cs.num_node_blocks)[None, :].repeat(2, 1),
+                                           params=torch.eye(cs.block_size)[None, :, :].repeat(cs.num_node_blocks, 1, 1))
+                        pass_prod = multiply(pass_sum)
+                        ns.chs[idx] = pass_prod
+                        depth2nodes[cd]["sum"].append(pass_sum)
+                        nodes2depth[pass_sum] = cd
+                        nodes2depth[pass_prod] = depth
+                depth2nodes[depth]["sum"].append(ns)
+                stats["max_node_bs"] = max(stats["max_node_bs"], ns.block_size)
+            elif ns.is_prod():
+                stats["max_ele_bs"] = max(stats["max_ele_bs"], ns.block_size)
+            else:
+                raise NotImplementedError(f"Unsupported node type {type(ns)}.")
+
+        dfs_postorder(self.root_ns, finish)
+
+        pns2layer = dict()
+        for layer in range(1, len(depth2nodes)):
+            for ns in depth2nodes[layer]["sum"]:
+                for cs in ns.chs:
+                    if cs.is_prod():
+                        if cs in pns2layer:
+                            assert pns2layer[cs] == layer, \
+                                "Disallowed circumstance: a product node requested by sum nodes at different layers."
+                        else:
+                            depth2nodes[layer]["prod"].append(cs)
+                            pns2layer[cs] = layer
+        return depth2nodes, stats["num_layers"], stats["max_node_bs"], stats["max_ele_bs"]
+
+
+def compile(ns: CircuitNodes, layer_sparsity_tol: float = 0.5, max_num_partitions: Optional[int] = None,
+            disable_gpu_compilation: bool = False, force_gpu_compilation: bool = False,
+            max_tied_ns_per_parflow_block: int = 8, device: Optional[Union[int, torch.device]] = None,
+            verbose: bool = True, **kwargs) -> TensorCircuit:
+    """Compile a node DAG into a ``TensorCircuit`` (``tensorcircuit.py:1224-1261``)."""
+    return TensorCircuit(ns, layer_sparsity_tol=layer_sparsity_tol, max_num_partitions=max_num_partitions,
+                         disable_gpu_compilation=disable_gpu_compilation, force_gpu_compilation=force_gpu_compilation,
+                         max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block, device=device,
+                         verbose=verbose, **kwargs)

@@ -1,0 +1,256 @@
+"""Vectors of sum nodes (``pyjuice/nodes/sum_nodes.py``).
+
+``edge_ids`` is ``[2, E_blk]``: column ``(i, j)`` fully connects parent node block ``i``
+with child node block ``j`` (child blocks are numbered across all children in ``chs``
+order).  DAG-level parameters ``_params`` are ``[E_blk, block_size, ch_block_size]``.
+"""
+from __future__ import annotations
+
+from functools import reduce
+from typing import Optional, Sequence, Union
+
+import numpy as np
+import torch
+
+from pyjuice_amd.graph import InnerRegionNode
+from .nodes import CircuitNodes
+from .prod_nodes import ProdNodes
+
+__all__ = ["SumNodes", "normalize_ns_parameters"]
+
+
+def normalize_ns_parameters(params: torch.Tensor, node_ids: torch.Tensor, block_size: int, ch_block_size: int,
+                            node_nchs: Optional[torch.Tensor] = None, pseudocount: float = 0.0):
+    """In-place per-node normalisation of ``[E_blk, bs, cbs]`` parameters.
+
+    Semantics of the CPU branch of ``pyjuice/nodes/backend/normalize.py:67-129``: for every
+    node ``(n, i)``, ``theta <- theta * ((rowsum + pc/nch) / (sum over the node's edge
+    blocks of rowsum + pc)) / rowsum`` with ``rowsum`` the sum over the ``cbs`` child nodes
+    of one edge block (a zero rowsum is treated as 1e-12).
+    """
+    assert params.dim() == 3 and params.size(1) == block_size and params.size(2) == ch_block_size
+    node_ids = node_ids.long()
+    num_node_blocks = int(node_ids.max().item()) + 1
+    if node_nchs is None:
+        node_nchs = torch.bincount(node_ids, minlength=num_node_blocks) * ch_block_size
+    with torch.no_grad():
+        rowsum = params.sum(dim=2)  # [E, bs]
+        node_tot = torch.zeros([num_node_blocks, block_size], dtype=params.dtype)
+        node_tot.index_add_(0, node_ids, rowsum)
+        node_tot = node_tot + pseudocount
+        target = (rowsum + pseudocount / node_nchs[node_ids].to(params.dtype).unsqueeze(1)) / node_tot[node_ids]
+        denom = torch.where(rowsum == 0.0, torch.full_like(rowsum, 1e-12), rowsum)
+        params *= (target / denom).unsqueeze(2)
+    return params
+
+
+class SumNodes(CircuitNodes):
+    def __init__(self, num_node_blocks: int, chs: Sequence[CircuitNodes], edge_ids=None,
+                 params: Optional[torch.Tensor] = None, zero_param_mask: Optional[torch.Tensor] = None,
+                 block_size: int = 0, **kwargs):
+        assert len(chs) > 0, "`SumNodes` must have at least one child."
+        for i in range(1, len(chs)):
+            assert chs[0].block_size == chs[i].block_size, \
+                "`block_size` of the children of a `SumNodes` should be the same."
+        rg = InnerRegionNode([c.region_node for c in chs])
+        super().__init__(num_node_blocks, rg, block_size=block_size, **kwargs)
+
+        self.chs = self._standardize_chs(chs)
+        self.num_ch_node_blocks = reduce(lambda a, b: a + b, [c.num_node_blocks for c in chs])
+        self.ch_block_size = self.chs[0].block_size
+        self._construct_edges(edge_ids)
+
+        self._zero_param_mask = None
+        if zero_param_mask is not None:
+            self.set_zero_param_mask(zero_param_mask)
+        if params is not None:
+            self.set_params(params, pseudocount=0.0)
+        self._run_init_callbacks(**kwargs)
+
+    def is_sum(self):
+        return True
+
+    @property
+    def num_edges(self):
+        return self.edge_ids.size(1) * self.block_size * self.ch_block_size
+
+    @property
+    def num_ch_nodes(self):
+        return self.num_ch_node_blocks * self.ch_block_size
+
+    def duplicate(self, *args, tie_params: bool = False) -> "SumNodes":
+        chs = list(args)
+        if len(chs) == 0:
+            chs = self.chs
+        else:
+            assert self.num_chs == len(chs), \
+                f"Number of new children ({len(chs)}) must match the number of original children ({self.num_chs})."
+            for o, n in zip(self.chs, chs):
+                assert type(o) == type(n), f"Child type not match: ({type(n)} != {type(o)})."
+                assert o.num_node_blocks == n.num_node_blocks and o.block_size == n.block_size
+        params = None
+        if getattr(self, "_params", None) is not None and not tie_params:
+            params = self._params.clone()
+        return SumNodes(self.num_node_blocks, chs, self.edge_ids.clone(), params=params, block_size=self.block_size,
+                        source_node=self if tie_params else None)
+
+    def _dense_shape(self):
+        return (self.num_node_blocks, self.num_ch_node_blocks, self.block_size, self.ch_block_size)
+
+    def _to_matrix(self, blocks: torch.Tensor) -> torch.Tensor:
+        nb, ncb, bs, cbs = self._dense_shape()
+        dense = torch.zeros([nb * ncb, bs, cbs], dtype=torch.float32)
+        dense[self.edge_ids[0, :] * ncb + self.edge_ids[1, :]] = blocks.float()
+        return dense.reshape(nb, ncb, bs, cbs).permute(0, 2, 1, 3).reshape(nb * bs, ncb * cbs)
+
+    def get_params(self, as_matrix: bool = False):
+        if getattr(self, "_params", None) is None:
+            return None
+        return self._to_matrix(self._params) if as_matrix else self._params
+
+    def get_param_flows(self, as_matrix: bool = False):
+        if not self.provided("_param_flows"):
+            return None
+        return self._to_matrix(self._param_flows) if as_matrix else self._param_flows
+
+    def set_params(self, params: torch.Tensor, normalize: bool = True, pseudocount: float = 0.0):
+        if self._source_node is not None:
+            self._source_node.set_params(params, normalize=normalize, pseudocount=pseudocount)
+            return None
+        params = params.float()
+        if params.dim() == 1:
+            assert self.block_size == 1 and self.ch_block_size == 1
+            assert self.edge_ids.size(1) == params.size(0)
+            self._params = params.clone().view(-1, 1, 1)
+        elif params.dim() == 2:
+            nb = sum(c.num_node_blocks for c in self.chs)
+            assert params.size(0) == self.num_nodes and params.size(1) == self.ch_block_size * nb
+            full = params.reshape(self.num_node_blocks, self.block_size, nb, self.ch_block_size).permute(0, 2, 1, 3)
+            # Dense [n_blk, c_blk, bs, cbs]; keep the edge blocks (sum_nodes.py:300-304 keeps all
+            # blocks via flatten for dense edges; selecting by edge_ids is equivalent there).
+            self._params = full[self.edge_ids[0, :], self.edge_ids[1, :]].contiguous()
+        elif params.dim() == 3:
+            assert self.edge_ids.size(1) == params.size(0) and self.block_size == params.size(1) \
+                and self.ch_block_size == params.size(2)
+            self._params = params.clone()
+        elif params.dim() == 4:
+            assert params.size(0) == self.num_node_blocks and params.size(1) == self.num_ch_node_blocks \
+                and self.block_size == params.size(2) and self.ch_block_size == params.size(3)
+            self._params = params[self.edge_ids[0, :], self.edge_ids[1, :], :, :].clone().contiguous()
+        else:
+            raise ValueError("Unsupported parameter input.")
+        if self.provided("_zero_param_mask"):
+            self._params[self._zero_param_mask] = 0.0
+        if normalize:
+            normalize_ns_parameters(self._params, self.edge_ids[0, :], block_size=self.block_size,
+                                    ch_block_size=self.ch_block_size, pseudocount=pseudocount)
+
+    def set_zero_param_mask(self, zero_param_mask: Optional[torch.Tensor] = None):
+        if zero_param_mask is None:
+            return None
+        if self._source_node is not None:
+            self._source_node.set_zero_param_mask(zero_param_mask)
+            return None
+        assert zero_param_mask.dim() == 3 and zero_param_mask.size(0) == self.edge_ids.size(1)
+        assert zero_param_mask.size(1) == self.block_size and zero_param_mask.size(2) == self.ch_block_size
+        assert zero_param_mask.dtype == torch.bool
+        self._zero_param_mask = zero_param_mask
+
+    def get_zero_param_mask(self):
+        return self._zero_param_mask if self.provided("_zero_param_mask") else None
+
+    def set_edges(self, edge_ids):
+        self._construct_edges(edge_ids)
+        self._params = None
+
+    def _draw_own_params(self, perturbation: float, **kwargs):
+        # sum_nodes.py:251-262: exp(-perturbation * U) then per-node normalisation.
+        if self._source_node is None:
+            self._params = torch.exp(torch.rand([self.edge_ids.size(1), self.block_size, self.ch_block_size])
+                                     * -perturbation)
+            if self.provided("_zero_param_mask"):
+                self._params[self._zero_param_mask] = 0.0
+            normalize_ns_parameters(self._params, self.edge_ids[0, :], block_size=self.block_size,
+                                    ch_block_size=self.ch_block_size, pseudocount=0.0)
+
+    def update_parameters(self, params: torch.Tensor, clone: bool = True):
+        """Copy the compiled circuit's flat parameters back into ``_params`` (sum_nodes.py:271-296)."""
+        assert self.provided("_param_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        if self.is_tied():
+            return None
+        psid, peid = self._param_range
+        p = params[psid:peid].detach().cpu().float()
+        p = p.clone() if clone else p
+        nblk = self.block_size * self.ch_block_size
+        local = (self._param_ids - psid) // nblk
+        p = p.reshape(-1, self.ch_block_size, self.block_size)
+        self._params = p[local].permute(0, 2, 1).contiguous()
+
+    def update_param_flows(self, param_flows: torch.Tensor, origin_ns_only: bool = True, clone: bool = True):
+        assert self.provided("_param_flow_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        if origin_ns_only and self.is_tied():
+            return None
+        pfsid, pfeid = self._param_flow_range
+        pf = param_flows[pfsid:pfeid].detach().cpu().float()
+        pf = pf.clone() if clone else pf
+        nblk = self.block_size * self.ch_block_size
+        local = (self._param_flow_ids - pfsid) // nblk
+        pf = pf.reshape(-1, self.ch_block_size, self.block_size)
+        self._param_flows = pf[local].permute(0, 2, 1).contiguous()
+
+    def gather_parameters(self, params: torch.Tensor):
+        """Scatter ``_params`` into the compiled flat parameter vector (sum_nodes.py:333-347)."""
+        assert self.provided("_param_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        if self.is_tied() or not self.has_params():
+            return None
+        psid, peid = self._param_range
+        flat = self._params[self._inverse_param_ids].permute(0, 2, 1).reshape(-1)
+        params[psid:peid] = flat.to(params.device, params.dtype)
+
+    def _standardize_chs(self, chs):
+        out = []
+        for cs in chs:
+            if cs.is_input():
+                out.append(ProdNodes(num_node_blocks=cs.num_node_blocks, chs=[cs],
+                                     edge_ids=torch.arange(0, cs.num_node_blocks).reshape(-1, 1),
+                                     block_size=cs.block_size))
+            else:
+                out.append(cs)
+        return out
+
+    def _construct_edges(self, edge_ids, reorder: bool = False):
+        if edge_ids is None:
+            edge_ids = torch.stack((
+                torch.arange(self.num_node_blocks).unsqueeze(1).repeat(1, self.num_ch_node_blocks).reshape(-1),
+                torch.arange(self.num_ch_node_blocks).unsqueeze(0).repeat(self.num_node_blocks, 1).reshape(-1),
+            ), dim=0)
+        elif isinstance(edge_ids, (list, tuple)):
+            assert len(edge_ids) == len(self.chs)
+            parts, base = [], 0
+            for i, e in enumerate(edge_ids):
+                e = torch.as_tensor(e).clone().long()
+                e[1, :] += base
+                parts.append(e)
+                base += self.chs[i].num_node_blocks
+            edge_ids = torch.cat(parts, dim=1)
+        if isinstance(edge_ids, np.ndarray):
+            edge_ids = torch.from_numpy(edge_ids)
+        if edge_ids.dim() == 2 and edge_ids.dtype == torch.bool:
+            assert edge_ids.size(0) == self.num_node_blocks and edge_ids.size(1) == self.num_ch_node_blocks
+            x, y = torch.where(edge_ids)
+            edge_ids = torch.stack((x, y), dim=0)
+        edge_ids = edge_ids.long()
+        if reorder:
+            order = torch.argsort(edge_ids[0, :] * self.num_ch_node_blocks + edge_ids[1, :])
+            edge_ids = edge_ids[:, order].contiguous()
+        assert edge_ids.size(0) == 2, "Expect `edge_ids.size(0) == 2`."
+        assert torch.all(edge_ids >= 0), "Edge index underflow."
+        assert torch.all(edge_ids[0, :] < self.num_node_blocks) and \
+            torch.all(edge_ids[1, :] < self.num_ch_node_blocks), "Edge index overflow."
+        par = torch.unique(edge_ids[0, :])
+        assert par.size(0) == self.num_node_blocks and par.max() == self.num_node_blocks - 1, "Some node has no edge."
+        self.edge_ids = edge_ids
+
+    def __repr__(self):
+        return (f"SumNodes(num_node_blocks={self.num_node_blocks}, block_size={self.block_size}, "
+                f"num_chs={self.num_chs}, num_edges={self.num_edges}, scope_size={len(self.scope)})")

@@ -1,0 +1,140 @@
+"""HIP path against the FP64 oracle: forward, flow backward, parameter flows, EM, sampling.
+
+Tolerances (FP32 arithmetic on the device, FP64 oracle):
+* log-likelihood: |dLL| <= 1e-4 nats per variable per sample;
+* node marginals of every vector: |d| <= 1e-4 * (scope size) + 1e-5;
+* node / element flows and parameter flows: max error <= 1e-4 relative to the largest value;
+* EM parameters: |d| <= 2e-6 absolute;
+* samples: bit-identical given the same uniforms.
+"""
+import numpy as np
+import pytest
+import torch
+
+import kat_circuits
+import parity_utils as U
+from oracle import dag_eval
+from pyjuice_amd.model import TensorCircuit
+from pyjuice_amd.nodes.nodes import node_iterator
+
+pytestmark = pytest.mark.gpu
+
+
+def _circuit(name):
+    if name == "simple_model":
+        return kat_circuits.simple_model_dag(), [4, 4, 6, 6], dict(layer_sparsity_tol=0.1)
+    if name == "tying_block1":
+        return kat_circuits.tying_block1_dag(), [5, 5, 5, 5], dict(max_tied_ns_per_parflow_block=2)
+    if name == "hclt_bern16":
+        ns, x, cards = U.hclt_bernoulli()
+        return ns, cards, dict()
+    if name == "hclt_cat_bs32":
+        ns, x, cards = U.hclt_categorical(L=64, block_size=32)
+        return ns, cards, dict()
+    if name == "hclt_cat_bs64":
+        ns, x, cards = U.hclt_categorical(L=128, block_size=64, K=16)
+        return ns, cards, dict()
+    if name == "pd_small":
+        ns, cards = U.pd_small()
+        return ns, cards, dict()
+    if name == "hmm_small":
+        ns, cards = U.hmm_small()
+        return ns, cards, dict()
+    raise KeyError(name)
+
+
+CIRCUITS = ["simple_model", "tying_block1", "hclt_bern16", "hclt_cat_bs32", "hclt_cat_bs64", "pd_small", "hmm_small"]
+
+
+@pytest.fixture(scope="module")
+def device():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")
+
+
+@pytest.mark.parametrize("name", CIRCUITS)
+@pytest.mark.parametrize("B", [64, 13])
+def test_forward_backward_em(name, B, device):
+    U.say(f"[parity] {name} B={B}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.to(device)
+    X = U.random_data(cards, B, seed=11)
+    xt = torch.from_numpy(X).to(device)
+
+    lls = pc(xt).detach().cpu().numpy()[:, 0]
+    pc.update_parameters()
+    vals = dag_eval.forward(ns, X)
+    V = len(cards)
+    assert np.abs(lls - vals[ns][0]).max() <= 1e-4 * V, np.abs(lls - vals[ns][0]).max()
+    nm = pc.node_mars.detach().cpu().numpy()
+    for n in node_iterator(ns):
+        if n.is_sum() or n.is_input():
+            a, b = n._output_ind_range
+            tol = 1e-4 * len(n.scope) + 1e-5
+            d = np.abs(nm[a:b] - vals[n])
+            d[np.isinf(vals[n]) & np.isinf(nm[a:b])] = 0
+            assert d.max() <= tol, (n, d.max())
+
+    # flows without the in-place log transform, so node flows can be compared
+    pc.backward(xt, allow_modify_flows=False, flows_memory=0.0)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    nf = pc.node_flows.detach().cpu().numpy()
+    for n in node_iterator(ns):
+        if n.is_sum() or n.is_input():
+            a, b = n._output_ind_range
+            assert U.rel_err(nf[a:b], flows[n]) <= 1e-4, (n, U.rel_err(nf[a:b], flows[n]))
+    pc.update_param_flows(origin_ns_only=False)
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            ref = pflows[n]
+            # flows of the tied copies sharing this range are accumulated in it too
+            for m in node_iterator(ns):
+                if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
+                    ref = ref + pflows[m]
+            assert U.rel_err(got, ref) <= 1e-4, (n, U.rel_err(got, ref))
+    for layer in pc.input_layer_group:
+        pfi = layer.param_flows.detach().cpu().numpy().astype(np.float64)
+        for n in layer.nodes:
+            a, b = n._param_flow_range
+            K = n.dist.num_param_flows()
+            got = pfi[a:b].reshape(n.num_nodes, K)
+            ref = sum(pflows[m] for m in layer.nodes if m._param_flow_range == n._param_flow_range)
+            assert U.rel_err(got, ref) <= 1e-4, (n, U.rel_err(got, ref))
+
+    # default backward (in-place log(flow) - mar on sum nodes) must give the same parameter flows
+    pf_plain = pc.param_flows.detach().clone()
+    pc.backward(xt, flows_memory=0.0)
+    assert U.rel_err(pc.param_flows.detach().cpu().numpy(), pf_plain.cpu().numpy()) <= 1e-5
+
+    step, pcount = 0.5, 0.1
+    pc.mini_batch_em(step_size=step, pseudocount=pcount)
+    new = dag_eval.em_update(ns, pflows, step_size=step, pseudocount=pcount)
+    pc.update_parameters()
+    for s, p in new.items():
+        if s.is_sum():
+            got = U.dense_pf_from_blocks(s, s._params.numpy())
+        else:
+            got = s._params.numpy().astype(np.float64).reshape(p.shape)
+        assert np.abs(got - p).max() <= 2e-6, (s, np.abs(got - p).max())
+
+
+@pytest.mark.parametrize("name", ["simple_model", "hclt_bern16", "hclt_cat_bs32", "pd_small", "hmm_small"])
+def test_sampling_bit_exact(name, device):
+    U.say(f"[sample] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.to(device)
+    S = 200
+    u = torch.rand([pc.num_nodes, S], generator=torch.Generator().manual_seed(3))
+    smp = pyjuice_sample(pc, u.to(device)).cpu().numpy()
+    pc.update_parameters()
+    ref = dag_eval.sample(ns, u.numpy(), pc.num_vars)
+    assert np.array_equal(smp.astype(np.int64), ref), np.argwhere(smp != ref)[:5]
+
+
+def pyjuice_sample(pc, u):
+    import pyjuice_amd as juice
+    return juice.queries.sample(pc, num_samples=u.size(1), uniforms=u)
