@@ -1,0 +1,249 @@
+#!/usr/bin/env python
+"""Training-step throughput of a compiled HCLT on one MI355X.
+
+Workload (the headline configuration): HCLT over 784 categorical variables (28 x 28 grid,
+256 categories), ``num_latents = 256`` (block size 256), batch 512 of seeded synthetic
+uint8 images (Gaussian-copula, AR(1) rho = 0.6).  One step is the training loop of the
+reference's examples: ``zero_grad`` (clear parameter flows), ``lls = pc(x)``,
+``lls.mean().backward()`` (flow backward incl. parameter flows) and ``step`` (mini-batch EM,
+step 0.1, pseudocount 0.1).  Inputs and parameters are resident on the device.
+
+Prints one JSON line.  ``value`` is samples/s of the precision given by ``--precision``
+(default: the MFMA form when available, else FP32); the other precision's samples/s and the
+|dLL| per variable against the FP64 oracle are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
+               "pjamd_input_em", "pjamd_sum_em", "pjamd_sum_modify_flows", "pjamd_pflow_fuse")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--latents", type=int, default=256)
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "fp16"])
+    ap.add_argument("--graph", action="store_true", help="replay the inner layers from HIP graphs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-samples", type=int, default=8)
+    ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
+    return ap.parse_args()
+
+
+def build_circuit(args, dev):
+    import torch
+
+    import pyjuice_amd as juice
+    from pyjuice_amd import synthetic
+
+    t0 = time.time()
+    data = synthetic.image_categorical(args.fit + args.batch, H=28, W=28, K=256, rho=0.6, seed=0, field_seed=1)
+    log(f"[bench] data {data.shape} in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    torch.manual_seed(0)
+    x_fit = torch.from_numpy(data[:args.fit]).float().to(dev)
+    ns = juice.structures.HCLT(x_fit, num_latents=args.latents, input_dist=juice.distributions.Categorical(256),
+                               chunk_size=64, block_size=args.block_size or None)
+    log(f"[bench] HCLT structure in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    pc = juice.compile(ns, verbose=False)
+    pc.to(dev)
+    log(f"[bench] compiled: {pc.num_nodes} nodes, {pc.num_edges} edges, {pc.num_sum_params} sum params, "
+        f"{len(pc.inner_layer_groups)} layer groups in {time.time() - t0:.1f}s")
+    xb = torch.from_numpy(data[args.fit:args.fit + args.batch]).to(dev)
+    return ns, pc, xb, data[args.fit:args.fit + args.batch]
+
+
+def make_step(pc, xb, prec, graph):
+    import pyjuice_amd as juice
+
+    opt = juice.optim.CircuitOptimizer(pc, lr=0.1, pseudocount=0.1)
+
+    def step():
+        opt.zero_grad()
+        lls = pc(xb, precision=prec, record_cudagraph=graph, apply_cudagraph=graph)
+        lls.mean().backward()
+        opt.step()
+        return lls
+
+    return step
+
+
+def time_steps(step, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    b.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1000.0 / steps
+    return a.elapsed_time(b) / steps, wall
+
+
+def kernel_breakdown(step, steps):
+    from pyjuice_amd import _native as N
+
+    with N.LaunchTimer() as T:
+        for _ in range(steps):
+            step()
+    s = T.summary()
+    for d in s.values():
+        d["ms"] /= steps
+        d["bytes"] /= steps
+        d["flops"] /= steps
+        d["launches"] //= steps
+    return s
+
+
+def roofline_of(name, d, prec):
+    if name in HBM_KERNELS:
+        ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+    peak = PEAK_TFLOPS[prec]
+    ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
+    return {"kernel": name, "bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(ach / peak, 4), "traffic": None}
+
+
+def cpu_baseline(ns, pc, x_np, lls_by_prec, args):
+    """CPU leg: the FP64 oracle timed on a bounded sample of the batch (same circuit and
+    parameters), and the accuracy of every device precision against it."""
+    import torch
+
+    from oracle import dag_eval
+
+    n = min(args.cpu_samples, x_np.shape[0])
+    pc.update_parameters()
+    t0 = time.perf_counter()
+    vals = dag_eval.forward(ns, x_np[:n])
+    flows, pflows = dag_eval.backward(ns, x_np[:n], vals)
+    dag_eval.em_update(ns, pflows, step_size=0.1, pseudocount=0.1)
+    dt = time.perf_counter() - t0
+    ref = vals[ns][0]
+    V = x_np.shape[1]
+    acc = dict()
+    for prec, lls in lls_by_prec.items():
+        d = np.abs(lls[:n] - ref) / V
+        acc[prec] = {"mean": float(d.mean()), "max": float(d.max())}
+    return {"value": round(n / dt, 3), "unit": "samples/s", "cores": int(torch.get_num_threads()), "kind": "port",
+            "sample": f"{n} samples of the bench batch: FP64 numpy forward + flows + EM on the node DAG"}, acc
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the data-parallel design for "
+              f"several GPUs is described in DESIGN.md and is not run here.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from pyjuice_amd.layer.sum_layer import PRECISIONS
+
+    assert torch.cuda.is_available(), "bench.py needs the HIP device"
+    dev = torch.device("cuda:0")
+    ns, pc, xb, x_np = build_circuit(args, dev)
+
+    precs = ["fp32", "bf16"]
+    main_prec = "bf16" if args.precision == "auto" else args.precision
+    if args.only:
+        precs = [args.only]
+        main_prec = args.only
+    results = dict()
+    lls_by_prec = dict()
+    state = {k: v.detach().clone() for k, v in [("params", pc.params)]}
+    in_state = [l.params.detach().clone() for l in pc.input_layer_group]
+    for prec in precs:
+        # same starting parameters for every precision
+        pc.params.data.copy_(state["params"])
+        for l, p in zip(pc.input_layer_group, in_state):
+            l.params.data.copy_(p)
+        step = make_step(pc, xb, prec, args.graph)
+        try:
+            with torch.no_grad():
+                lls_by_prec[prec] = pc(xb, precision=prec).detach().cpu().numpy()[:, 0]
+            ms, wall = time_steps(step, args.steps, args.warmup)
+        except Exception as e:  # an unsupported precision is reported, not fatal
+            log(f"[bench] {prec}: {e}")
+            continue
+        results[prec] = {"ms_per_step": ms, "wall_ms_per_step": wall, "samples_per_s": args.batch / (ms * 1e-3)}
+        log(f"[bench] {prec}: {ms:.3f} ms/step ({args.batch / (ms * 1e-3):.0f} samples/s)")
+        results[prec]["kernels"] = kernel_breakdown(step, min(args.steps, 5))
+    if main_prec not in results:
+        main_prec = "fp32"
+    r = results[main_prec]
+    kern = r["kernels"]
+    dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
+    roof = roofline_of(dom[0], dom[1], main_prec)
+
+    cpu, acc = (None, None)
+    if not args.no_cpu_baseline and not args.only:
+        pc.params.data.copy_(state["params"])
+        for l, p in zip(pc.input_layer_group, in_state):
+            l.params.data.copy_(p)
+        cpu, acc = cpu_baseline(ns, pc, x_np, lls_by_prec, args)
+
+    out = {
+        "metric": "fwd+bwd+EM samples/s, BF16-MFMA vs FP32 HIP at |dLL/var|<=1e-3",
+        "value": round(r["samples_per_s"], 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16": "bf16", "fp16": "f16"}[main_prec],
+        "data": "synthetic",
+        "config": {"workload": f"HCLT-784 (28x28, 256 categories), num_latents={args.latents}, "
+                               f"block_size={args.block_size or 'default'}, batch={args.batch}; "
+                               f"forward + flow backward + mini-batch EM",
+                   "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
+                   "precision": main_prec, "hip_graph": bool(args.graph)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "precisions": {p: {"samples_per_s": round(v["samples_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 4)}
+                       for p, v in results.items()},
+        "dll_per_var_vs_fp64": acc,
+        "kernels_ms_per_step": {k: {"ms": round(v["ms"], 4), "launches": v["launches"],
+                                    "GB/s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
+                                    "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2)}
+                                for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
+    }
+    if "fp32" in results and "bf16" in results:
+        out["bf16_over_fp32"] = round(results["bf16"]["samples_per_s"] / results["fp32"]["samples_per_s"], 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,6 +44,7 @@ _SIGNATURES = {
     "pjamd_input_em": [_i32, _c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _i64, _f32, _f32, _i32, _c_p],
     "pjamd_input_sample": [_i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i64, _i64, _c_p],
     "pjamd_prod_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
+    "pjamd_prod_backward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_sum_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_sum_modify_flows": [_c_p, _c_p, _c_p, _i32, _i32, _i64, _c_p],
     "pjamd_sum_backward_ele": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
@@ -55,6 +56,7 @@ _SIGNATURES = {
     "pjamd_sample_sum": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_sample_prod": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_fill_f32": [_c_p, _f32, _i64, _c_p],
+    "pjamd_debug_mfma_probe": [_i32, _c_p, _c_p, _c_p, _c_p],
 }
 
 
@@ -106,5 +108,47 @@ def check(rc: int, name: str):
         raise NativeError(f"{name} failed with status {rc}: {msg.decode() if msg else '?'}")
 
 
-def call(name: str, *args):
+# Optional per-launch timing: while a ``LaunchTimer`` is active, every ``call`` made with an
+# ``algo=(bytes, flops)`` description is bracketed by HIP events on the current stream.
+_timer = None
+
+
+class LaunchTimer:
+    """Collects ``(name, start_event, end_event, algo_bytes, algo_flops)`` per launch."""
+
+    def __init__(self, names=None):
+        self.names = None if names is None else set(names)
+        self.records = []
+
+    def __enter__(self):
+        global _timer
+        _timer = self
+        return self
+
+    def __exit__(self, *exc):
+        global _timer
+        _timer = None
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = dict()
+        for name, a, b, nbytes, flops in self.records:
+            d = out.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            d["launches"] += 1
+            d["ms"] += a.elapsed_time(b)
+            d["bytes"] += nbytes
+            d["flops"] += flops
+        return out
+
+
+def call(name: str, *args, algo=None):
+    t = _timer
+    if t is not None and algo is not None and (t.names is None or name in t.names):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        check(getattr(lib(), name)(*args), name)
+        b.record()
+        t.records.append((name, a, b, float(algo[0]), float(algo[1])))
+        return
     check(getattr(lib(), name)(*args), name)

@@ -77,3 +77,8 @@ PJAMD_EXPORT int pjamd_prod_forward(float* out, const float* in, const int* nids
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
+
+PJAMD_EXPORT int pjamd_prod_backward(float* node_flows, const float* element_flows, const int* u_cids,
+                                     const int* parids, int n_blocks, int K, int bs, int64_t B, hipStream_t stream) {
+    return pjamd_prod_forward(node_flows, element_flows, u_cids, parids, n_blocks, K, bs, B, 1, stream);
+}

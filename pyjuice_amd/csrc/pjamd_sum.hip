@@ -453,17 +453,26 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__
 
 }  // namespace
 
-// precision: 0 = FP32 (this file); 1 = BF16 MFMA, 2 = FP16 MFMA (pjamd_sum_mfma.hip).
+// precision: 0 = FP32 operands on MFMA, 1 = BF16 MFMA, 2 = FP16 MFMA (pjamd_sum_mfma.hip, block sizes
+// >= 32); 3 = FP32 on the vector ALUs (this file).  Smaller blocks always use this file's kernels.
 int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const float* params, const int* nids,
                            const int* cids, const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B,
                            int precision, hipStream_t stream);
+int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,
+                                const float* params, const int* chids, const int* parids, const int* parpids,
+                                const int* npar, int n, int KP, int cbs, int bs, int64_t B, bool mod, bool accum,
+                                int precision, hipStream_t stream);
+int pjamd_sum_backward_par_mfma(void* pflows, bool pf64, const float* nflows, const float* nmars, const float* emars,
+                                const float* params, const int* nids, const int* cids, const int* pids, const int* pfids,
+                                const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, bool shared, float sign,
+                                int precision, hipStream_t stream);
 
 PJAMD_EXPORT int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
                                    const int* cids, const int* pids, const int* nch, int n_blocks, int K, int bs,
                                    int64_t B, int precision, hipStream_t stream) {
-    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 2);
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_blocks == 0) return PJAMD_OK;
-    if (precision != 0 && bs >= 16)
+    if (precision != 3 && bs >= 32)
         return pjamd_sum_forward_mfma(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B,
                                       precision, stream);
     const unsigned gy = cdiv(B, kTN);
@@ -499,9 +508,12 @@ PJAMD_EXPORT int pjamd_sum_backward_ele(float* element_flows, const float* node_
                                         const float* element_mars, const float* params, const int* chids,
                                         const int* parids, const int* parpids, const int* npar, int n_cblocks, int KP,
                                         int cbs, int bs, int64_t B, int flags, int precision, hipStream_t stream) {
-    PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && cbs >= 1 && bs >= 1 && B > 0);
+    PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && cbs >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_cblocks == 0) return PJAMD_OK;
     const bool mod = flags & 1, accum = (flags & 2) != 0;
+    if (precision != 3 && cbs >= 32)
+        return pjamd_sum_backward_ele_mfma(element_flows, node_flows, node_mars, element_mars, params, chids, parids,
+                                           parpids, npar, n_cblocks, KP, cbs, bs, B, mod, accum, precision, stream);
     const unsigned gy = cdiv(B, kTN);
     if (cbs >= 64) {
         sum_bwd_ele_tiled<64><<<dim3((unsigned)((int64_t)n_cblocks * (cbs / 64)), gy), kThreads, 0, stream>>>(
@@ -556,10 +568,14 @@ PJAMD_EXPORT int pjamd_sum_backward_par(void* param_flows, const float* node_flo
                                         const float* element_mars, const float* params, const int* nids, const int* cids,
                                         const int* pids, const int* pfids, const int* nch, int n_blocks, int K, int bs,
                                         int64_t B, int flags, int precision, int reserved, hipStream_t stream) {
-    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0);
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_blocks == 0) return PJAMD_OK;
     const bool mod = flags & 1, atomic_out = (flags & 2) != 0;
     const float sign = (flags & 4) ? -1.0f : 1.0f;
+    if (precision != 3 && bs >= 32)
+        return pjamd_sum_backward_par_mfma(param_flows, (flags & 8) != 0, node_flows, node_mars, element_mars, params,
+                                           nids, cids, pids, pfids, nch, n_blocks, K, bs, B, mod, atomic_out, sign,
+                                           precision, stream);
     if (flags & 8)
         return sum_bwd_par_impl<double>((double*)param_flows, node_flows, node_mars, element_mars, params, nids, cids,
                                         pids, pfids, nch, n_blocks, K, bs, B, mod, atomic_out, sign, stream);

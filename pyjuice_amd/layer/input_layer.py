@@ -275,7 +275,8 @@ class InputLayer(Layer, nn.Module):
         t = self._dev
         B = node_mars.size(1)
         N.call("pjamd_input_forward", self._kind(), N.ptr(p), N.ptr(node_mars), N.ptr(data), N.ptr(t["vids"]),
-               N.ptr(t["s_pids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0], B, N.stream())
+               N.ptr(t["s_pids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0], B, N.stream(),
+               algo=(4 * (self.num_nodes + len(self.layer_scope)) * B, 0))
 
     def backward(self, data: torch.Tensor, node_flows: torch.Tensor, node_mars: torch.Tensor,
                  params: Optional[Dict] = None, logspace_flows: bool = False, **kwargs):
@@ -289,7 +290,8 @@ class InputLayer(Layer, nn.Module):
         flags = (t["kmax"] << 1) | (1 if t["shared_pf"] else 0)
         N.call("pjamd_input_backward", self._kind(), N.ptr(self.param_flows), N.ptr(node_flows), pf64, N.ptr(data),
                N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0],
-               B, flags, N.stream())
+               B, flags, N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B
+                                            + 2 * self.param_flows.element_size() * self.num_param_flows, 0))
 
     def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False):
         """Tied-flow fusing then the closed-form EM update (``input_layer.py:663-724``)."""
@@ -304,7 +306,8 @@ class InputLayer(Layer, nn.Module):
                 N.call("pjamd_pflow_fuse", N.ptr(self.param_flows), pf64, start, n, N.ptr(ch), ch.numel(), N.stream())
             N.call("pjamd_input_em", self._kind(), N.ptr(self.params), N.ptr(self.param_flows), pf64, N.ptr(t["src"]),
                    N.ptr(t["s_pids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), t["src"].numel(),
-                   float(step_size), float(pseudocount), 1 if keep_zero_params else 0, N.stream())
+                   float(step_size), float(pseudocount), 1 if keep_zero_params else 0, N.stream(),
+                   algo=(8 * self.num_parameters + self.param_flows.element_size() * self.num_param_flows, 0))
 
     def sample(self, samples: torch.Tensor, node_active: torch.Tensor, uniforms: torch.Tensor, **kwargs):
         """Draw ``samples[v_i, s]`` for every active input node ``i`` with uniform ``u[off+i, s]``:

@@ -85,23 +85,27 @@ class ProdLayer(Layer, nn.Module):
         self._dev = None
         if device.type == "cuda":
             i32 = lambda t: t.to(torch.int32).contiguous()
+            # algorithmic FP32 words per sample: rows gathered + rows written (+ read for accumulation)
+            bs = self.block_size
             self._dev = {
-                "fw": [(i32(n), i32(c)) for n, c in zip(self.partitioned_nids, self.partitioned_cids) if n.numel() > 0],
-                "bk": [(i32(n), i32(c)) for n, c in zip(self.partitioned_u_cids, self.partitioned_parids) if n.numel() > 0],
+                "fw": [(i32(n), i32(c), int((c != 0).sum()) * bs + n.numel() * bs)
+                       for n, c in zip(self.partitioned_nids, self.partitioned_cids) if n.numel() > 0],
+                "bk": [(i32(n), i32(c), int((c != 0).sum()) * bs + 2 * n.numel() * bs)
+                       for n, c in zip(self.partitioned_u_cids, self.partitioned_parids) if n.numel() > 0],
             }
         return self
 
     def forward(self, node_mars: torch.Tensor, element_mars: torch.Tensor, _for_backward: bool = False, **kwargs):
         """``element_mars[nid + j] = sum_k node_mars[cid_k + j]`` (``prod_layer.py:160-204``)."""
         B = node_mars.size(1)
-        for nids, cids in self._dev["fw"]:
+        for nids, cids, words in self._dev["fw"]:
             N.call("pjamd_prod_forward", N.ptr(element_mars), N.ptr(node_mars), N.ptr(nids), N.ptr(cids),
-                   nids.numel(), cids.size(1), self.block_size, B, 0, N.stream())
+                   nids.numel(), cids.size(1), self.block_size, B, 0, N.stream(), algo=(4 * words * B, 0))
 
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, logspace_flows: bool = False, **kwargs):
         """``node_flows[u + j] += sum_p element_flows[par_p + j]`` (``prod_layer.py:206-237``)."""
         assert not logspace_flows, "logspace flows are not supported by the HIP product layers"
         B = node_flows.size(1)
-        for u, parids in self._dev["bk"]:
-            N.call("pjamd_prod_forward", N.ptr(node_flows), N.ptr(element_flows), N.ptr(u), N.ptr(parids),
-                   u.numel(), parids.size(1), self.block_size, B, 1, N.stream())
+        for u, parids, words in self._dev["bk"]:
+            N.call("pjamd_prod_backward", N.ptr(node_flows), N.ptr(element_flows), N.ptr(u), N.ptr(parids),
+                   u.numel(), parids.size(1), self.block_size, B, N.stream(), algo=(4 * words * B, 0))

@@ -17,9 +17,10 @@ from .partition import next_power_of_2, partition_nodes_by_n_edges
 
 __all__ = ["SumLayer", "PRECISIONS"]
 
-# Arithmetic of the intra-block contractions: 0 = FP32 FMA, 1 = BF16 MFMA, 2 = FP16 MFMA
-# (FP32 accumulation and per-tile max subtraction in every case).
-PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+# Arithmetic of the intra-block contractions (FP32 accumulation and max subtraction in every
+# case): FP32 / BF16 / FP16 operands on the MFMA matrix cores, or FP32 on the vector ALUs.
+# Blocks smaller than 32 always use the vector-ALU FP32 kernels.
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp32_valu": 3}
 
 
 def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
@@ -121,14 +122,16 @@ class SumLayer(Layer, nn.Module):
                     continue
                 # Real children of block j occupy columns [0, n_chs_j); the rest is padding.
                 fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
-                           _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device))))
+                           _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device)),
+                           int(self._fw_n_chs[p].sum())))
             bk = []
             for q in range(self.num_bk_partitions):
                 c = self.partitioned_chids[q]
                 if c.numel() == 0:
                     continue
                 bk.append((_i32(c), _i32(self.partitioned_parids[q]), _i32(self.partitioned_parpids[q]),
-                           _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q])))
+                           _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q]),
+                           int(self._bk_n_pars[q].sum())))
             self._dev = {"fw": fw, "bk": bk}
         return self
 
@@ -144,10 +147,14 @@ class SumLayer(Layer, nn.Module):
         if force_use_fp32:
             prec = 0
         B = node_mars.size(1)
-        for nids, cids, pids, pfids, nch in self._dev["fw"]:
+        bs = self.block_size
+        for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
+            # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
+            # per sample, parameters read once
+            algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
             N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
-                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), self.block_size, B, prec,
-                   N.stream())
+                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, prec,
+                   N.stream(), algo=algo)
 
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
                  element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,
@@ -167,20 +174,26 @@ class SumLayer(Layer, nn.Module):
             prec = 0
         B = node_flows.size(1)
         s = N.stream()
+        bs = self.block_size
         if allow_modify_flows:
-            for nids, cids, pids, pfids, nch in self._dev["fw"]:
+            for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
                 N.call("pjamd_sum_modify_flows", N.ptr(node_flows), N.ptr(node_mars), N.ptr(nids), nids.numel(),
-                       self.block_size, B, s)
+                       bs, B, s, algo=(12 * nids.numel() * bs * B, 0))
         mod = 1 if allow_modify_flows else 0
-        for chids, parids, parpids, npar, cbs in self._dev["bk"]:
+        for chids, parids, parpids, npar, cbs, totp in self._dev["bk"]:
+            algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
+                    2 * totp * bs * cbs * B)
             N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
                    N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar),
-                   chids.numel(), parids.size(1), cbs, self.block_size, B, mod | (2 if accumulate_ch_flows else 0),
-                   prec, s)
+                   chids.numel(), parids.size(1), cbs, bs, B, mod | (2 if accumulate_ch_flows else 0),
+                   prec, s, algo=algo)
         if param_flows is not None:
             pf64 = 1 if param_flows.dtype == torch.float64 else 0
             flags = mod | (2 if self._pf_shared else 0) | (4 if negate_pflows else 0) | (8 * pf64)
-            for nids, cids, pids, pfids, nch in self._dev["fw"]:
+            pfw = 8 if pf64 else 4
+            for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
+                algo = (4 * (nids.numel() * bs * (1 if mod else 2) + tot) * B + (4 + 2 * pfw) * tot * bs,
+                        2 * tot * bs * B)
                 N.call("pjamd_sum_backward_par", N.ptr(param_flows), N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),
-                       N.ptr(nch), nids.numel(), cids.size(1), self.block_size, B, flags, prec, 0, s)
+                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, flags, prec, 0, s, algo=algo)

@@ -68,7 +68,7 @@ class TensorCircuit(nn.Module):
                  precision: str = "fp32", pflow_dtype: torch.dtype = torch.float64):
         super().__init__()
         assert isinstance(root_ns, CircuitNodes), "`root_ns` should be an instance of `CircuitNodes`."
-        assert precision in ("fp32", "bf16", "fp16")
+        assert precision in ("fp32", "bf16", "fp16", "fp32_valu")
         self.root_ns = root_ns
         self.device = torch.device("cpu")
         self.num_vars = self._get_num_vars(root_ns)
@@ -328,9 +328,10 @@ class TensorCircuit(nn.Module):
                 N.call("pjamd_pflow_fuse", N.ptr(self.param_flows), pf64, t, n, N.ptr(ch), ch.numel(), s)
             blocks = self._em_dev["blocks"]
             if blocks.numel() > 0:
+                n = self.num_sum_params - self.num_dummy_params
                 N.call("pjamd_sum_em", N.ptr(self.params), N.ptr(self.param_flows), pf64, N.ptr(blocks),
                        blocks.size(0), float(step_size), float(pseudocount), 1 if keep_zero_params else 0,
-                       self._em_dev["max_bs"], s)
+                       self._em_dev["max_bs"], s, algo=(n * (8 + self.param_flows.element_size()), 0))
 
     def cumulate_flows(self, inputs: torch.Tensor, params: Optional[torch.Tensor] = None):
         with torch.no_grad():

@@ -47,13 +47,13 @@ def sample(pc, num_samples: Optional[int] = None, conditional: bool = False, uni
         if g.is_sum():
             elem_act.zero_()
             for layer in g:
-                for nids, cids, pids, pfids, nch in layer._dev["fw"]:
+                for nids, cids, pids, pfids, nch, _ in layer._dev["fw"]:
                     N.call("pjamd_sample_sum", N.ptr(elem_act), N.ptr(node_act), N.ptr(uniforms), N.ptr(pc.params),
                            N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1),
                            layer.block_size, S, s)
         else:
             for layer in g:
-                for nids, cids in layer._dev["fw"]:
+                for nids, cids, _ in layer._dev["fw"]:
                     N.call("pjamd_sample_prod", N.ptr(node_act), N.ptr(elem_act), N.ptr(nids), N.ptr(cids),
                            nids.numel(), cids.size(1), layer.block_size, S, s)
     dtype = pc.input_layer_group[0].get_data_dtype()
