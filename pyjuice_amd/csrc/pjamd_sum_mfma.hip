@@ -72,28 +72,12 @@ __device__ __forceinline__ float lfdm_at(const float* nflows, const float* nmars
     return (m == kNegInf) ? kNegInf : __logf(nflows[o]) - m;
 }
 
-// Column max of TN batch columns over `rows` rows given by a row-id functor.
-template <int TN, typename RowFn>
-__device__ __forceinline__ void column_max(float* cmax, float (*red)[TN], int rows, int64_t b0, int64_t B, RowFn rowval) {
-    constexpr int G = 256 / TN;
-    const int tid = threadIdx.x;
-    const int col = tid % TN, g = tid / TN;
-    const int64_t b = b0 + col;
-    float m = kNegInf;
-    if (b < B)
-        for (int k = g; k < rows; k += G) m = fmaxf(m, rowval(k, b));
-    red[g][col] = m;
-    __syncthreads();
-    if (tid < TN) {
-        float t = red[0][tid];
-#pragma unroll
-        for (int q = 1; q < G; ++q) t = fmaxf(t, red[q][tid]);
-        cmax[tid] = t;
-    }
-    __syncthreads();
-}
+constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per chunk
 
 // ---------------------------------------------------------------------------------------------
+// Forward.  Workgroup = (node block j, TM parent rows) x TN batch columns.  The child index rows
+// are staged in LDS; the next stage's operands are prefetched into registers while the matrix
+// cores work on the current one; the column max is a running max (accumulators are rescaled).
 template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ nids,
@@ -103,10 +87,12 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
     constexpr int PAD = 16 / sizeof(T);
     constexpr int LD = TK + PAD;
+    constexpr int G = 256 / TN, PB = TK / G;       // element tile: G row groups x TN columns
+    constexpr int PA = TK * TM / 256;              // parameter tile: PA consecutive k per thread
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ float cmax[TN];
-    __shared__ float red[256 / TN][TN];
+    __shared__ int sc[KCAP], sp[KCAP];
+    __shared__ float red[G][TN], mrun[TN], msc[TN];
 
     const int mtiles = bs / TM;
     const int j = blockIdx.x / mtiles, m0 = (blockIdx.x % mtiles) * TM;
@@ -116,38 +102,68 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     const int n = nch[j];
     const int* crow = cids + (int64_t)j * K;
     const int* prow = pids + (int64_t)j * K;
-
-    column_max<TN>(cmax, red, n, b0, B, [&](int k, int64_t b) { return emars[(int64_t)crow[k] * B + b]; });
-
-    constexpr int PB = TK * TN / 256;
-    const int bcol = tid % TN, bk0 = (tid / TN) * PB;
-    const float bm = cmax[bcol];
+    const int bcol = tid % TN, bg = tid / TN, bk0 = bg * PB;
     const int64_t bb = b0 + bcol;
-    constexpr int PA = TK * TM / 256;
+    const bool bok = bb < B;
     const int arow = tid % TM, ak0 = (tid / TM) * PA;
+    const int mycol = wn * 32 + (lane & 31);
 
+    if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
-    for (int k0 = 0; k0 < n; k0 += TK) {
-#pragma unroll
-        for (int q = 0; q < PB; ++q) {
-            const int k = k0 + bk0 + q;
-            float x = 0.0f;
-            if (k < n && bb < B && bm != kNegInf) x = __expf(emars[(int64_t)crow[k] * B + bb] - bm);
-            Bs[bcol][bk0 + q] = MF<T>::cvt(x);
-        }
-#pragma unroll
-        for (int q = 0; q < PA; ++q) {
-            const int k = k0 + ak0 + q;
-            As[arow][ak0 + q] = MF<T>::cvt(k < n ? params[(int64_t)prow[k] + m0 + arow] : 0.0f);
+    float ev[PB], av[PA];
+    for (int cb = 0; cb < n; cb += KCAP) {
+        const int cn = (n - cb < KCAP) ? n - cb : KCAP;
+        __syncthreads();
+        for (int t = tid; t < cn; t += 256) {
+            sc[t] = crow[cb + t];
+            sp[t] = prow[cb + t];
         }
         __syncthreads();
-        acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
-        __syncthreads();
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < PB; ++q) {
+                const int k = k0 + bk0 + q;
+                ev[q] = (k < cn && bok) ? emars[(int64_t)sc[k] * B + bb] : kNegInf;
+            }
+#pragma unroll
+            for (int q = 0; q < PA; ++q) {
+                const int k = k0 + ak0 + q;
+                av[q] = (k < cn) ? params[(int64_t)sp[k] + m0 + arow] : 0.0f;
+            }
+        };
+        load(0);
+        for (int k0 = 0; k0 < cn; k0 += TK) {
+            float lm = kNegInf;
+#pragma unroll
+            for (int q = 0; q < PB; ++q) lm = fmaxf(lm, ev[q]);
+            red[bg][bcol] = lm;
+            __syncthreads();
+            if (tid < TN) {
+                float t = red[0][tid];
+#pragma unroll
+                for (int g = 1; g < G; ++g) t = fmaxf(t, red[g][tid]);
+                const float mo = mrun[tid];
+                const float mn = fmaxf(mo, t);
+                msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
+                mrun[tid] = mn;
+            }
+            __syncthreads();
+            const float mn = mrun[bcol];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(ev[q] - mn));
+#pragma unroll
+            for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
+            const float s = msc[mycol];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] *= s;
+            __syncthreads();
+            if (k0 + TK < cn) load(k0 + TK);
+            acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+        }
     }
-    const int col = wn * 32 + (lane & 31);
-    const int64_t b = b0 + col;
+    const int64_t b = b0 + mycol;
     if (b < B) {
-        const float m = cmax[col];
+        const float m = mrun[mycol];
         const int64_t base = (int64_t)nids[j] + m0 + wm * 32;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -158,6 +174,8 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 }
 
 // ---------------------------------------------------------------------------------------------
+// Element flows.  Workgroup = (child block j, TM child rows) x TN batch columns; the reduction
+// runs over the flattened (parent block, parent row) index with a running max of log(f / m).
 template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* __restrict__ nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
@@ -169,63 +187,98 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
     constexpr int PAD = 16 / sizeof(T);
     constexpr int LD = TK + PAD;
+    constexpr int G = 256 / TN, PB = TK / G;
+    constexpr int PA = TK * TM / 256;
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ float cmax[TN];
-    __shared__ float red[256 / TN][TN];
+    __shared__ int spar[KCAP], spp[KCAP];
+    __shared__ float red[G][TN], mrun[TN], msc[TN];
 
     const int mtiles = cbs / TM;
     const int j = blockIdx.x / mtiles, m0 = (blockIdx.x % mtiles) * TM;
     const int64_t b0 = (int64_t)blockIdx.y * TN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
-    const int kflat = npar[j] * bs;
+    const int np = npar[j];
     const int* parrow = parids + (int64_t)j * KP;
     const int* pprow = parpids + (int64_t)j * KP;
-
-    column_max<TN>(cmax, red, kflat, b0, B, [&](int kk, int64_t b) {
-        const int kb = kk / bs, pi = kk - kb * bs;
-        return lfdm_at(nflows, nmars, ((int64_t)parrow[kb] + pi) * B + b, modified);
-    });
-
-    constexpr int PB = TK * TN / 256;
-    const int bcol = tid % TN, bk0 = (tid / TN) * PB;
-    const float bm = cmax[bcol];
+    const int bcol = tid % TN, bg = tid / TN, bk0 = bg * PB;
     const int64_t bb = b0 + bcol;
-    constexpr int PA = TK * TM / 256;
-    const int arow = tid % TM, ak0 = (tid / TM) * PA;
+    const bool bok = bb < B;
+    // parameter tile: a child row's parameters are contiguous along the parent index, so lanes
+    // take PA consecutive reduction indices of one row
+    constexpr int AT = TK / PA;                    // threads per row
+    const int arow = tid / AT, ak0 = (tid % AT) * PA;
+    const int mycol = wn * 32 + (lane & 31);
+    const int64_t arow_off = (int64_t)(m0 + arow) * bs;
 
+    if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
-    for (int k0 = 0; k0 < kflat; k0 += TK) {
-#pragma unroll
-        for (int q = 0; q < PB; ++q) {
-            const int kk = k0 + bk0 + q;
-            float x = 0.0f;
-            if (kk < kflat && bb < B && bm != kNegInf) {
-                const int kb = kk / bs, pi = kk - kb * bs;
-                const float l = lfdm_at(nflows, nmars, ((int64_t)parrow[kb] + pi) * B + bb, modified);
-                x = __expf(l - bm);
-            }
-            Bs[bcol][bk0 + q] = MF<T>::cvt(x);
-        }
-#pragma unroll
-        for (int q = 0; q < PA; ++q) {
-            const int kk = k0 + ak0 + q;
-            float a = 0.0f;
-            if (kk < kflat) {
-                const int kb = kk / bs, pi = kk - kb * bs;
-                a = params[(int64_t)pprow[kb] + (int64_t)(m0 + arow) * bs + pi];
-            }
-            As[arow][ak0 + q] = MF<T>::cvt(a);
+    float lv[PB], av[PA];
+    for (int pb = 0; pb < np; pb += KCAP) {
+        const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
+        const int kf = cnp * bs;
+        __syncthreads();
+        for (int t = tid; t < cnp; t += 256) {
+            spar[t] = parrow[pb + t];
+            spp[t] = pprow[pb + t];
         }
         __syncthreads();
-        acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
-        __syncthreads();
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < PB; ++q) {
+                const int kk = k0 + bk0 + q;
+                float v = kNegInf;
+                if (kk < kf && bok) {
+                    const int kb = kk / bs, pi = kk - kb * bs;
+                    v = lfdm_at(nflows, nmars, ((int64_t)spar[kb] + pi) * B + bb, modified);
+                }
+                lv[q] = v;
+            }
+#pragma unroll
+            for (int q = 0; q < PA; ++q) {
+                const int kk = k0 + ak0 + q;
+                float a = 0.0f;
+                if (kk < kf) {
+                    const int kb = kk / bs, pi = kk - kb * bs;
+                    a = params[(int64_t)spp[kb] + arow_off + pi];
+                }
+                av[q] = a;
+            }
+        };
+        load(0);
+        for (int k0 = 0; k0 < kf; k0 += TK) {
+            float lm = kNegInf;
+#pragma unroll
+            for (int q = 0; q < PB; ++q) lm = fmaxf(lm, lv[q]);
+            red[bg][bcol] = lm;
+            __syncthreads();
+            if (tid < TN) {
+                float t = red[0][tid];
+#pragma unroll
+                for (int g = 1; g < G; ++g) t = fmaxf(t, red[g][tid]);
+                const float mo = mrun[tid];
+                const float mn = fmaxf(mo, t);
+                msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
+                mrun[tid] = mn;
+            }
+            __syncthreads();
+            const float mn = mrun[bcol];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(lv[q] - mn));
+#pragma unroll
+            for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
+            const float s = msc[mycol];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] *= s;
+            __syncthreads();
+            if (k0 + TK < kf) load(k0 + TK);
+            acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+        }
     }
-    const int col = wn * 32 + (lane & 31);
-    const int64_t b = b0 + col;
+    const int64_t b = b0 + mycol;
     if (b < B) {
-        const float m = cmax[col];
+        const float m = mrun[mycol];
         const int64_t base = (int64_t)chids[j] + m0 + wm * 32;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -239,6 +292,8 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Parameter flows.  Workgroup = (node block j, TM parent rows) x (TN children), reduction over
+// the batch range [blockIdx.z * bchunk, ...).  Operands of the next stage are prefetched.
 template <typename T, int TM, int TN, typename PF>
 __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
@@ -253,7 +308,7 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
     constexpr int LD = TK + PAD;
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ float Xr[TM][TK + 1];
+    __shared__ float xred[256 / TK][TK];
     __shared__ float sh[TK];
 
     const int mtiles = bs / TM;
@@ -268,50 +323,58 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
     const int64_t nrow0 = (int64_t)nids[j] + m0;
     const int* crow = cids + (int64_t)j * K;
 
-    constexpr int PX = TK * TM / 256;   // consecutive batch columns per thread, X tile
-    constexpr int XT = TK / PX;         // threads per row
-    const int xrow = tid / XT, xb0 = (tid % XT) * PX;
+    constexpr int XG = 256 / TK;        // row groups of the X tile; lanes run along the batch
+    constexpr int PX = TM / XG;         // rows per thread
+    const int xb = tid % TK, xr0 = (tid / TK) * PX;
     constexpr int PY = TK * TN / 256;
     constexpr int YT = TK / PY;
     const int yrow = tid / YT, yb0 = (tid % YT) * PY;
     const int yk = kt0 + yrow;
     const int64_t ybase = (yk < n) ? (int64_t)crow[yk] * B : -1;
+    const int64_t xbase = (nrow0 + xr0) * B;
 
-    floatx16 acc = {0};
-    for (int64_t b0 = bstart; b0 < bend; b0 += TK) {
-        float xv[PX];
+    float xv[PX], yv[PY];
+    auto load = [&](int64_t b0) {
+        const int64_t b = b0 + xb;
 #pragma unroll
-        for (int q = 0; q < PX; ++q) {
-            const int64_t b = b0 + xb0 + q;
-            xv[q] = (b < bend) ? lfdm_at(nflows, nmars, (nrow0 + xrow) * B + b, modified) : kNegInf;
-            Xr[xrow][xb0 + q] = xv[q];
-        }
-        __syncthreads();
-        if (tid < TK) {
-            float t = kNegInf;
-            for (int r = 0; r < TM; ++r) t = fmaxf(t, Xr[r][tid]);
-            sh[tid] = t;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PX; ++q) {
-            const float s = sh[xb0 + q];
-            As[xrow][xb0 + q] = MF<T>::cvt((s == kNegInf || xv[q] == kNegInf) ? 0.0f : __expf(xv[q] - s));
-        }
+        for (int q = 0; q < PX; ++q)
+            xv[q] = (b < bend) ? lfdm_at(nflows, nmars, xbase + (int64_t)q * B + b, modified) : kNegInf;
 #pragma unroll
         for (int q = 0; q < PY; ++q) {
             const int64_t b = b0 + yb0 + q;
-            const float s = sh[yb0 + q];
-            float y = 0.0f;
-            if (ybase >= 0 && b < bend && s != kNegInf) {
-                const float e = emars[ybase + b];
-                y = (e == kNegInf) ? 0.0f : __expf(e + s);
-            }
-            Bs[yrow][yb0 + q] = MF<T>::cvt(y);
+            yv[q] = (ybase >= 0 && b < bend) ? emars[ybase + b] : kNegInf;
+        }
+    };
+
+    floatx16 acc = {0};
+    load(bstart);
+    for (int64_t b0 = bstart; b0 < bend; b0 += TK) {
+        float lm = kNegInf;
+#pragma unroll
+        for (int q = 0; q < PX; ++q) lm = fmaxf(lm, xv[q]);
+        xred[tid / TK][xb] = lm;
+        __syncthreads();
+        if (tid < TK) {
+            float t = xred[0][tid];
+#pragma unroll
+            for (int g = 1; g < XG; ++g) t = fmaxf(t, xred[g][tid]);
+            sh[tid] = t;
         }
         __syncthreads();
-        acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+        {
+            const float s = sh[xb];
+#pragma unroll
+            for (int q = 0; q < PX; ++q)
+                As[xr0 + q][xb] = MF<T>::cvt((s == kNegInf || xv[q] == kNegInf) ? 0.0f : __expf(xv[q] - s));
+        }
+#pragma unroll
+        for (int q = 0; q < PY; ++q) {
+            const float s = sh[yb0 + q];
+            Bs[yrow][yb0 + q] = MF<T>::cvt((s == kNegInf || yv[q] == kNegInf) ? 0.0f : __expf(yv[q] + s));
+        }
         __syncthreads();
+        if (b0 + TK < bend) load(b0 + TK);
+        acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
     }
     const int k = kt0 + wn * 32 + (lane & 31);
     if (k < n) {
@@ -383,11 +446,12 @@ int launch_par(PF* pflows, const float* nflows, const float* nmars, const float*
     const int TM = bs >= 64 ? 64 : 32;
     const int TN = bs >= 64 ? 64 : 128;
     const int64_t tiles = (int64_t)n_blocks * (bs / TM) * ((K + TN - 1) / TN);
-    // Split the batch so that the launch has >= ~2048 workgroups; partial sums are added atomically.
-    int64_t splits = (2048 + tiles - 1) / tiles;
+    // Split the batch only when the tile grid alone cannot fill the chip (<= 4 ways); split partial
+    // sums are added atomically.
+    int64_t splits = (512 + tiles - 1) / tiles;
     const int64_t max_splits = (B + TK - 1) / TK;
     if (splits > max_splits) splits = max_splits;
-    if (splits > 16) splits = 16;
+    if (splits > 4) splits = 4;
     if (splits < 1) splits = 1;
     int64_t bchunk = (B + splits - 1) / splits;
     bchunk = ((bchunk + TK - 1) / TK) * TK;

@@ -91,6 +91,7 @@ class TensorCircuit(nn.Module):
         self._run_params = dict()
         self._cum_flow = 0.0
         self._em_dev = None
+        self.modify_flows_in_place = True
 
     # ------------------------------------------------------------------ device placement
 
@@ -131,9 +132,10 @@ class TensorCircuit(nn.Module):
         assert dev.type == "cuda", "TensorCircuit passes run on the HIP device; call `pc.to('cuda')` first"
         if self.node_mars is None or self.node_mars.size(1) != B:
             self.node_mars = torch.empty([self.num_nodes, B], device=dev, dtype=torch.float32)
-            self.element_mars = torch.empty([self.num_elements, B], device=dev, dtype=torch.float32)
+            self.element_mars = torch.empty([self.num_element_rows, B], device=dev, dtype=torch.float32)
             self._fill(self.node_mars[:self.num_dummy_nodes], 0.0)
-            self._fill(self.element_mars[:self.num_dummy_eles], float("-inf"))
+            for off in self._elem_offsets:
+                self._fill(self.element_mars[off:off + self.num_dummy_eles], float("-inf"))
             self.node_flows = self.element_flows = None
         if need_flows and (self.node_flows is None or self.node_flows.size(1) != B):
             self.node_flows = torch.empty([self.num_nodes, B], device=dev, dtype=torch.float32)
@@ -164,12 +166,17 @@ class TensorCircuit(nn.Module):
             raise NotImplementedError(f"Propagation algorithm {propagation_alg} is not implemented by the HIP layers.")
         self.default_propagation_alg = "LL"
 
+    def _slab(self, group_id: int) -> torch.Tensor:
+        """``element_mars`` rows of the product group ``group_id`` (or of the sum group right above it)."""
+        off = self._elem_offsets[group_id // 2]
+        return self.element_mars.narrow(0, off, self._group_elems[group_id // 2])
+
     def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
-        for g in self.inner_layer_groups:
+        for gid, g in enumerate(self.inner_layer_groups):
             if g.is_prod():
-                g(self.node_mars, self.element_mars)
+                g(self.node_mars, self._slab(gid))
             else:
-                g(self.node_mars, self.element_mars, self.params, force_use_bf16=force_use_bf16,
+                g(self.node_mars, self._slab(gid), self.params, force_use_bf16=force_use_bf16,
                   force_use_fp32=force_use_fp32, precision=precision)
 
     def forward(self, inputs: torch.Tensor, input_layer_fn: Optional[Union[str, Callable]] = None,
@@ -234,15 +241,20 @@ class TensorCircuit(nn.Module):
         return g
 
     def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision):
+        # Element marginals of every layer are still resident (one slab per product group), so no
+        # product layer is recomputed here (the reference recomputes, tensorcircuit.py:446-447).
+        # ``allow_modify_flows`` permits the in-place log(flow) - mar transform of the sum nodes'
+        # flows (one pass per layer, re-read by the element- and parameter-flow kernels); with
+        # ``modify_flows_in_place = False`` the kernels compute it on the fly instead.
+        modify = allow_modify_flows and self.modify_flows_in_place
         for lid in range(len(self.inner_layer_groups) - 1, -1, -1):
             g = self.inner_layer_groups[lid]
             if g.is_prod():
                 g.backward(self.node_flows, self.element_flows)
             else:
-                self.inner_layer_groups[lid - 1].forward(self.node_mars, self.element_mars, _for_backward=True)
-                g.backward(self.node_flows, self.element_flows, self.node_mars, self.element_mars, self.params,
+                g.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
                            param_flows=self.param_flows if compute_param_flows else None,
-                           allow_modify_flows=allow_modify_flows, precision=precision)
+                           allow_modify_flows=modify, precision=precision)
 
     def backward(self, inputs: Optional[torch.Tensor] = None, ll_weights: Optional[torch.Tensor] = None,
                  compute_param_flows: bool = True, flows_memory: float = 1.0,
@@ -382,11 +394,12 @@ class TensorCircuit(nn.Module):
         a, b = ns._output_ind_range
         if ns.is_sum() or ns.is_input():
             return self.node_mars[a:b, :].detach()
-        for g in self.inner_layer_groups:
+        for gid, g in enumerate(self.inner_layer_groups):
             for layer in g:
                 if layer.is_prod() and any(n is ns for n in layer.nodes):
-                    layer(self.node_mars, self.element_mars)
-                    return self.element_mars[a:b, :].detach()
+                    slab = self._slab(gid)
+                    layer(self.node_mars, slab)
+                    return slab[a:b, :].detach()
         raise ValueError("node not found")
 
     def get_node_flows(self, ns: CircuitNodes, **kwargs):
@@ -397,10 +410,9 @@ class TensorCircuit(nn.Module):
         for lid, g in enumerate(self.inner_layer_groups):
             for layer in g:
                 if layer.is_prod() and any(n is ns for n in layer.nodes):
-                    g.forward(self.node_mars, self.element_mars, _for_backward=True)
                     self.inner_layer_groups[lid + 1].backward(
-                        self.node_flows, self.element_flows, self.node_mars, self.element_mars, self.params,
-                        param_flows=None, allow_modify_flows=self._run_params.get("allow_modify_flows", False),
+                        self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
+                        param_flows=None, allow_modify_flows=False,
                         precision=self._run_params.get("precision", self.precision))
                     return self.element_flows[a:b, :].detach()
         raise ValueError("node not found")
@@ -437,6 +449,7 @@ class TensorCircuit(nn.Module):
         node2tiednodes = dict()
 
         layer_id = 0
+        self._group_elems = []
         for depth in range(num_layers):
             if depth == 0:
                 sig2nodes = dict()
@@ -469,6 +482,7 @@ class TensorCircuit(nn.Module):
             self.inner_layer_groups.append(pg)
             self.add_module(f"prod_layer_{layer_id}", pg)
             num_elements = max(num_elements, layer_num_elements)
+            self._group_elems.append(layer_num_elements)
 
             g2s = dict()
             for ns in depth2nodes[depth]["sum"]:
@@ -492,6 +506,14 @@ class TensorCircuit(nn.Module):
         self.num_nodes = num_nodes
         self.num_edges = num_edges
         self.num_elements = num_elements
+        # Element rows of every product group live in their own slab of ``element_mars`` (HBM is
+        # large enough to keep all of them), so the backward pass needs no product recompute.
+        offs, tot = [], 0
+        for n_el in self._group_elems:
+            offs.append(tot)
+            tot += n_el
+        self._elem_offsets = offs
+        self.num_element_rows = tot
         self.num_sum_params = num_params
         self.num_param_flows = num_pflows
         self.node2tiednodes = node2tiednodes
