@@ -14,6 +14,14 @@ enum {
     PJAMD_EUNSUPPORTED = -2 // feature not implemented by this library
 };
 
+// Row of the partition descriptor table of pjamd_sum_backward_par_batched (int32 fields):
+//   [0] first logical tile of the partition   [1] K (index row width)
+//   [2] m-tiles per node block                [3] child tiles per node block
+//   [4] offset of the partition's node blocks in nids / nch
+//   [5] offset of its index rows in cids / pids / pfids
+//   [6] first element row of its layer's element slab   [7] 1 if parameter-flow slots are shared
+#define PJAMD_PAR_DESC_FIELDS 8
+
 #define PJAMD_LAUNCH_CHECK()                      \
     do {                                          \
         hipError_t e_ = hipGetLastError();        \

@@ -4,10 +4,17 @@
 //
 // Every kernel is one workgroup of 4 wavefronts computing a TM x TN tile (each wavefront one
 // 32 x 32 MFMA accumulator).  Operand tiles are staged through LDS with K (the reduction index)
-// contiguous, TK = 32 reduction steps per stage.  Range is controlled by max subtraction:
+// contiguous, TK = 128 reduction steps per stage, and the next stage is prefetched into registers
+// while the matrix cores work on the current one.  Deep stages matter because most layers of a
+// tree-shaped circuit are narrow (a few node blocks): there the kernel time is the number of
+// dependent memory round trips, not bandwidth.  Range is controlled by max subtraction:
 //   forward   : exp(e - colmax_over_children)          (values in [0, 1])
 //   ele-flows : exp(log(f / m) - colmax_over_parents)  (values in [0, 1])
 //   par-flows : exp(log(f / m) - s_b) * exp(e + s_b),  s_b = max over the tile's parent rows
+//
+// Grids are one-dimensional over logical tiles.  The hardware hands workgroup i to XCD i % 8;
+// xcd_tile() gives each XCD a contiguous range of logical tiles so that the tiles of one node
+// block (which share parameters and child rows) meet in the same L2.
 #include "pjamd_common.h"
 
 using namespace pjamd;
@@ -18,7 +25,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
-constexpr int TK = 32;
+constexpr int TK = 128;
+constexpr int kXcds = 8;
 
 template <typename T>
 struct MF;
@@ -26,6 +34,7 @@ struct MF;
 template <>
 struct MF<__bf16> {
     static constexpr int KPL = 8;  // reduction values per lane per instruction (instruction K = 16)
+    static constexpr int PAD = 8;  // LDS row stride 272 B: 16-byte aligned, 4-bank shift per row
     static __device__ __forceinline__ floatx16 mma(const __bf16* a, const __bf16* b, floatx16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(a),
                                                        *reinterpret_cast<const bf16x8*>(b), c, 0, 0, 0);
@@ -36,6 +45,7 @@ struct MF<__bf16> {
 template <>
 struct MF<_Float16> {
     static constexpr int KPL = 8;
+    static constexpr int PAD = 8;
     static __device__ __forceinline__ floatx16 mma(const _Float16* a, const _Float16* b, floatx16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const halfx8*>(a),
                                                       *reinterpret_cast<const halfx8*>(b), c, 0, 0, 0);
@@ -46,6 +56,7 @@ struct MF<_Float16> {
 template <>
 struct MF<float> {
     static constexpr int KPL = 1;  // instruction K = 2
+    static constexpr int PAD = 1;  // LDS row stride 129 words: lanes of a column read distinct banks
     static __device__ __forceinline__ floatx16 mma(const float* a, const float* b, floatx16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x2f32(*a, *b, c, 0, 0, 0);
     }
@@ -55,8 +66,8 @@ struct MF<float> {
 // Row of accumulator register r for this lane inside a 32 x 32 tile (the column is lane & 31).
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-template <typename T, int WM, int WN, int LDA, int LDB>
-__device__ __forceinline__ floatx16 stage_mma(const T (*As)[LDA], const T (*Bs)[LDB], int wm, int wn, int lane,
+template <typename T, int LD>
+__device__ __forceinline__ floatx16 stage_mma(const T (*As)[LD], const T (*Bs)[LD], int wm, int wn, int lane,
                                               floatx16 acc) {
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 2 * MF<T>::KPL) {
@@ -64,6 +75,14 @@ __device__ __forceinline__ floatx16 stage_mma(const T (*As)[LDA], const T (*Bs)[
                          &Bs[wn * 32 + (lane & 31)][kk + MF<T>::KPL * (lane >> 5)], acc);
     }
     return acc;
+}
+
+// Logical tile of workgroup `bid` among `nb`: XCD x = bid % 8 owns the contiguous logical range
+// [start(x), start(x) + count(x)), where the first nb % 8 XCDs own one tile more.  Bijective.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+    const int per = nb / kXcds, rem = nb % kXcds;
+    const int x = bid % kXcds, s = bid / kXcds;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + s;
 }
 
 __device__ __forceinline__ float lfdm_at(const float* nflows, const float* nmars, int64_t o, bool modified) {
@@ -75,18 +94,18 @@ __device__ __forceinline__ float lfdm_at(const float* nflows, const float* nmars
 constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per chunk
 
 // ---------------------------------------------------------------------------------------------
-// Forward.  Workgroup = (node block j, TM parent rows) x TN batch columns.  The child index rows
-// are staged in LDS; the next stage's operands are prefetched into registers while the matrix
-// cores work on the current one; the column max is a running max (accumulators are rescaled).
+// Forward.  Tile = (node block j, TM parent rows) x TN batch columns; logical tile order is
+// m-tile fastest, then batch tile, then node block.  The child index rows are staged in LDS; the
+// column max is a running max (accumulators are rescaled when it grows).
 template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ nids,
                                                     const int* __restrict__ cids, const int* __restrict__ pids,
-                                                    const int* __restrict__ nch, int K, int bs, int64_t B) {
+                                                    const int* __restrict__ nch, int K, int64_t B, int mtiles,
+                                                    int btiles) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
-    constexpr int PAD = 16 / sizeof(T);
-    constexpr int LD = TK + PAD;
+    constexpr int LD = TK + MF<T>::PAD;
     constexpr int G = 256 / TN, PB = TK / G;       // element tile: G row groups x TN columns
     constexpr int PA = TK * TM / 256;              // parameter tile: PA consecutive k per thread
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
@@ -94,9 +113,10 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     __shared__ int sc[KCAP], sp[KCAP];
     __shared__ float red[G][TN], mrun[TN], msc[TN];
 
-    const int mtiles = bs / TM;
-    const int j = blockIdx.x / mtiles, m0 = (blockIdx.x % mtiles) * TM;
-    const int64_t b0 = (int64_t)blockIdx.y * TN;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int n = nch[j];
@@ -114,9 +134,9 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     for (int cb = 0; cb < n; cb += KCAP) {
         const int cn = (n - cb < KCAP) ? n - cb : KCAP;
         __syncthreads();
-        for (int t = tid; t < cn; t += 256) {
-            sc[t] = crow[cb + t];
-            sp[t] = prow[cb + t];
+        for (int u = tid; u < cn; u += 256) {
+            sc[u] = crow[cb + u];
+            sp[u] = prow[cb + u];
         }
         __syncthreads();
         auto load = [&](int k0) {
@@ -139,11 +159,11 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
             red[bg][bcol] = lm;
             __syncthreads();
             if (tid < TN) {
-                float t = red[0][tid];
+                float v = red[0][tid];
 #pragma unroll
-                for (int g = 1; g < G; ++g) t = fmaxf(t, red[g][tid]);
+                for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
                 const float mo = mrun[tid];
-                const float mn = fmaxf(mo, t);
+                const float mn = fmaxf(mo, v);
                 msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
                 mrun[tid] = mn;
             }
@@ -158,7 +178,7 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
             for (int r = 0; r < 16; ++r) acc[r] *= s;
             __syncthreads();
             if (k0 + TK < cn) load(k0 + TK);
-            acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
         }
     }
     const int64_t b = b0 + mycol;
@@ -174,29 +194,32 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 }
 
 // ---------------------------------------------------------------------------------------------
-// Element flows.  Workgroup = (child block j, TM child rows) x TN batch columns; the reduction
-// runs over the flattened (parent block, parent row) index with a running max of log(f / m).
+// Element flows.  Tile = (child block j, TM child rows) x TN batch columns; the reduction runs
+// over the flattened (parent block, parent row) index kk -> (kk >> lbs, kk & (bs - 1)) with a
+// running max of log(f / m).  Parameters of one child row are contiguous along the parent row,
+// so the parameter tile is loaded with lanes along kk.
 template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* __restrict__ nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ chids,
                                                     const int* __restrict__ parids, const int* __restrict__ parpids,
-                                                    const int* __restrict__ npar, int KP, int cbs, int bs, int64_t B,
-                                                    bool modified, bool accumulate) {
+                                                    const int* __restrict__ npar, int KP, int lbs, int64_t B,
+                                                    int mtiles, int btiles, bool modified, bool accumulate) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
-    constexpr int PAD = 16 / sizeof(T);
-    constexpr int LD = TK + PAD;
+    constexpr int LD = TK + MF<T>::PAD;
     constexpr int G = 256 / TN, PB = TK / G;
-    constexpr int PA = TK * TM / 256;
+    constexpr int RA = 256 / TK, PA = TM / RA;     // parameter tile: RA rows per pass, PA passes
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
     __shared__ int spar[KCAP], spp[KCAP];
     __shared__ float red[G][TN], mrun[TN], msc[TN];
 
-    const int mtiles = cbs / TM;
-    const int j = blockIdx.x / mtiles, m0 = (blockIdx.x % mtiles) * TM;
-    const int64_t b0 = (int64_t)blockIdx.y * TN;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = 1 << lbs;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int np = npar[j];
@@ -205,23 +228,19 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     const int bcol = tid % TN, bg = tid / TN, bk0 = bg * PB;
     const int64_t bb = b0 + bcol;
     const bool bok = bb < B;
-    // parameter tile: a child row's parameters are contiguous along the parent index, so lanes
-    // take PA consecutive reduction indices of one row
-    constexpr int AT = TK / PA;                    // threads per row
-    const int arow = tid / AT, ak0 = (tid % AT) * PA;
+    const int ak = tid % TK, ar0 = tid / TK;
     const int mycol = wn * 32 + (lane & 31);
-    const int64_t arow_off = (int64_t)(m0 + arow) * bs;
 
     if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
     float lv[PB], av[PA];
     for (int pb = 0; pb < np; pb += KCAP) {
         const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
-        const int kf = cnp * bs;
+        const int kf = cnp << lbs;
         __syncthreads();
-        for (int t = tid; t < cnp; t += 256) {
-            spar[t] = parrow[pb + t];
-            spp[t] = pprow[pb + t];
+        for (int u = tid; u < cnp; u += 256) {
+            spar[u] = parrow[pb + u];
+            spp[u] = pprow[pb + u];
         }
         __syncthreads();
         auto load = [&](int k0) {
@@ -229,21 +248,18 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
             for (int q = 0; q < PB; ++q) {
                 const int kk = k0 + bk0 + q;
                 float v = kNegInf;
-                if (kk < kf && bok) {
-                    const int kb = kk / bs, pi = kk - kb * bs;
-                    v = lfdm_at(nflows, nmars, ((int64_t)spar[kb] + pi) * B + bb, modified);
-                }
+                if (kk < kf && bok)
+                    v = lfdm_at(nflows, nmars, ((int64_t)spar[kk >> lbs] + (kk & (bs - 1))) * B + bb, modified);
                 lv[q] = v;
             }
+            const int kk = k0 + ak;
+            if (kk < kf) {
+                const int64_t base = (int64_t)spp[kk >> lbs] + (int64_t)(m0 + ar0) * bs + (kk & (bs - 1));
 #pragma unroll
-            for (int q = 0; q < PA; ++q) {
-                const int kk = k0 + ak0 + q;
-                float a = 0.0f;
-                if (kk < kf) {
-                    const int kb = kk / bs, pi = kk - kb * bs;
-                    a = params[(int64_t)spp[kb] + arow_off + pi];
-                }
-                av[q] = a;
+                for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
+            } else {
+#pragma unroll
+                for (int q = 0; q < PA; ++q) av[q] = 0.0f;
             }
         };
         load(0);
@@ -254,11 +270,11 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
             red[bg][bcol] = lm;
             __syncthreads();
             if (tid < TN) {
-                float t = red[0][tid];
+                float v = red[0][tid];
 #pragma unroll
-                for (int g = 1; g < G; ++g) t = fmaxf(t, red[g][tid]);
+                for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
                 const float mo = mrun[tid];
-                const float mn = fmaxf(mo, t);
+                const float mn = fmaxf(mo, v);
                 msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
                 mrun[tid] = mn;
             }
@@ -267,13 +283,13 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
 #pragma unroll
             for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(lv[q] - mn));
 #pragma unroll
-            for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
+            for (int q = 0; q < PA; ++q) As[ar0 + q * RA][ak] = MF<T>::cvt(av[q]);
             const float s = msc[mycol];
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] *= s;
             __syncthreads();
             if (k0 + TK < kf) load(k0 + TK);
-            acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
         }
     }
     const int64_t b = b0 + mycol;
@@ -292,57 +308,51 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Parameter flows.  Workgroup = (node block j, TM parent rows) x (TN children), reduction over
-// the batch range [blockIdx.z * bchunk, ...).  Operands of the next stage are prefetched.
+// Parameter flows of one tile = (node block j, TM parent rows from m0) x (TN children from kt0),
+// reduced over the batch range [bstart, bend).  Both operand tiles are loaded with lanes along
+// the batch.  Shared by the per-layer and the batched kernels below.
 template <typename T, int TM, int TN, typename PF>
-__global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, const float* __restrict__ nflows,
-                                                    const float* __restrict__ nmars, const float* __restrict__ emars,
-                                                    const float* __restrict__ params, const int* __restrict__ nids,
-                                                    const int* __restrict__ cids, const int* __restrict__ pids,
-                                                    const int* __restrict__ pfids, const int* __restrict__ nch, int K,
-                                                    int bs, int64_t B, int64_t bchunk, bool modified, bool atomic_out,
-                                                    float sign) {
+__device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* __restrict__ nflows,
+                                         const float* __restrict__ nmars, const float* __restrict__ emars,
+                                         const float* __restrict__ params, const int* __restrict__ nids,
+                                         const int* __restrict__ cids, const int* __restrict__ pids,
+                                         const int* __restrict__ pfids, const int* __restrict__ nch, int K, int64_t B,
+                                         int j, int m0, int kt0, int64_t bstart, int64_t bend, bool modified,
+                                         bool atomic_out, float sign) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
-    constexpr int PAD = 16 / sizeof(T);
-    constexpr int LD = TK + PAD;
+    constexpr int LD = TK + MF<T>::PAD;
+    constexpr int RG = 256 / TK;        // rows per pass of a tile load (lanes run along the batch)
+    constexpr int PX = TM / RG, PY = TN / RG;
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ float xred[256 / TK][TK];
+    __shared__ float xred[RG][TK];
     __shared__ float sh[TK];
+    __shared__ int64_t sy[TN];          // element row offsets of the tile's children (-1: padding)
 
-    const int mtiles = bs / TM;
-    const int j = blockIdx.x / mtiles, m0 = (blockIdx.x % mtiles) * TM;
-    const int kt0 = blockIdx.y * TN;
     const int n = nch[j];
     if (kt0 >= n) return;
-    const int64_t bstart = (int64_t)blockIdx.z * bchunk;
-    const int64_t bend = (bstart + bchunk < B) ? bstart + bchunk : B;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int64_t nrow0 = (int64_t)nids[j] + m0;
     const int* crow = cids + (int64_t)j * K;
 
-    constexpr int XG = 256 / TK;        // row groups of the X tile; lanes run along the batch
-    constexpr int PX = TM / XG;         // rows per thread
-    const int xb = tid % TK, xr0 = (tid / TK) * PX;
-    constexpr int PY = TK * TN / 256;
-    constexpr int YT = TK / PY;
-    const int yrow = tid / YT, yb0 = (tid % YT) * PY;
-    const int yk = kt0 + yrow;
-    const int64_t ybase = (yk < n) ? (int64_t)crow[yk] * B : -1;
-    const int64_t xbase = (nrow0 + xr0) * B;
+    const int xb = tid % TK, r0 = tid / TK;
+    if (tid < TN) sy[tid] = (kt0 + tid < n) ? (int64_t)crow[kt0 + tid] * B : -1;
+    __syncthreads();
+    const int64_t xbase = (nrow0 + r0) * B;
 
     float xv[PX], yv[PY];
     auto load = [&](int64_t b0) {
         const int64_t b = b0 + xb;
+        const bool ok = b < bend;
 #pragma unroll
         for (int q = 0; q < PX; ++q)
-            xv[q] = (b < bend) ? lfdm_at(nflows, nmars, xbase + (int64_t)q * B + b, modified) : kNegInf;
+            xv[q] = ok ? lfdm_at(nflows, nmars, xbase + (int64_t)q * RG * B + b, modified) : kNegInf;
 #pragma unroll
         for (int q = 0; q < PY; ++q) {
-            const int64_t b = b0 + yb0 + q;
-            yv[q] = (ybase >= 0 && b < bend) ? emars[ybase + b] : kNegInf;
+            const int64_t yb = sy[r0 + q * RG];
+            yv[q] = (ok && yb >= 0) ? emars[yb + b] : kNegInf;
         }
     };
 
@@ -352,29 +362,25 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
         float lm = kNegInf;
 #pragma unroll
         for (int q = 0; q < PX; ++q) lm = fmaxf(lm, xv[q]);
-        xred[tid / TK][xb] = lm;
+        xred[r0][xb] = lm;
         __syncthreads();
         if (tid < TK) {
-            float t = xred[0][tid];
+            float v = xred[0][tid];
 #pragma unroll
-            for (int g = 1; g < XG; ++g) t = fmaxf(t, xred[g][tid]);
-            sh[tid] = t;
+            for (int g = 1; g < RG; ++g) v = fmaxf(v, xred[g][tid]);
+            sh[tid] = v;
         }
         __syncthreads();
-        {
-            const float s = sh[xb];
+        const float s = sh[xb];
 #pragma unroll
-            for (int q = 0; q < PX; ++q)
-                As[xr0 + q][xb] = MF<T>::cvt((s == kNegInf || xv[q] == kNegInf) ? 0.0f : __expf(xv[q] - s));
-        }
+        for (int q = 0; q < PX; ++q)
+            As[r0 + q * RG][xb] = MF<T>::cvt((s == kNegInf || xv[q] == kNegInf) ? 0.0f : __expf(xv[q] - s));
 #pragma unroll
-        for (int q = 0; q < PY; ++q) {
-            const float s = sh[yb0 + q];
-            Bs[yrow][yb0 + q] = MF<T>::cvt((s == kNegInf || yv[q] == kNegInf) ? 0.0f : __expf(yv[q] + s));
-        }
+        for (int q = 0; q < PY; ++q)
+            Bs[r0 + q * RG][xb] = MF<T>::cvt((s == kNegInf || yv[q] == kNegInf) ? 0.0f : __expf(yv[q] + s));
         __syncthreads();
         if (b0 + TK < bend) load(b0 + TK);
-        acc = stage_mma<T, WM, WN, LD, LD>(As, Bs, wm, wn, lane, acc);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
     }
     const int k = kt0 + wn * 32 + (lane & 31);
     if (k < n) {
@@ -387,6 +393,52 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
             pf_add<PF>(&pflows[f0 + i], (PF)v, atomic_out);
         }
     }
+}
+
+// Per-layer launch: logical tile order is m-tile fastest, then child tile, then batch split,
+// then node block.
+template <typename T, int TM, int TN, typename PF>
+__global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, const float* __restrict__ nflows,
+                                                    const float* __restrict__ nmars, const float* __restrict__ emars,
+                                                    const float* __restrict__ params, const int* __restrict__ nids,
+                                                    const int* __restrict__ cids, const int* __restrict__ pids,
+                                                    const int* __restrict__ pfids, const int* __restrict__ nch, int K,
+                                                    int64_t B, int64_t bchunk, int mtiles, int ktiles, int splits,
+                                                    bool modified, bool atomic_out, float sign) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int kt0 = ((t / mtiles) % ktiles) * TN;
+    const int z = (t / (mtiles * ktiles)) % splits;
+    const int j = t / (mtiles * ktiles * splits);
+    const int64_t bstart = (int64_t)z * bchunk;
+    const int64_t bend = (bstart + bchunk < B) ? bstart + bchunk : B;
+    par_tile<T, TM, TN, PF>(pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, K, B, j, m0, kt0,
+                            bstart, bend, modified, atomic_out, sign);
+}
+
+// Batched launch over the partitions of many sum layers (one descriptor row per partition, see
+// pjamd_sum_backward_par_batched).  The tile order inside a partition is m-tile fastest, then
+// child tile, then node block; each tile reduces over the whole batch.
+template <typename T, int TM, int TN, typename PF>
+__global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, const float* __restrict__ nflows,
+                                                       const float* __restrict__ nmars,
+                                                       const float* __restrict__ emars,
+                                                       const float* __restrict__ params, const int* __restrict__ nids,
+                                                       const int* __restrict__ cids, const int* __restrict__ pids,
+                                                       const int* __restrict__ pfids, const int* __restrict__ nch,
+                                                       const int* __restrict__ desc,
+                                                       const int* __restrict__ tile_desc, int64_t B, bool modified,
+                                                       float sign) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int* D = desc + (int64_t)tile_desc[t] * PJAMD_PAR_DESC_FIELDS;
+    const int local = t - D[0];
+    const int K = D[1], mtiles = D[2], ktiles = D[3];
+    const int64_t bo = D[4], io = D[5];
+    const int m0 = (local % mtiles) * TM;
+    const int kt0 = ((local / mtiles) % ktiles) * TN;
+    const int j = local / (mtiles * ktiles);
+    par_tile<T, TM, TN, PF>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io, pids + io,
+                            pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, D[7] != 0, sign);
 }
 
 // One 32 x 32 x KI product with the operand / accumulator lane layouts the kernels above assume:
@@ -408,16 +460,29 @@ __global__ void mfma_layout_probe(const float* A, const float* B, float* C) {
     for (int r = 0; r < 16; ++r) C[acc_row(r, lane) * 32 + (lane & 31)] = acc[r];
 }
 
+bool grid_fits(int64_t tiles) { return tiles > 0 && tiles <= (int64_t)0x7fffffff / 256; }
+
+int log2_exact(int v) {
+    if (v < 1 || (v & (v - 1)) != 0) return -1;
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
 template <typename T>
 int launch_fwd(float* node_mars, const float* emars, const float* params, const int* nids, const int* cids,
                const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B, hipStream_t stream) {
-    if (bs >= 64) {
-        dim3 grid((unsigned)((int64_t)n_blocks * (bs / 64)), cdiv(B, 64));
-        sum_fwd_mfma<T, 64, 64><<<grid, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, bs, B);
-    } else {
-        dim3 grid((unsigned)n_blocks, cdiv(B, 128));
-        sum_fwd_mfma<T, 32, 128><<<grid, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, bs, B);
-    }
+    const int TM = bs >= 64 ? 64 : 32, TN = bs >= 64 ? 64 : 128;
+    const int mtiles = bs / TM;
+    const int64_t btiles = cdiv(B, TN);
+    const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
+    PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles));
+    if (bs >= 64)
+        sum_fwd_mfma<T, 64, 64><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K,
+                                                                     B, mtiles, (int)btiles);
+    else
+        sum_fwd_mfma<T, 32, 128><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch,
+                                                                      K, B, mtiles, (int)btiles);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -426,15 +491,20 @@ template <typename T>
 int launch_ele(float* eflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* chids, const int* parids, const int* parpids, const int* npar, int n, int KP, int cbs, int bs,
                int64_t B, bool mod, bool accum, hipStream_t stream) {
-    if (cbs >= 64) {
-        dim3 grid((unsigned)((int64_t)n * (cbs / 64)), cdiv(B, 64));
-        sum_ele_mfma<T, 64, 64><<<grid, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids, parpids,
-                                                         npar, KP, cbs, bs, B, mod, accum);
-    } else {
-        dim3 grid((unsigned)n, cdiv(B, 128));
-        sum_ele_mfma<T, 32, 128><<<grid, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids, parpids,
-                                                          npar, KP, cbs, bs, B, mod, accum);
-    }
+    const int lbs = log2_exact(bs);
+    const int TM = cbs >= 64 ? 64 : 32, TN = cbs >= 64 ? 64 : 128;
+    const int mtiles = cbs / TM;
+    const int64_t btiles = cdiv(B, TN);
+    const int64_t tiles = (int64_t)n * mtiles * btiles;
+    PJAMD_REQUIRE(lbs >= 0 && cbs % TM == 0 && grid_fits(tiles));
+    if (cbs >= 64)
+        sum_ele_mfma<T, 64, 64><<<(unsigned)tiles, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
+                                                                     parpids, npar, KP, lbs, B, mtiles, (int)btiles,
+                                                                     mod, accum);
+    else
+        sum_ele_mfma<T, 32, 128><<<(unsigned)tiles, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids,
+                                                                      parids, parpids, npar, KP, lbs, B, mtiles,
+                                                                      (int)btiles, mod, accum);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -443,9 +513,9 @@ template <typename T, typename PF>
 int launch_par(PF* pflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* nids, const int* cids, const int* pids, const int* pfids, const int* nch, int n_blocks, int K,
                int bs, int64_t B, bool mod, bool shared, float sign, hipStream_t stream) {
-    const int TM = bs >= 64 ? 64 : 32;
-    const int TN = bs >= 64 ? 64 : 128;
-    const int64_t tiles = (int64_t)n_blocks * (bs / TM) * ((K + TN - 1) / TN);
+    const int TM = bs >= 64 ? 64 : 32, TN = bs >= 64 ? 64 : 128;
+    const int mtiles = bs / TM, ktiles = (K + TN - 1) / TN;
+    const int64_t tiles = (int64_t)n_blocks * mtiles * ktiles;
     // Split the batch only when the tile grid alone cannot fill the chip (<= 4 ways); split partial
     // sums are added atomically.
     int64_t splits = (512 + tiles - 1) / tiles;
@@ -457,13 +527,32 @@ int launch_par(PF* pflows, const float* nflows, const float* nmars, const float*
     bchunk = ((bchunk + TK - 1) / TK) * TK;
     splits = (B + bchunk - 1) / bchunk;
     const bool atomic_out = shared || splits > 1;
-    dim3 grid((unsigned)((int64_t)n_blocks * (bs / TM)), (unsigned)((K + TN - 1) / TN), (unsigned)splits);
+    PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles * splits));
+    const unsigned grid = (unsigned)(tiles * splits);
     if (bs >= 64)
         sum_par_mfma<T, 64, 64, PF><<<grid, 256, 0, stream>>>(pflows, nflows, nmars, emars, params, nids, cids, pids,
-                                                             pfids, nch, K, bs, B, bchunk, mod, atomic_out, sign);
+                                                             pfids, nch, K, B, bchunk, mtiles, ktiles, (int)splits, mod,
+                                                             atomic_out, sign);
     else
         sum_par_mfma<T, 32, 128, PF><<<grid, 256, 0, stream>>>(pflows, nflows, nmars, emars, params, nids, cids, pids,
-                                                              pfids, nch, K, bs, B, bchunk, mod, atomic_out, sign);
+                                                              pfids, nch, K, B, bchunk, mtiles, ktiles, (int)splits,
+                                                              mod, atomic_out, sign);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+template <typename T, typename PF>
+int launch_par_batched(PF* pflows, const float* nflows, const float* nmars, const float* emars, const float* params,
+                       const int* nids, const int* cids, const int* pids, const int* pfids, const int* nch,
+                       const int* desc, const int* tile_desc, int n_tiles, int tile_rows, int64_t B, bool mod,
+                       float sign, hipStream_t stream) {
+    PJAMD_REQUIRE(grid_fits(n_tiles));
+    if (tile_rows == 64)
+        sum_par_batched<T, 64, 64, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
+            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, sign);
+    else
+        sum_par_batched<T, 32, 128, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
+            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, sign);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -526,4 +615,32 @@ PJAMD_EXPORT int pjamd_debug_mfma_probe(int precision, const float* A, const flo
     else return PJAMD_EINVAL;
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sum_backward_par_batched(void* param_flows, const float* node_flows, const float* node_mars,
+                                                const float* element_mars, const float* params, const int* nids,
+                                                const int* cids, const int* pids, const int* pfids, const int* nch,
+                                                const int* desc, const int* tile_desc, int n_tiles, int tile_rows,
+                                                int64_t B, int flags, int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_tiles >= 0 && B > 0 && (tile_rows == 32 || tile_rows == 64) && precision >= 0 && precision <= 2);
+    if (n_tiles == 0) return PJAMD_OK;
+    const bool mod = flags & 1;
+    const float sign = (flags & 4) ? -1.0f : 1.0f;
+    const bool pf64 = (flags & 8) != 0;
+    if (precision == 0) {
+        if (pf64)
+            return launch_par_batched<float, double>((double*)param_flows, node_flows, node_mars, element_mars, params,
+                                                     nids, cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows,
+                                                     B, mod, sign, stream);
+        return launch_par_batched<float, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
+                                                cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod,
+                                                sign, stream);
+    }
+    if (pf64)
+        return launch_par_batched<__bf16, double>((double*)param_flows, node_flows, node_mars, element_mars, params,
+                                                  nids, cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B,
+                                                  mod, sign, stream);
+    return launch_par_batched<__bf16, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
+                                             cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod, sign,
+                                             stream);
 }

@@ -20,6 +20,7 @@ import torch
 import torch.nn as nn
 
 from pyjuice_amd import _native as N
+from pyjuice_amd.layer.sum_layer import PRECISIONS
 from pyjuice_amd.layer import InputLayer, LayerGroup, ProdLayer, SumLayer
 from pyjuice_amd.nodes import CircuitNodes, multiply, summate
 from pyjuice_amd.nodes.nodes import dfs_postorder
@@ -92,6 +93,7 @@ class TensorCircuit(nn.Module):
         self._cum_flow = 0.0
         self._em_dev = None
         self.modify_flows_in_place = True
+        self._par_batches = {"tables": dict(), "layers": set()}
 
     # ------------------------------------------------------------------ device placement
 
@@ -120,7 +122,94 @@ class TensorCircuit(nn.Module):
                 "fuse": [(int(t), int(n), torch.tensor(ch, dtype=torch.int64, device=device))
                          for t, n, ch in self.parflow_fusing_kwargs],
             }
+            self._par_batches = self._build_par_batches(device)
         return self
+
+    def _build_par_batches(self, device):
+        """Descriptor tables of the batched parameter-flow launch: every sum layer with block size
+        >= 32 gets its parameter flows from one launch per tile shape after the whole flow
+        backward (they feed only EM), instead of one latency-bound launch per layer.  Rows follow
+        ``PJAMD_PAR_DESC_FIELDS``."""
+        sum_layers = [(gid, layer) for gid, g in enumerate(self.inner_layer_groups) if not g.is_prod() for layer in g]
+        # Tiles of different layers run concurrently here, so a layer whose parameter-flow range
+        # meets another layer's (tied nodes share slots with their source) adds atomically, as
+        # does a layer with tied nodes inside it.
+        def pf_span(layer):
+            lo, hi = 2 ** 62, -1
+            for nids, cids, pids, pfids, nch, tot in layer._dev["fw"]:
+                real = torch.arange(pfids.size(1), device=pfids.device).unsqueeze(0) < nch.unsqueeze(1)
+                if bool(real.any()):
+                    v = pfids[real]
+                    lo, hi = min(lo, int(v.min())), max(hi, int(v.max()) + layer.block_size)
+            return (lo, hi) if hi >= 0 else (0, 0)
+
+        ranges = sorted((pf_span(layer), id(layer)) for _, layer in sum_layers)
+        overlapping = set()
+        hi, hid = -1, None
+        for (r, i) in ranges:
+            if r[0] < r[1] and r[0] < hi:
+                overlapping.update((i, hid))
+            if r[1] > hi:
+                hi, hid = r[1], i
+        acc = {64: [], 32: []}
+        for gid, layer in sum_layers:
+            bs = layer.block_size
+            if bs < 32:
+                continue
+            erow = self._elem_offsets[gid // 2]
+            tm, tn = (64, 64) if bs >= 64 else (32, 128)
+            atomic = layer._pf_shared or id(layer) in overlapping
+            for nids, cids, pids, pfids, nch, tot in layer._dev["fw"]:
+                acc[tm].append((nids, cids, pids, pfids, nch, tot, bs, tn, erow, atomic))
+        batches = dict()
+        for tm, parts in acc.items():
+            if not parts:
+                continue
+            desc, tile_desc = [], []
+            n_tiles = blk_off = idx_off = 0
+            tot_rows = tot_edges = tot_edges_bs = 0
+            for d, (nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared) in enumerate(parts):
+                nb, K = nids.numel(), cids.size(1)
+                mt, kt = bs // tm, (K + tn - 1) // tn
+                tiles = nb * mt * kt
+                desc.append([n_tiles, K, mt, kt, blk_off, idx_off, erow, 1 if shared else 0])
+                tile_desc.append(torch.full([tiles], d, dtype=torch.int32))
+                n_tiles += tiles
+                blk_off += nb
+                idx_off += nb * K
+                tot_rows += nb * bs
+                tot_edges += tot
+                tot_edges_bs += tot * bs
+            assert idx_off < 2 ** 31 and n_tiles < 2 ** 31 // 256
+            cat = lambda k: torch.cat([p[k].reshape(-1) for p in parts]).to(torch.int32).to(device).contiguous()
+            batches[tm] = {
+                "nids": cat(0), "cids": cat(1), "pids": cat(2), "pfids": cat(3), "nch": cat(4),
+                "desc": torch.tensor(desc, dtype=torch.int32).reshape(-1).to(device),
+                "tile_desc": torch.cat(tile_desc).to(device), "n_tiles": n_tiles,
+                "work": (tot_rows, tot_edges, tot_edges_bs),
+            }
+        batched_layers = set()
+        for gid, g in enumerate(self.inner_layer_groups):
+            if not g.is_prod():
+                batched_layers.update(id(layer) for layer in g if layer.block_size >= 32)
+        return {"tables": batches, "layers": batched_layers}
+
+    def _run_par_batched(self, modify: bool, precision: str):
+        pf = self.param_flows
+        pf64 = pf.dtype == torch.float64
+        flags = (1 if modify else 0) | (8 if pf64 else 0)
+        prec = PRECISIONS[precision]
+        B = self.node_flows.size(1)
+        for tm, t in self._par_batches["tables"].items():
+            rows, edges, edges_bs = t["work"]
+            # algorithmic work: node-flow rows (and marginals unless modified) and child rows once
+            # per sample, parameters read and parameter flows read + written once
+            algo = (4 * (rows * (1 if modify else 2) + edges) * B + (4 + 2 * (8 if pf64 else 4)) * edges_bs,
+                    2 * edges_bs * B)
+            N.call("pjamd_sum_backward_par_batched", N.ptr(pf), N.ptr(self.node_flows), N.ptr(self.node_mars),
+                   N.ptr(self.element_mars), N.ptr(self.params), N.ptr(t["nids"]), N.ptr(t["cids"]),
+                   N.ptr(t["pids"]), N.ptr(t["pfids"]), N.ptr(t["nch"]), N.ptr(t["desc"]), N.ptr(t["tile_desc"]),
+                   t["n_tiles"], tm, B, flags, prec, N.stream(), algo=algo)
 
     def cuda(self, device=None):
         return self.to(torch.device("cuda" if device is None else f"cuda:{device}" if isinstance(device, int) else device))
@@ -246,15 +335,21 @@ class TensorCircuit(nn.Module):
         # ``allow_modify_flows`` permits the in-place log(flow) - mar transform of the sum nodes'
         # flows (one pass per layer, re-read by the element- and parameter-flow kernels); with
         # ``modify_flows_in_place = False`` the kernels compute it on the fly instead.
+        # Parameter flows of sum layers with block size >= 32 come from one batched launch after
+        # the loop (the flows and element marginals they read stay untouched until then).
         modify = allow_modify_flows and self.modify_flows_in_place
+        batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu") else set()
         for lid in range(len(self.inner_layer_groups) - 1, -1, -1):
             g = self.inner_layer_groups[lid]
             if g.is_prod():
                 g.backward(self.node_flows, self.element_flows)
-            else:
-                g.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
-                           param_flows=self.param_flows if compute_param_flows else None,
-                           allow_modify_flows=modify, precision=precision)
+                continue
+            for layer in g:
+                pf = self.param_flows if compute_param_flows and id(layer) not in batched else None
+                layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
+                               param_flows=pf, allow_modify_flows=modify, precision=precision)
+        if batched:
+            self._run_par_batched(modify, precision)
 
     def backward(self, inputs: Optional[torch.Tensor] = None, ll_weights: Optional[torch.Tensor] = None,
                  compute_param_flows: bool = True, flows_memory: float = 1.0,
