@@ -1,0 +1,231 @@
+/*
+ * pyjuice_amd — C ABI of the MI355X (gfx950) probabilistic-circuit kernels.
+ *
+ * One shared library, libpyjuice_amd.so, built from pyjuice_amd/csrc/*.hip.  Every entry point
+ * is a plain C function over device pointers and sizes; it enqueues its kernels on `stream`
+ * and returns without synchronising.  Return value: 0 on success, a positive hipError_t when
+ * a launch failed, PJAMD_EINVAL (-1) for invalid sizes / argument combinations and
+ * PJAMD_EUNSUPPORTED (-2) for a configuration the library does not implement
+ * (pjamd_error_string() names each code).
+ *
+ * Memory layout shared by all entry points (device memory, row-major):
+ *   node_mars / node_flows        float [num_nodes][B]      one row per circuit node
+ *   element_mars / element_flows  float [num_elements][B]   one row per product node
+ *   params                        float [num_params]        sum-edge parameters; the bs
+ *                                  parameters of a (node block, child) edge are contiguous
+ *   param_flows                   float or double [num_param_flows]
+ *   data                          int32 [num_vars][B]       (see pjamd_data_to_vb)
+ * Node / element row 0 .. (max block size - 1) are the dummy rows the layer compiler reserves:
+ * index 0 in any child / parent index table means "padding" and is never written.
+ *
+ * The index tables (nids, cids, pids, pfids, chids, parids, parpids, n_chs / n_pars) are the
+ * partition tensors produced by the layer compiler (pyjuice_amd/layer/compilation.py, which
+ * reproduces src/pyjuice/layer/compilation.py of PyJuice bit for bit), as int32.
+ *
+ * Each declaration cites the PyJuice function whose behaviour it replaces
+ * (paths relative to the PyJuice source tree).
+ */
+#ifndef PYJUICE_AMD_H
+#define PYJUICE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t; a null stream means the default stream. */
+typedef struct ihipStream_t* pjamd_stream_t;
+
+enum {
+    PJAMD_STATUS_OK = 0,
+    PJAMD_STATUS_EINVAL = -1,
+    PJAMD_STATUS_EUNSUPPORTED = -2
+};
+
+/* Sum-layer arithmetic (`precision` arguments). */
+enum {
+    PJAMD_PREC_FP32_MFMA = 0, /* FP32 operands on v_mfma_f32_32x32x2_f32 (block size >= 32) */
+    PJAMD_PREC_BF16_MFMA = 1, /* BF16 operands, FP32 accumulate, per-tile max subtraction      */
+    PJAMD_PREC_FP16_MFMA = 2, /* FP16 operands (parameter flows run in BF16), FP32 accumulate  */
+    PJAMD_PREC_FP32_VALU = 3  /* FP32 on the vector ALUs, any block size                      */
+};
+/* Block sizes below 32 always run on the FP32 vector path, whatever `precision` says. */
+
+/* Input distributions (`kind` arguments). */
+enum {
+    PJAMD_DIST_CATEGORICAL = 0, /* src/pyjuice/nodes/distributions/categorical.py */
+    PJAMD_DIST_BERNOULLI = 1    /* src/pyjuice/nodes/distributions/bernoulli.py   */
+};
+
+/* ----------------------------------------------------------------------------- library */
+
+/* ABI revision of this header (no PyJuice counterpart). */
+int pjamd_abi_version(void);
+
+/* Static text for a status code (no PyJuice counterpart). */
+const char* pjamd_error_string(int code);
+
+/* p[0 .. n) = v.  Replaces the buffer initialisation of TensorCircuit._init_buffer
+ * (src/pyjuice/model/tensorcircuit.py:858-878) and the flow / dummy-row fills of
+ * TensorCircuit.forward / backward (tensorcircuit.py:207-342, 344-538).  A kernel, not
+ * hipMemsetAsync, so it can be recorded into HIP graphs. */
+int pjamd_fill_f32(float* p, float v, int64_t n, pjamd_stream_t stream);
+
+/* out[v][b] = (int32) src[b][v] for a [B][V] batch of one-byte (uint8 / bool, src_code 0),
+ * int32 (src_code 1) or int64 (src_code 2) values.  Replaces `inputs.permute(1, 0)` of TensorCircuit.forward
+ * (tensorcircuit.py:226-230). */
+int pjamd_data_to_vb(int* out, int src_code, const void* src, int64_t B, int64_t V, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- input layers */
+
+/* node_mars[node_offset + n][b] = log p_n(data[vids[n]][b]) for the num_nodes nodes of one
+ * input layer; s_pids[n] is node n's first parameter, ncats[n] its category count.
+ * Replaces InputLayer.forward (src/pyjuice/layer/input_layer.py:283-404) with
+ * Categorical.fw_mar_fn (categorical.py:83-92) / Bernoulli.fw_mar_fn (bernoulli.py:65-71). */
+int pjamd_input_forward(int kind, const float* params, float* node_mars, const int* data, const int* vids,
+                        const int64_t* s_pids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
+                        pjamd_stream_t stream);
+
+/* pflows[s_pfids[n] + data[vids[n]][b]] += node_flows[node_offset + n][b] summed over b
+ * (Categorical; Bernoulli adds the flows of samples with x = 1 to slot 0 and of x = 0 to
+ * slot 1).  pf64 selects a double accumulator.  kmax_or_tied = (largest category count of the
+ * layer << 1) | 1 if several nodes share parameter-flow slots (then the adds are atomic).  Replaces InputLayer.backward
+ * (input_layer.py:406-571) with Categorical.bk_flow_fn (categorical.py:94-101) /
+ * Bernoulli.bk_flow_fn (bernoulli.py:73-77). */
+int pjamd_input_backward(int kind, void* pflows, const float* node_flows, int pf64, const int* data, const int* vids,
+                         const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
+                         int kmax_or_tied, pjamd_stream_t stream);
+
+/* Mini-batch EM of the num_src source nodes src[] of an input layer:
+ * theta <- (1 - step) theta + step * (flows + pseudocount / K) / (sum flows + pseudocount),
+ * accumulated in double.  Replaces InputLayer.mini_batch_em (input_layer.py:663-700) with
+ * Categorical.small_ncats_em_fn / large_ncats_em_fn (categorical.py:164-245) and
+ * Bernoulli.em_fn (bernoulli.py:91-101). */
+int pjamd_input_em(int kind, float* params, const void* pflows, int pf64, const int* src, const int64_t* s_pids,
+                   const int64_t* s_pfids, const int* ncats, int64_t num_src, float step, float pseudocount,
+                   int keep_zero, pjamd_stream_t stream);
+
+/* For every (node n, sample s) with node_act[node_offset + n][s] != 0, write a value of
+ * variable vids[n] drawn with the caller's uniform uniforms[node_offset + n][s]: Categorical
+ * takes the first category whose FP32 running CDF reaches u, Bernoulli takes u < theta.
+ * out is int64 (out_i64 = 1) or int32, [num_vars][S].  Replaces InputLayer.sample
+ * (input_layer.py:573-661) with Categorical.sample_fn (categorical.py:137-162) /
+ * Bernoulli.sample_fn (bernoulli.py:79-89); uniforms come from the caller instead of a seed. */
+int pjamd_input_sample(int kind, void* out, const float* params, const uint8_t* node_act, const float* uniforms,
+                       const int* vids, const int64_t* s_pids, const int* ncats, int out_i64, int64_t num_nodes,
+                       int64_t node_offset, int64_t S, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- product layers */
+
+/* out[nids[j] + i][b] (+)= sum_k in[cids[j][k] + i][b] for node block j < n_blocks, row i < bs;
+ * cids[j][k] == 0 is padding.  accum = 1 adds to out.  Replaces ProdLayer.forward
+ * (src/pyjuice/layer/prod_layer.py:160-204, kernels at 265-555). */
+int pjamd_prod_forward(float* out, const float* in, const int* nids, const int* cids, int n_blocks, int K, int bs,
+                       int64_t B, int accum, pjamd_stream_t stream);
+
+/* node_flows[u_cids[j] + i][b] += sum_k element_flows[parids[j][k] + i][b]: the backward
+ * partitions of a product layer.  Replaces ProdLayer.backward (prod_layer.py:206-261). */
+int pjamd_prod_backward(float* node_flows, const float* element_flows, const int* u_cids, const int* parids,
+                        int n_blocks, int K, int bs, int64_t B, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- sum layers */
+
+/* node_mars[nids[j] + i][b] = log sum_{k < n_chs[j]} params[pids[j][k] + i] *
+ * exp(element_mars[cids[j][k]][b]) for one forward partition (n_blocks node blocks of bs nodes,
+ * index rows of width K).  Replaces SumLayer.forward (src/pyjuice/layer/sum_layer.py:220-262)
+ * with _forward_block_sparse (sum_layer.py:802-966) and _forward_sparse (1106-1214). */
+int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
+                      const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
+                      int precision, pjamd_stream_t stream);
+
+/* node_flows[r][b] <- log(node_flows[r][b]) - node_mars[r][b] for the rows of the partition's
+ * node blocks (-inf where the marginal is -inf).  Replaces SumLayer._bk_triton_modify_flow
+ * (sum_layer.py:1355-1515). */
+int pjamd_sum_modify_flows(float* node_flows, const float* node_mars, const int* nids, int n_blocks, int bs,
+                           int64_t B, pjamd_stream_t stream);
+
+/* element_flows[chids[j] + c][b] (+)= exp(element_mars[..][b]) * sum_{k < n_pars[j], i < bs}
+ * params[parpids[j][k] + c * bs + i] * node_flows[parids[j][k] + i][b] / exp(node_mars[..][b])
+ * for the n_cblocks child blocks (cbs rows) of one backward partition.  flags bit 0: node flows
+ * already hold log(flow) - mar; bit 1: add to element_flows.  Replaces
+ * SumLayer._backward_block_sparse_ele_flows (sum_layer.py:1891-2048) and
+ * _backward_sparse_ele_flows (2668-2791). */
+int pjamd_sum_backward_ele(float* element_flows, const float* node_flows, const float* node_mars,
+                           const float* element_mars, const float* params, const int* chids, const int* parids,
+                           const int* parpids, const int* n_pars, int n_cblocks, int KP, int cbs, int bs, int64_t B,
+                           int flags, int precision, pjamd_stream_t stream);
+
+/* param_flows[pfids[j][k] + i] += sum_b params[pids[j][k] + i] * node_flows[nids[j] + i][b] *
+ * exp(element_mars[cids[j][k]][b] - node_mars[nids[j] + i][b]) for one forward partition.
+ * flags bit 0: flows hold log(flow) - mar; bit 1: slots are shared (atomic adds); bit 2: negate;
+ * bit 3: param_flows is double.  Replaces SumLayer._backward_block_sparse_par_flows
+ * (sum_layer.py:2284-2412) and _backward_sparse_par_flows (2904-3023). */
+int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const float* node_mars,
+                           const float* element_mars, const float* params, const int* nids, const int* cids,
+                           const int* pids, const int* pfids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
+                           int flags, int precision, pjamd_stream_t stream);
+
+/* Number of int32 fields per row of the `desc` table below. */
+#define PJAMD_PAR_DESC_FIELDS 8
+
+/* pjamd_sum_backward_par for many partitions (of many sum layers) in one launch, after the
+ * whole flow backward.  nids / n_chs are the partitions' node-block tables concatenated,
+ * cids / pids / pfids their index rows concatenated (each partition keeps its own width K).
+ * desc[p][0..7] = {first tile, K, m-tiles per block (bs / tile_rows), child tiles per block,
+ * block offset, index-row offset, first element row of the layer, shared-slot flag};
+ * tile_desc[t] = p for each of the n_tiles tiles.  tile_rows is 64 (block sizes >= 64, child
+ * tiles of 64) or 32 (block size 32, child tiles of 128).  element_mars is the buffer that
+ * holds every layer's element rows.  flags as pjamd_sum_backward_par (bit 1 unused).
+ * Same per-partition results as pjamd_sum_backward_par; the reference runs one launch per
+ * layer inside SumLayer.backward (sum_layer.py:264-384) under TensorCircuit.backward
+ * (tensorcircuit.py:344-538). */
+int pjamd_sum_backward_par_batched(void* param_flows, const float* node_flows, const float* node_mars,
+                                   const float* element_mars, const float* params, const int* nids, const int* cids,
+                                   const int* pids, const int* pfids, const int* n_chs, const int* desc,
+                                   const int* tile_desc, int n_tiles, int tile_rows, int64_t B, int flags,
+                                   int precision, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- EM */
+
+/* Mini-batch EM of the sum parameters.  recs is int32 [n_recs][5] = {first parameter, first
+ * parameter flow, node-block id, block size, number of children} per node block (the node
+ * blocks of compile_par_update_fn); each node's parameters become
+ * (1 - step) theta + step * (f + pseudocount / n) / (sum f + pseudocount), in double.
+ * Replaces em_par_update (src/pyjuice/model/backend/par_update.py:214-257, 296-330). */
+int pjamd_sum_em(float* params, const void* pflows, int pf64, const int* recs, int n_recs, float step,
+                 float pseudocount, int keep_zero, int max_bs, pjamd_stream_t stream);
+
+/* pflows[target + i] += sum_c pflows[copies[c] + i] for i < n: the flows of tied copies are
+ * added to their source.  Replaces compute_cum_par_flows
+ * (src/pyjuice/model/backend/parflow_fusing.py:81-120). */
+int pjamd_pflow_fuse(void* pflows, int pf64, int64_t target, int64_t n, const int64_t* copies, int ncopies,
+                     pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- sampling */
+
+/* Top-down sampling through one sum partition: for each active node (node_act != 0) and
+ * sample s, select child k = #{k : cumsum_k(theta) <= u} (clamped to the last real child) with
+ * the caller's uniform u = uniforms[node][s], and mark element_act[cids[j][k] + ...][s].
+ * Replaces sample_sum_layer (src/pyjuice/queries/sample.py:38-181). */
+int pjamd_sample_sum(uint8_t* elem_act, const uint8_t* node_act, const float* uniforms, const float* params,
+                     const int* nids, const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs,
+                     int64_t S, pjamd_stream_t stream);
+
+/* Marks every child of an active product node active.  Replaces sample_prod_layer
+ * (src/pyjuice/queries/sample.py:270-333). */
+int pjamd_sample_prod(uint8_t* node_act, const uint8_t* elem_act, const int* nids, const int* cids, int n_blocks,
+                      int K, int bs, int64_t S, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- diagnostics */
+
+/* C = A (32 x KI) * B (KI x 32) on one MFMA instruction (KI = 2 for precision 0, 16 for 1 and 2):
+ * checks the operand / accumulator lane layouts the sum kernels assume.  No PyJuice
+ * counterpart; used by the tests. */
+int pjamd_debug_mfma_probe(int precision, const float* A, const float* B, float* C, pjamd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYJUICE_AMD_H */

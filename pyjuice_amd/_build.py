@@ -13,8 +13,11 @@ OUT = os.path.join(HERE, "libpyjuice_amd.so")
 ARCH = os.environ.get("PJAMD_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops off: hipcc must not emit packed FP32 VALU instructions.
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+# Only the extern "C" entry points of include/pyjuice_amd.h are exported (-fvisibility=hidden).
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
-               "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast", "-munsafe-fp-atomics"]
+               "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast", "-munsafe-fp-atomics", "-fvisibility=hidden",
+               f"-I{INCLUDE}"]
 
 
 def _hipcc():

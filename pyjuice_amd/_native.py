@@ -50,7 +50,7 @@ _SIGNATURES = {
     "pjamd_sum_backward_ele": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
                                _i32, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_backward_par": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
-                               _i32, _i32, _i32, _i64, _i32, _i32, _i32, _c_p],
+                               _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_backward_par_batched": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
                                        _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_em": [_c_p, _c_p, _i32, _c_p, _i32, _f32, _f32, _i32, _i32, _c_p],

@@ -5,6 +5,9 @@
 #include <stdint.h>
 #include <math.h>
 
+// The public declarations: every PJAMD_EXPORT definition must match its declaration there.
+#include "pyjuice_amd.h"
+
 #define PJAMD_EXPORT extern "C" __attribute__((visibility("default")))
 
 // Status codes returned by the C ABI: 0 = success, > 0 = hipError_t, < 0 = argument errors.
@@ -13,14 +16,6 @@ enum {
     PJAMD_EINVAL = -1,      // invalid size / unsupported combination of arguments
     PJAMD_EUNSUPPORTED = -2 // feature not implemented by this library
 };
-
-// Row of the partition descriptor table of pjamd_sum_backward_par_batched (int32 fields):
-//   [0] first logical tile of the partition   [1] K (index row width)
-//   [2] m-tiles per node block                [3] child tiles per node block
-//   [4] offset of the partition's node blocks in nids / nch
-//   [5] offset of its index rows in cids / pids / pfids
-//   [6] first element row of its layer's element slab   [7] 1 if parameter-flow slots are shared
-#define PJAMD_PAR_DESC_FIELDS 8
 
 #define PJAMD_LAUNCH_CHECK()                      \
     do {                                          \

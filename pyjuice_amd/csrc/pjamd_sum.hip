@@ -567,7 +567,7 @@ static int sum_bwd_par_impl(PF* pflows, const float* node_flows, const float* no
 PJAMD_EXPORT int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const float* node_mars,
                                         const float* element_mars, const float* params, const int* nids, const int* cids,
                                         const int* pids, const int* pfids, const int* nch, int n_blocks, int K, int bs,
-                                        int64_t B, int flags, int precision, int reserved, hipStream_t stream) {
+                                        int64_t B, int flags, int precision, hipStream_t stream) {
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_blocks == 0) return PJAMD_OK;
     const bool mod = flags & 1, atomic_out = (flags & 2) != 0;

@@ -196,4 +196,4 @@ class SumLayer(Layer, nn.Module):
                         2 * tot * bs * B)
                 N.call("pjamd_sum_backward_par", N.ptr(param_flows), N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),
-                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, flags, prec, 0, s, algo=algo)
+                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, flags, prec, s, algo=algo)
