@@ -125,14 +125,19 @@ def kernel_breakdown(step, steps):
 
 
 def roofline_of(name, d, prec):
-    if name in HBM_KERNELS:
+    """Roofline of one kernel from its algorithmic bytes and flops per step: the bound is the
+    resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
+    256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
+    peak_f = PEAK_TFLOPS[prec]
+    t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
+    t_mfma = d["flops"] / (peak_f * 1e12)
+    if name in HBM_KERNELS or t_hbm >= t_mfma:
         ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
         return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
-    peak = PEAK_TFLOPS[prec]
     ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
-    return {"kernel": name, "bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(ach / peak, 4), "traffic": None}
+    return {"kernel": name, "bound": "mfma", "achieved": round(ach, 3), "peak": peak_f, "unit": "TFLOP/s",
+            "frac": round(ach / peak_f, 4), "traffic": None}
 
 
 def cpu_baseline(ns, pc, x_np, lls_by_prec, args):
