@@ -13,15 +13,14 @@ Contents
   ``pyjuice/layer/prod_layer.py:398-466``; ``pyjuice/nodes/distributions/categorical.py:83-245``,
   ``bernoulli.py:65-101``; ``pyjuice/model/backend/par_update.py:173-256``;
   ``pyjuice/queries/sample.py:38-140``).
-* ``compile_ref``: loop-based restatement of the reference compiler's CPU branches
-  (``pyjuice/layer/compilation.py``, ``pyjuice/layer/backend/node_partition.py``) that the
-  product's vectorised compiler is checked against.
-* ``pc_cpu.c``: C / OpenMP FP32 restatement of forward, flow backward and EM on the
-  compiled index tensors; the CPU baseline of ``bench.py``.
+* ``pc_cpu.c`` + ``pc_cpu.py``: C / OpenMP restatement of forward, flow backward and EM on the
+  compiled index tables, built in FP64 (an oracle fast enough for full-size circuits, pinned
+  to ``dag_eval`` by ``tests/test_oracle_cpu.py``) and in FP32 (the CPU baseline of
+  ``bench.py``) by ``build_cpu.py``.
 
 Pinning (see ``tests/test_oracle.py``): every function is checked against what the
 mathematics fixes -- exhaustive enumeration (sum of probabilities = 1), central finite
 differences of the log-likelihood (parameter flows = theta * dLL/dtheta), flow
-conservation, the closed-form EM fixed point -- and against the literal index tables
-transcribed from the reference's tests (``tests/golden/``).
+conservation, the closed-form EM fixed point.  The compiler is pinned separately against the
+literal index tables transcribed from the reference's tests (``tests/golden/``).
 """

@@ -1,0 +1,218 @@
+/*
+ * CPU restatement of the layer algorithms on the compiled index tables.  TEST INFRASTRUCTURE:
+ * only tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py load it.
+ *
+ * Built twice by oracle/build_cpu.py: with PCC_REAL=double (an FP64 oracle on the compiled
+ * circuit, fast enough for the full-size circuits) and PCC_REAL=float (the FP32 CPU baseline),
+ * both with OpenMP over node blocks.  Index tables are the compiler's int64 partition tensors;
+ * buffers are row-major [rows][B] like the device buffers.
+ *
+ * Each function follows the reference layer it is named after (paths in the PyJuice tree):
+ *   pcc_input_forward   layer/input_layer.py:283-404, categorical.py:83-92, bernoulli.py:65-71
+ *   pcc_input_backward  layer/input_layer.py:406-571, categorical.py:94-101, bernoulli.py:73-77
+ *   pcc_input_em        layer/input_layer.py:663-724, categorical.py:164-245, bernoulli.py:91-101
+ *   pcc_prod            layer/prod_layer.py:398-466 (forward and flow backward gather-add)
+ *   pcc_sum_forward     layer/sum_layer.py:454-568
+ *   pcc_sum_ele         layer/sum_layer.py:1569-1730 (element flows, unmodified node flows)
+ *   pcc_sum_par         layer/sum_layer.py:2049-2169 (parameter flows)
+ *   pcc_sum_em          model/backend/par_update.py:173-256
+ *   pcc_pflow_fuse      model/backend/parflow_fusing.py:81-112
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#ifndef PCC_REAL
+#define PCC_REAL double
+#endif
+typedef PCC_REAL real;
+
+#define NEG_INF (-INFINITY)
+
+/* node_mars[off + i][b] = log p_i(x[vids[i]][b]) */
+void pcc_input_forward(int kind, const real* params, real* node_mars, const int32_t* data, const int64_t* vids,
+                       const int64_t* s_pids, const int64_t* ncats, int64_t num_nodes, int64_t off, int64_t B) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < num_nodes; ++i) {
+        const int32_t* x = data + vids[i] * B;
+        real* out = node_mars + (off + i) * B;
+        for (int64_t b = 0; b < B; ++b) {
+            if (kind == 0) {
+                const int64_t k = x[b];
+                out[b] = (k >= 0 && k < ncats[i]) ? log(params[s_pids[i] + k]) : NEG_INF;
+            } else {
+                const real p = params[s_pids[i]];
+                out[b] = log(x[b] > 0 ? p : (real)1 - p);
+            }
+        }
+    }
+}
+
+/* Categorical: pf[s_pfids[i] + x] += flow; Bernoulli: slot 0 collects x = 1, slot 1 x = 0. */
+void pcc_input_backward(int kind, real* pflows, const real* node_flows, const int32_t* data, const int64_t* vids,
+                        const int64_t* s_pfids, const int64_t* ncats, int64_t num_nodes, int64_t off, int64_t B) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < num_nodes; ++i) {
+        const int32_t* x = data + vids[i] * B;
+        const real* f = node_flows + (off + i) * B;
+        for (int64_t b = 0; b < B; ++b) {
+            int64_t slot;
+            if (kind == 0) {
+                if (x[b] < 0 || x[b] >= ncats[i]) continue;
+                slot = s_pfids[i] + x[b];
+            } else {
+                slot = s_pfids[i] + (x[b] > 0 ? 0 : 1);
+            }
+#pragma omp atomic
+            pflows[slot] += f[b];
+        }
+    }
+}
+
+/* theta <- (1 - s) theta + s (f + pc / K) / (sum f + pc) for the source nodes src[]. */
+void pcc_input_em(int kind, real* params, const real* pflows, const int64_t* src, const int64_t* s_pids,
+                  const int64_t* s_pfids, const int64_t* ncats, int64_t num_src, double step, double pc) {
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < num_src; ++w) {
+        const int64_t i = src[w];
+        if (kind == 0) {
+            const int64_t K = ncats[i];
+            double cum = pc;
+            for (int64_t k = 0; k < K; ++k) cum += pflows[s_pfids[i] + k];
+            for (int64_t k = 0; k < K; ++k) {
+                const double nv = ((double)pflows[s_pfids[i] + k] + pc / (double)K) / cum;
+                params[s_pids[i] + k] = (real)((1.0 - step) * params[s_pids[i] + k] + step * nv);
+            }
+        } else {
+            const double f1 = pflows[s_pfids[i]], f0 = pflows[s_pfids[i] + 1];
+            const double nv = (f1 + pc / 2.0) / (f1 + f0 + pc);
+            params[s_pids[i]] = (real)((1.0 - step) * params[s_pids[i]] + step * nv);
+        }
+    }
+}
+
+/* out[nids[j] + r][b] (+)= sum_k in[cids[j][k] + r][b], cids == 0 is padding. */
+void pcc_prod(real* out, const real* in, const int64_t* nids, const int64_t* cids, int64_t n_blocks, int64_t K,
+              int64_t bs, int64_t B, int accum) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n_blocks * bs; ++t) {
+        const int64_t j = t / bs, r = t % bs;
+        real* o = out + (nids[j] + r) * B;
+        if (!accum)
+            for (int64_t b = 0; b < B; ++b) o[b] = 0;
+        for (int64_t k = 0; k < K; ++k) {
+            const int64_t c = cids[j * K + k];
+            if (c == 0) continue;
+            const real* src = in + (c + r) * B;
+            for (int64_t b = 0; b < B; ++b) o[b] += src[b];
+        }
+    }
+}
+
+/* node_mars[nids[j] + r][b] = log sum_k theta[pids[j][k] + r] exp(e[cids[j][k]][b]) over the
+ * children k < nch[j], with the per-(node, sample) maximum subtracted. */
+void pcc_sum_forward(real* node_mars, const real* emars, const real* params, const int64_t* nids,
+                     const int64_t* cids, const int64_t* pids, const int64_t* nch, int64_t n_blocks, int64_t K,
+                     int64_t bs, int64_t B) {
+#pragma omp parallel
+    {
+        real* mx = (real*)malloc(sizeof(real) * B);
+        real* acc = (real*)malloc(sizeof(real) * B);
+#pragma omp for schedule(static)
+        for (int64_t t = 0; t < n_blocks * bs; ++t) {
+            const int64_t j = t / bs, r = t % bs;
+            const int64_t n = nch[j];
+            for (int64_t b = 0; b < B; ++b) { mx[b] = NEG_INF; acc[b] = 0; }
+            for (int64_t k = 0; k < n; ++k) {
+                const real* e = emars + cids[j * K + k] * B;
+                for (int64_t b = 0; b < B; ++b) if (e[b] > mx[b]) mx[b] = e[b];
+            }
+            for (int64_t k = 0; k < n; ++k) {
+                const real* e = emars + cids[j * K + k] * B;
+                const real th = params[pids[j * K + k] + r];
+                for (int64_t b = 0; b < B; ++b)
+                    if (mx[b] != NEG_INF) acc[b] += th * exp(e[b] - mx[b]);
+            }
+            real* o = node_mars + (nids[j] + r) * B;
+            for (int64_t b = 0; b < B; ++b) o[b] = (acc[b] > 0 && mx[b] != NEG_INF) ? log(acc[b]) + mx[b] : NEG_INF;
+        }
+        free(mx);
+        free(acc);
+    }
+}
+
+/* eflows[chids[j] + c][b] (+)= sum_{k < npar[j], i < bs} theta[parpids[j][k] + c bs + i]
+ *                              * f[parids[j][k] + i][b] * exp(e[chids[j] + c][b] - m[parids[j][k] + i][b]) */
+void pcc_sum_ele(real* eflows, const real* nflows, const real* nmars, const real* emars, const real* params,
+                 const int64_t* chids, const int64_t* parids, const int64_t* parpids, const int64_t* npar, int64_t n,
+                 int64_t KP, int64_t cbs, int64_t bs, int64_t B, int accum) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n * cbs; ++t) {
+        const int64_t j = t / cbs, c = t % cbs;
+        const int64_t row = chids[j] + c;
+        const real* e = emars + row * B;
+        real* o = eflows + row * B;
+        if (!accum)
+            for (int64_t b = 0; b < B; ++b) o[b] = 0;
+        for (int64_t k = 0; k < npar[j]; ++k) {
+            for (int64_t i = 0; i < bs; ++i) {
+                const real th = params[parpids[j * KP + k] + c * bs + i];
+                const real* f = nflows + (parids[j * KP + k] + i) * B;
+                const real* m = nmars + (parids[j * KP + k] + i) * B;
+                for (int64_t b = 0; b < B; ++b)
+                    if (f[b] != 0 && e[b] != NEG_INF) o[b] += th * f[b] * exp(e[b] - m[b]);
+            }
+        }
+    }
+}
+
+/* pflows[pfids[j][k] + r] += theta[pids[j][k] + r] * sum_b f[nids[j] + r][b]
+ *                            * exp(e[cids[j][k]][b] - m[nids[j] + r][b])  (atomic: slots may be shared) */
+void pcc_sum_par(real* pflows, const real* nflows, const real* nmars, const real* emars, const real* params,
+                 const int64_t* nids, const int64_t* cids, const int64_t* pids, const int64_t* pfids,
+                 const int64_t* nch, int64_t n_blocks, int64_t K, int64_t bs, int64_t B) {
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n_blocks * bs; ++t) {
+        const int64_t j = t / bs, r = t % bs;
+        const real* f = nflows + (nids[j] + r) * B;
+        const real* m = nmars + (nids[j] + r) * B;
+        for (int64_t k = 0; k < nch[j]; ++k) {
+            const real* e = emars + cids[j * K + k] * B;
+            real s = 0;
+            for (int64_t b = 0; b < B; ++b)
+                if (f[b] != 0 && e[b] != NEG_INF) s += f[b] * exp(e[b] - m[b]);
+            const real v = params[pids[j * K + k] + r] * s;
+#pragma omp atomic
+            pflows[pfids[j * K + k] + r] += v;
+        }
+    }
+}
+
+/* Sum EM over node blocks rec = {par_start, pflow_start, node_start, bs, nch}: the parameter of
+ * child c of node i is at par_start + c bs + i. */
+void pcc_sum_em(real* params, const real* pflows, const int64_t* recs, int64_t n_recs, double step, double pc) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < n_recs; ++q) {
+        const int64_t* R = recs + q * 5;
+        const int64_t par0 = R[0], pf0 = R[1], bs = R[3], nch = R[4];
+        for (int64_t i = 0; i < bs; ++i) {
+            double cum = pc;
+            for (int64_t c = 0; c < nch; ++c) cum += pflows[pf0 + c * bs + i];
+            for (int64_t c = 0; c < nch; ++c) {
+                const int64_t o = c * bs + i;
+                const double nv = ((double)pflows[pf0 + o] + pc / (double)nch) / cum;
+                params[par0 + o] = (real)((1.0 - step) * params[par0 + o] + step * nv);
+            }
+        }
+    }
+}
+
+/* pf[target + j] = sum_c pf[copies[c] + j] (the copies include the target). */
+void pcc_pflow_fuse(real* pflows, int64_t target, int64_t n, const int64_t* copies, int64_t ncopies) {
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) {
+        real s = 0;
+        for (int64_t c = 0; c < ncopies; ++c) s += pflows[copies[c] + j];
+        pflows[target + j] = s;
+    }
+}
