@@ -28,6 +28,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0}
+# HBM bytes per launch (MB) measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload,
+# scaled by the counters' reading of a 1 GiB copy; commands and table in
+# profiles/hclt784_L256_B512_bf16_hbm_traffic.txt.  Reported as roofline.traffic for the dominant
+# kernel (counters cannot be read from inside the timed process).
+TRAFFIC_MB_PER_LAUNCH = {
+    "pjamd_sum_backward_ele": 20.76, "pjamd_sum_forward": 15.51, "pjamd_sum_backward_par_batched": 1330.06,
+    "pjamd_prod_forward": 16.39, "pjamd_prod_backward": 33.62, "pjamd_input_backward": 1090.38,
+    "pjamd_input_forward": 843.47, "pjamd_sum_em": 854.16, "pjamd_input_em": 847.78,
+    "pjamd_sum_modify_flows": 16.14,
+}
 HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
                "pjamd_input_em", "pjamd_sum_em", "pjamd_sum_modify_flows", "pjamd_pflow_fuse")
 
@@ -50,6 +60,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=32)
     ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="profiling aid: run one device-to-device copy of this many MiB first, the known-size "
+                         "reference for hardware byte counters")
     return ap.parse_args()
 
 
@@ -125,20 +138,25 @@ def kernel_breakdown(step, steps):
     return s
 
 
-def roofline_of(name, d, prec):
+def roofline_of(name, d, prec, args):
     """Roofline of one kernel from its algorithmic bytes and flops per step: the bound is the
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
+    headline = args.latents == 256 and args.batch == 512 and not args.block_size and prec == "bf16"
+    traffic = TRAFFIC_MB_PER_LAUNCH.get(name) if headline else None
+    traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
     t_mfma = d["flops"] / (peak_f * 1e12)
     if name in HBM_KERNELS or t_hbm >= t_mfma:
         ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
         return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1))}
     ach = d["flops"] / (d["ms"] * 1e-3) / 1e12
     return {"kernel": name, "bound": "mfma", "achieved": round(ach, 3), "peak": peak_f, "unit": "TFLOP/s",
-            "frac": round(ach / peak_f, 4), "traffic": None}
+            "frac": round(ach / peak_f, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1))}
 
 
 def _host_threads():
@@ -195,6 +213,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs the HIP device"
     dev = torch.device("cuda:0")
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy} MiB ({n} floats) done")
+        del src, dst
     ns, pc, xb, x_np = build_circuit(args, dev)
 
     # fp32_valu: the plain FP32 HIP form (vector ALUs); fp32: FP32 operands on the matrix cores
@@ -228,7 +255,7 @@ def main():
     r = results[main_prec]
     kern = r["kernels"]
     dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
-    roof = roofline_of(dom[0], dom[1], main_prec)
+    roof = roofline_of(dom[0], dom[1], main_prec, args)
 
     cpu, acc = (None, None)
     if not args.no_cpu_baseline and not args.only:
