@@ -1,60 +1,97 @@
 // Product layers: gather-add over child rows.  Forward: element_mars[nid + j] = sum_k
 // node_mars[cid_k + j]; backward (accum = 1): node_flows[u + j] += sum_p element_flows[par_p + j].
-// Index 0 marks padding and is skipped.  HBM-bound: rows are read / written as float4 along
-// the batch dimension when B is a multiple of 4.
+// Index 0 marks padding and is skipped.
+//
+// HBM-bound.  One wavefront owns one output row and 64 float4 (256 batch columns) of it: the
+// row's K child indices are uniform across the wavefront (scalar loads), and all K child
+// loads are issued before the first is consumed, so a launch costs two dependent memory round
+// trips (indices, then data) whatever K is.  K is a template parameter for the padded widths
+// the compiler produces (powers of two up to 16); wider rows take a runtime loop.
 #include "pjamd_common.h"
 
 using namespace pjamd;
 
 namespace {
 
-constexpr int kRows = 4;  // rows (j offsets inside a block) handled by one workgroup
+constexpr int kWaves = 4;  // rows per workgroup (one per wavefront)
 
-template <int VEC, bool ACCUM>
+template <int KT, bool VEC, bool ACCUM>
 __global__ __launch_bounds__(256) void gather_add_kernel(float* __restrict__ out, const float* __restrict__ in,
                                                          const int* __restrict__ nids, const int* __restrict__ cids,
-                                                         int K, int bs, int64_t B) {
-    // blockIdx.x -> (block j, row group); blockIdx.y -> batch chunk of 256 * VEC.
-    const int groups = (bs + kRows - 1) / kRows;
-    const int j = blockIdx.x / groups;
-    const int r0 = (blockIdx.x % groups) * kRows;
-    const int64_t b = ((int64_t)blockIdx.y * blockDim.x + threadIdx.x) * VEC;
+                                                         int K, int lbs, int64_t rows, int64_t B) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * kWaves + w;
+    if (row >= rows) return;
+    const int64_t j = row >> lbs;
+    const int r = (int)(row & ((1 << lbs) - 1));
+    constexpr int PER = VEC ? 4 : 1;
+    const int64_t b = ((int64_t)blockIdx.y * 64 + lane) * PER;
     if (b >= B) return;
-    const int* crow = cids + (int64_t)j * K;
-    const int64_t n0 = nids[j];
-    using V = typename std::conditional<VEC == 4, float4, float>::type;
+    const int* crow = cids + j * K;
+    float* dst = out + ((int64_t)nids[j] + r) * B + b;
+    const int kn = KT > 0 ? KT : K;
+
+    if (VEC) {
+        float4 acc = ACCUM ? *reinterpret_cast<const float4*>(dst) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (KT > 0) {
+            float4 v[KT > 0 ? KT : 1];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-        const int row = r0 + r;
-        if (row >= bs) break;
-        float acc[VEC];
+            for (int k = 0; k < (KT > 0 ? KT : 1); ++k) {
+                const int c = crow[k];
+                v[k] = c ? *reinterpret_cast<const float4*>(in + ((int64_t)c + r) * B + b) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
 #pragma unroll
-        for (int q = 0; q < VEC; ++q) acc[q] = 0.0f;
-        for (int k = 0; k < K; ++k) {
-            const int c = crow[k];
-            if (c == 0) continue;
-            const float* src = in + ((int64_t)c + row) * B + b;
-            if (VEC == 4) {
-                const float4 v = *reinterpret_cast<const float4*>(src);
-                acc[0] += v.x; acc[1] += v.y; acc[2] += v.z; acc[3] += v.w;
-            } else {
-                acc[0] += *src;
+            for (int k = 0; k < (KT > 0 ? KT : 1); ++k) {
+                acc.x += v[k].x; acc.y += v[k].y; acc.z += v[k].z; acc.w += v[k].w;
             }
-        }
-        float* dst = out + (n0 + row) * B + b;
-        if (VEC == 4) {
-            float4 o;
-            if (ACCUM) {
-                o = *reinterpret_cast<float4*>(dst);
-                o.x += acc[0]; o.y += acc[1]; o.z += acc[2]; o.w += acc[3];
-            } else {
-                o = make_float4(acc[0], acc[1], acc[2], acc[3]);
-            }
-            *reinterpret_cast<float4*>(dst) = o;
         } else {
-            if (ACCUM) *dst += acc[0]; else *dst = acc[0];
+            for (int k = 0; k < kn; ++k) {
+                const int c = crow[k];
+                if (c == 0) continue;
+                const float4 v = *reinterpret_cast<const float4*>(in + ((int64_t)c + r) * B + b);
+                acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+            }
         }
+        *reinterpret_cast<float4*>(dst) = acc;
+    } else {
+        float acc = ACCUM ? *dst : 0.0f;
+        if (KT > 0) {
+            float v[KT > 0 ? KT : 1];
+#pragma unroll
+            for (int k = 0; k < (KT > 0 ? KT : 1); ++k) {
+                const int c = crow[k];
+                v[k] = c ? in[((int64_t)c + r) * B + b] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < (KT > 0 ? KT : 1); ++k) acc += v[k];
+        } else {
+            for (int k = 0; k < kn; ++k) {
+                const int c = crow[k];
+                if (c != 0) acc += in[((int64_t)c + r) * B + b];
+            }
+        }
+        *dst = acc;
     }
+}
+
+template <bool VEC, bool ACCUM>
+void launch_gather(float* out, const float* in, const int* nids, const int* cids, int K, int lbs, int64_t rows,
+                   int64_t B, dim3 grid, hipStream_t stream) {
+    switch (K) {
+        case 1: gather_add_kernel<1, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+        case 2: gather_add_kernel<2, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+        case 4: gather_add_kernel<4, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+        case 8: gather_add_kernel<8, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+        case 16: gather_add_kernel<16, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+        default: gather_add_kernel<0, VEC, ACCUM><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, lbs, rows, B); break;
+    }
+}
+
+int log2_exact(int v) {
+    if (v < 1 || (v & (v - 1)) != 0) return -1;
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
 }
 
 }  // namespace
@@ -62,17 +99,21 @@ __global__ __launch_bounds__(256) void gather_add_kernel(float* __restrict__ out
 PJAMD_EXPORT int pjamd_prod_forward(float* out, const float* in, const int* nids, const int* cids, int n_blocks, int K,
                                     int bs, int64_t B, int accum, hipStream_t stream) {
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0);
+    const int lbs = log2_exact(bs);
+    PJAMD_REQUIRE(lbs >= 0);
     if (n_blocks == 0) return PJAMD_OK;
-    const int groups = (bs + kRows - 1) / kRows;
+    const int64_t rows = (int64_t)n_blocks * bs;
     const bool vec = (B % 4 == 0) && (((uintptr_t)out | (uintptr_t)in) % 16 == 0);
-    const int64_t lanes = vec ? B / 4 : B;
-    dim3 grid((unsigned)((int64_t)n_blocks * groups), cdiv(lanes, 256));
+    const int64_t per_wave = vec ? 256 : 64;
+    const int64_t gx = cdiv(rows, kWaves);
+    PJAMD_REQUIRE(gx < (int64_t)1 << 31);
+    dim3 grid((unsigned)gx, (unsigned)cdiv(B, per_wave));
     if (vec) {
-        if (accum) gather_add_kernel<4, true><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, bs, B);
-        else gather_add_kernel<4, false><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, bs, B);
+        if (accum) launch_gather<true, true>(out, in, nids, cids, K, lbs, rows, B, grid, stream);
+        else launch_gather<true, false>(out, in, nids, cids, K, lbs, rows, B, grid, stream);
     } else {
-        if (accum) gather_add_kernel<1, true><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, bs, B);
-        else gather_add_kernel<1, false><<<grid, 256, 0, stream>>>(out, in, nids, cids, K, bs, B);
+        if (accum) launch_gather<false, true>(out, in, nids, cids, K, lbs, rows, B, grid, stream);
+        else launch_gather<false, false>(out, in, nids, cids, K, lbs, rows, B, grid, stream);
     }
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;

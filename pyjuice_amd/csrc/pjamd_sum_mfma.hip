@@ -131,10 +131,13 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
     float ev[PB], av[PA];
+    // Index rows are staged by their padded width K, so the first chunk's loads do not wait
+    // for the child count n.
     for (int cb = 0; cb < n; cb += KCAP) {
         const int cn = (n - cb < KCAP) ? n - cb : KCAP;
+        const int ck = (K - cb < KCAP) ? K - cb : KCAP;
         __syncthreads();
-        for (int u = tid; u < cn; u += 256) {
+        for (int u = tid; u < ck; u += 256) {
             sc[u] = crow[cb + u];
             sp[u] = prow[cb + u];
         }
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 // over the flattened (parent block, parent row) index kk -> (kk >> lbs, kk & (bs - 1)) with a
 // running max of log(f / m).  Parameters of one child row are contiguous along the parent row,
 // so the parameter tile is loaded with lanes along kk.
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, bool ONE_BLOCK>
 __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* __restrict__ nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ chids,
@@ -234,16 +237,36 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
     float lv[PB], av[PA];
+    // The element marginals the epilogue needs do not depend on the reduction: load them now.
+    const int64_t b = b0 + mycol;
+    const int64_t obase = (int64_t)chids[j] + m0 + wm * 32;
+    float epre[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
     for (int pb = 0; pb < np; pb += KCAP) {
         const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
+        const int ckp = (KP - pb < KCAP) ? KP - pb : KCAP;
         const int kf = cnp << lbs;
         __syncthreads();
-        for (int u = tid; u < cnp; u += 256) {
+        for (int u = tid; u < ckp; u += 256) {
             spar[u] = parrow[pb + u];
             spp[u] = pprow[pb + u];
         }
         __syncthreads();
         auto load = [&](int k0) {
+            // ONE_BLOCK (bs >= TK, both powers of two): a stage of TK reduction steps lies inside
+            // one parent block, so its rows and parameters are contiguous runs.
+            if (ONE_BLOCK) {
+                const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
+                const int64_t rbase = ((int64_t)spar[kb] + k_in + bk0) * B + bb;
+#pragma unroll
+                for (int q = 0; q < PB; ++q)
+                    lv[q] = bok ? lfdm_at(nflows, nmars, rbase + (int64_t)q * B, modified) : kNegInf;
+                const int64_t base = (int64_t)spp[kb] + (int64_t)(m0 + ar0) * bs + k_in + ak;
+#pragma unroll
+                for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
+                return;
+            }
 #pragma unroll
             for (int q = 0; q < PB; ++q) {
                 const int kk = k0 + bk0 + q;
@@ -292,14 +315,12 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
             acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
         }
     }
-    const int64_t b = b0 + mycol;
     if (b < B) {
         const float m = mrun[mycol];
-        const int64_t base = (int64_t)chids[j] + m0 + wm * 32;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int64_t o = (base + acc_row(r, lane)) * B + b;
-            const float e = emars[o];
+            const int64_t o = (obase + acc_row(r, lane)) * B + b;
+            const float e = epre[r];
             float v = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
             if (accumulate) v += eflows[o];
             eflows[o] = v;
@@ -497,14 +518,24 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n * mtiles * btiles;
     PJAMD_REQUIRE(lbs >= 0 && cbs % TM == 0 && grid_fits(tiles));
-    if (cbs >= 64)
-        sum_ele_mfma<T, 64, 64><<<(unsigned)tiles, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
-                                                                     parpids, npar, KP, lbs, B, mtiles, (int)btiles,
-                                                                     mod, accum);
-    else
-        sum_ele_mfma<T, 32, 128><<<(unsigned)tiles, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids,
-                                                                      parids, parpids, npar, KP, lbs, B, mtiles,
-                                                                      (int)btiles, mod, accum);
+    const unsigned g = (unsigned)tiles;
+    const int bt = (int)btiles;
+    if (cbs >= 64) {
+        if (bs >= TK)
+            sum_ele_mfma<T, 64, 64, true><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
+                                                                 parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+        else
+            sum_ele_mfma<T, 64, 64, false><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
+                                                                  parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+    } else {
+        if (bs >= TK)
+            sum_ele_mfma<T, 32, 128, true><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
+                                                                  parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+        else
+            sum_ele_mfma<T, 32, 128, false><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids,
+                                                                   parids, parpids, npar, KP, lbs, B, mtiles, bt, mod,
+                                                                   accum);
+    }
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
