@@ -133,11 +133,15 @@ int pjamd_prod_backward(float* node_flows, const float* element_flows, const int
 
 /* node_mars[nids[j] + i][b] = log sum_{k < n_chs[j]} params[pids[j][k] + i] *
  * exp(element_mars[cids[j][k]][b]) for one forward partition (n_blocks node blocks of bs nodes,
- * index rows of width K).  Replaces SumLayer.forward (src/pyjuice/layer/sum_layer.py:220-262)
+ * index rows of width K).  child_run is a layout promise (0 = none): every child_run-aligned
+ * group of index entries names consecutive element rows, cids[j][k0 + i] = cids[j][k0] + i, with
+ * parameters bs apart, pids[j][k0 + i] = pids[j][k0] + i * bs; the compiler's index rows keep it
+ * for child blocks of that size (PyJuice's block-sparse kernels read them the same way through
+ * cids_start / cids_increment).  Replaces SumLayer.forward (src/pyjuice/layer/sum_layer.py:220-262)
  * with _forward_block_sparse (sum_layer.py:802-966) and _forward_sparse (1106-1214). */
 int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
                       const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
-                      int precision, pjamd_stream_t stream);
+                      int child_run, int precision, pjamd_stream_t stream);
 
 /* node_flows[r][b] <- log(node_flows[r][b]) - node_mars[r][b] for the rows of the partition's
  * node blocks (-inf where the marginal is -inf).  Replaces SumLayer._bk_triton_modify_flow
@@ -155,6 +159,21 @@ int pjamd_sum_backward_ele(float* element_flows, const float* node_flows, const 
                            const float* element_mars, const float* params, const int* chids, const int* parids,
                            const int* parpids, const int* n_pars, int n_cblocks, int KP, int cbs, int bs, int64_t B,
                            int flags, int precision, pjamd_stream_t stream);
+
+/* pjamd_sum_backward_ele fused with the product-layer backward below it: instead of element flows,
+ * the flow of each product row is written straight into the node flows of the product's
+ * children.  Valid when every child node has that product as its only parent (a product passes
+ * its flow unchanged to each child, so no accumulation is needed) and the children are blocks of
+ * the product block's size (cbs).  ch_nodes is int32 [n_cblocks][Kc]: for the product block
+ * chids[j], 2 * (first node of its k-th child block) + 1 if that child's flows are to be stored as
+ * log(flow) - mar, 0 = padding.  flags bit 0: the parents' flows hold log(flow) - mar; bit 1:
+ * honour the log(flow) - mar marks of ch_nodes.  Replaces SumLayer._backward_block_sparse_ele_flows
+ * (sum_layer.py:1891-2048) followed by ProdLayer.backward (prod_layer.py:206-261) and, for the
+ * children, SumLayer._bk_triton_modify_flow (sum_layer.py:1439-1515). */
+int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars, const float* element_mars,
+                                       const float* params, const int* chids, const int* parids, const int* parpids,
+                                       const int* n_pars, const int* ch_nodes, int Kc, int n_cblocks, int KP, int cbs,
+                                       int bs, int64_t B, int flags, int precision, pjamd_stream_t stream);
 
 /* param_flows[pfids[j][k] + i] += sum_b params[pids[j][k] + i] * node_flows[nids[j] + i][b] *
  * exp(element_mars[cids[j][k]][b] - node_mars[nids[j] + i][b]) for one forward partition.

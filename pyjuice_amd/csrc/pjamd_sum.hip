@@ -457,7 +457,7 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__
 // >= 32); 3 = FP32 on the vector ALUs (this file).  Smaller blocks always use this file's kernels.
 int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const float* params, const int* nids,
                            const int* cids, const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B,
-                           int precision, hipStream_t stream);
+                           int child_run, int precision, hipStream_t stream);
 int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* chids, const int* parids, const int* parpids,
                                 const int* npar, int n, int KP, int cbs, int bs, int64_t B, bool mod, bool accum,
@@ -469,12 +469,12 @@ int pjamd_sum_backward_par_mfma(void* pflows, bool pf64, const float* nflows, co
 
 PJAMD_EXPORT int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
                                    const int* cids, const int* pids, const int* nch, int n_blocks, int K, int bs,
-                                   int64_t B, int precision, hipStream_t stream) {
-    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
+                                   int64_t B, int child_run, int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3 && child_run >= 0);
     if (n_blocks == 0) return PJAMD_OK;
     if (precision != 3 && bs >= 32)
         return pjamd_sum_forward_mfma(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B,
-                                      precision, stream);
+                                      child_run, precision, stream);
     const unsigned gy = cdiv(B, kTN);
     if (bs >= 64) {
         sum_fwd_tiled<64><<<dim3((unsigned)((int64_t)n_blocks * (bs / 64)), gy), kThreads, 0, stream>>>(

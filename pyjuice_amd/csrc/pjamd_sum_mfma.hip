@@ -95,9 +95,15 @@ constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per ch
 
 // ---------------------------------------------------------------------------------------------
 // Forward.  Tile = (node block j, TM parent rows) x TN batch columns; logical tile order is
-// m-tile fastest, then batch tile, then node block.  The child index rows are staged in LDS; the
-// column max is a running max (accumulators are rescaled when it grows).
-template <typename T, int TM, int TN>
+// m-tile fastest, then batch tile, then node block.  The column max is a running max
+// (accumulators are rescaled when it grows).
+//
+// RUN: the caller promises that every TK-aligned group of index entries of a row names
+// consecutive element rows (cids[k0 + i] = cids[k0] + i) whose parameters are bs apart
+// (pids[k0 + i] = pids[k0] + i bs), which is how the compiler lays out a child block of size
+// >= TK.  A stage then needs two indices instead of TK, read straight from memory; otherwise
+// the index rows are staged through LDS.
+template <typename T, int TM, int TN, bool RUN>
 __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ nids,
                                                     const int* __restrict__ cids, const int* __restrict__ pids,
@@ -110,13 +116,14 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     constexpr int PA = TK * TM / 256;              // parameter tile: PA consecutive k per thread
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ int sc[KCAP], sp[KCAP];
+    __shared__ int sc[RUN ? 1 : KCAP], sp[RUN ? 1 : KCAP];
     __shared__ float red[G][TN], mrun[TN], msc[TN];
 
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int m0 = (t % mtiles) * TM;
     const int j = t / (mtiles * btiles);
     const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = mtiles * TM;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
     const int n = nch[j];
@@ -131,57 +138,82 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
     float ev[PB], av[PA];
-    // Index rows are staged by their padded width K, so the first chunk's loads do not wait
-    // for the child count n.
-    for (int cb = 0; cb < n; cb += KCAP) {
-        const int cn = (n - cb < KCAP) ? n - cb : KCAP;
-        const int ck = (K - cb < KCAP) ? K - cb : KCAP;
+
+    // One stage of the reduction over children [k0, k0 + TK) of the current index chunk.
+    auto process = [&](int k0, int cn, auto&& load) {
+        float lm = kNegInf;
+#pragma unroll
+        for (int q = 0; q < PB; ++q) lm = fmaxf(lm, ev[q]);
+        red[bg][bcol] = lm;
         __syncthreads();
-        for (int u = tid; u < ck; u += 256) {
-            sc[u] = crow[cb + u];
-            sp[u] = prow[cb + u];
+        if (tid < TN) {
+            float v = red[0][tid];
+#pragma unroll
+            for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
+            const float mo = mrun[tid];
+            const float mn = fmaxf(mo, v);
+            msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
+            mrun[tid] = mn;
         }
         __syncthreads();
+        const float mn = mrun[bcol];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(ev[q] - mn));
+#pragma unroll
+        for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < cn) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+    };
+
+    if (RUN) {
         auto load = [&](int k0) {
+            const int64_t c0 = crow[k0], p0 = prow[k0];
+            const float* ep = emars + (c0 + bk0) * B + (bok ? bb : 0);
+            const float* pp = params + p0 + (int64_t)ak0 * bs + m0 + arow;
+            if (k0 + TK <= n && bok) {   // full stage: no per-element predicates
 #pragma unroll
-            for (int q = 0; q < PB; ++q) {
-                const int k = k0 + bk0 + q;
-                ev[q] = (k < cn && bok) ? emars[(int64_t)sc[k] * B + bb] : kNegInf;
-            }
+                for (int q = 0; q < PB; ++q) ev[q] = ep[(int64_t)q * B];
 #pragma unroll
-            for (int q = 0; q < PA; ++q) {
-                const int k = k0 + ak0 + q;
-                av[q] = (k < cn) ? params[(int64_t)sp[k] + m0 + arow] : 0.0f;
+                for (int q = 0; q < PA; ++q) av[q] = pp[(int64_t)q * bs];
+            } else {
+#pragma unroll
+                for (int q = 0; q < PB; ++q) ev[q] = (k0 + bk0 + q < n && bok) ? ep[(int64_t)q * B] : kNegInf;
+#pragma unroll
+                for (int q = 0; q < PA; ++q) av[q] = (k0 + ak0 + q < n) ? pp[(int64_t)q * bs] : 0.0f;
             }
         };
         load(0);
-        for (int k0 = 0; k0 < cn; k0 += TK) {
-            float lm = kNegInf;
-#pragma unroll
-            for (int q = 0; q < PB; ++q) lm = fmaxf(lm, ev[q]);
-            red[bg][bcol] = lm;
+        for (int k0 = 0; k0 < n; k0 += TK) process(k0, n, load);
+    } else {
+        // Index rows are staged by their padded width K, so the first chunk's loads do not wait
+        // for the child count n.
+        for (int cb = 0; cb < n; cb += KCAP) {
+            const int cn = (n - cb < KCAP) ? n - cb : KCAP;
+            const int ck = (K - cb < KCAP) ? K - cb : KCAP;
             __syncthreads();
-            if (tid < TN) {
-                float v = red[0][tid];
-#pragma unroll
-                for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
-                const float mo = mrun[tid];
-                const float mn = fmaxf(mo, v);
-                msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
-                mrun[tid] = mn;
+            for (int u = tid; u < ck; u += 256) {
+                sc[u] = crow[cb + u];
+                sp[u] = prow[cb + u];
             }
             __syncthreads();
-            const float mn = mrun[bcol];
+            auto load = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(ev[q] - mn));
+                for (int q = 0; q < PB; ++q) {
+                    const int k = k0 + bk0 + q;
+                    ev[q] = (k < cn && bok) ? emars[(int64_t)sc[k] * B + bb] : kNegInf;
+                }
 #pragma unroll
-            for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
-            const float s = msc[mycol];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] *= s;
-            __syncthreads();
-            if (k0 + TK < cn) load(k0 + TK);
-            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+                for (int q = 0; q < PA; ++q) {
+                    const int k = k0 + ak0 + q;
+                    av[q] = (k < cn) ? params[(int64_t)sp[k] + m0 + arow] : 0.0f;
+                }
+            };
+            load(0);
+            for (int k0 = 0; k0 < cn; k0 += TK) process(k0, cn, load);
         }
     }
     const int64_t b = b0 + mycol;
@@ -201,13 +233,22 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 // over the flattened (parent block, parent row) index kk -> (kk >> lbs, kk & (bs - 1)) with a
 // running max of log(f / m).  Parameters of one child row are contiguous along the parent row,
 // so the parameter tile is loaded with lanes along kk.
-template <typename T, int TM, int TN, bool ONE_BLOCK>
-__global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* __restrict__ nflows,
+//
+// FUSE (children scatter): instead of element flows, the epilogue writes the flows of the product
+// nodes' children straight into node_flows.  A product passes its flow unchanged to every child;
+// the caller guarantees that each child node has that product as its only parent and that the
+// child node blocks have the product block's size.  ch_nodes[j][k] = 2 * (first node of child
+// block k of the product block j) + 1 if that child's flows are to be stored as
+// log(flow) - mar (and mod_children is set), 0 = padding.
+template <typename T, int TM, int TN, bool ONE_BLOCK, bool FUSE>
+__global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ chids,
                                                     const int* __restrict__ parids, const int* __restrict__ parpids,
                                                     const int* __restrict__ npar, int KP, int lbs, int64_t B,
-                                                    int mtiles, int btiles, bool modified, bool accumulate) {
+                                                    int mtiles, int btiles, bool modified, bool accumulate,
+                                                    float* child_flows, const int* __restrict__ ch_nodes, int Kc,
+                                                    bool mod_children) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
     constexpr int LD = TK + MF<T>::PAD;
@@ -215,7 +256,7 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     constexpr int RA = 256 / TK, PA = TM / RA;     // parameter tile: RA rows per pass, PA passes
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ int spar[KCAP], spp[KCAP];
+    __shared__ int spar[ONE_BLOCK ? 1 : KCAP], spp[ONE_BLOCK ? 1 : KCAP];
     __shared__ float red[G][TN], mrun[TN], msc[TN];
 
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -243,87 +284,122 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     float epre[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
-    for (int pb = 0; pb < np; pb += KCAP) {
-        const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
-        const int ckp = (KP - pb < KCAP) ? KP - pb : KCAP;
-        const int kf = cnp << lbs;
+    // One stage of the reduction over the flattened parent index [k0, k0 + TK).
+    auto process = [&](int k0, int kf, auto&& load) {
+        float lm = kNegInf;
+#pragma unroll
+        for (int q = 0; q < PB; ++q) lm = fmaxf(lm, lv[q]);
+        red[bg][bcol] = lm;
         __syncthreads();
-        for (int u = tid; u < ckp; u += 256) {
-            spar[u] = parrow[pb + u];
-            spp[u] = pprow[pb + u];
+        if (tid < TN) {
+            float v = red[0][tid];
+#pragma unroll
+            for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
+            const float mo = mrun[tid];
+            const float mn = fmaxf(mo, v);
+            msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
+            mrun[tid] = mn;
         }
         __syncthreads();
+        const float mn = mrun[bcol];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(lv[q] - mn));
+#pragma unroll
+        for (int q = 0; q < PA; ++q) As[ar0 + q * RA][ak] = MF<T>::cvt(av[q]);
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < kf) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+    };
+
+    if (ONE_BLOCK) {
+        // bs >= TK (both powers of two): a stage lies inside one parent block, so its rows and
+        // parameters are contiguous runs and need only that block's two indices.
+        const int kf = np << lbs;
         auto load = [&](int k0) {
-            // ONE_BLOCK (bs >= TK, both powers of two): a stage of TK reduction steps lies inside
-            // one parent block, so its rows and parameters are contiguous runs.
-            if (ONE_BLOCK) {
-                const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
-                const int64_t rbase = ((int64_t)spar[kb] + k_in + bk0) * B + bb;
+            const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
+            const int64_t rbase = ((int64_t)parrow[kb] + k_in + bk0) * B + bb;
 #pragma unroll
-                for (int q = 0; q < PB; ++q)
-                    lv[q] = bok ? lfdm_at(nflows, nmars, rbase + (int64_t)q * B, modified) : kNegInf;
-                const int64_t base = (int64_t)spp[kb] + (int64_t)(m0 + ar0) * bs + k_in + ak;
+            for (int q = 0; q < PB; ++q)
+                lv[q] = bok ? lfdm_at(nflows, nmars, rbase + (int64_t)q * B, modified) : kNegInf;
+            const int64_t base = (int64_t)pprow[kb] + (int64_t)(m0 + ar0) * bs + k_in + ak;
 #pragma unroll
-                for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
-                return;
-            }
-#pragma unroll
-            for (int q = 0; q < PB; ++q) {
-                const int kk = k0 + bk0 + q;
-                float v = kNegInf;
-                if (kk < kf && bok)
-                    v = lfdm_at(nflows, nmars, ((int64_t)spar[kk >> lbs] + (kk & (bs - 1))) * B + bb, modified);
-                lv[q] = v;
-            }
-            const int kk = k0 + ak;
-            if (kk < kf) {
-                const int64_t base = (int64_t)spp[kk >> lbs] + (int64_t)(m0 + ar0) * bs + (kk & (bs - 1));
-#pragma unroll
-                for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
-            } else {
-#pragma unroll
-                for (int q = 0; q < PA; ++q) av[q] = 0.0f;
-            }
+            for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
         };
         load(0);
-        for (int k0 = 0; k0 < kf; k0 += TK) {
-            float lm = kNegInf;
-#pragma unroll
-            for (int q = 0; q < PB; ++q) lm = fmaxf(lm, lv[q]);
-            red[bg][bcol] = lm;
+        for (int k0 = 0; k0 < kf; k0 += TK) process(k0, kf, load);
+    } else {
+        for (int pb = 0; pb < np; pb += KCAP) {
+            const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
+            const int ckp = (KP - pb < KCAP) ? KP - pb : KCAP;
+            const int kf = cnp << lbs;
             __syncthreads();
-            if (tid < TN) {
-                float v = red[0][tid];
-#pragma unroll
-                for (int g = 1; g < G; ++g) v = fmaxf(v, red[g][tid]);
-                const float mo = mrun[tid];
-                const float mn = fmaxf(mo, v);
-                msc[tid] = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
-                mrun[tid] = mn;
+            for (int u = tid; u < ckp; u += 256) {
+                spar[u] = parrow[pb + u];
+                spp[u] = pprow[pb + u];
             }
             __syncthreads();
-            const float mn = mrun[bcol];
+            auto load = [&](int k0) {
 #pragma unroll
-            for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(lv[q] - mn));
+                for (int q = 0; q < PB; ++q) {
+                    const int kk = k0 + bk0 + q;
+                    float v = kNegInf;
+                    if (kk < kf && bok)
+                        v = lfdm_at(nflows, nmars, ((int64_t)spar[kk >> lbs] + (kk & (bs - 1))) * B + bb, modified);
+                    lv[q] = v;
+                }
+                const int kk = k0 + ak;
+                if (kk < kf) {
+                    const int64_t base = (int64_t)spp[kk >> lbs] + (int64_t)(m0 + ar0) * bs + (kk & (bs - 1));
 #pragma unroll
-            for (int q = 0; q < PA; ++q) As[ar0 + q * RA][ak] = MF<T>::cvt(av[q]);
-            const float s = msc[mycol];
+                    for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
+                } else {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] *= s;
-            __syncthreads();
-            if (k0 + TK < kf) load(k0 + TK);
-            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+                    for (int q = 0; q < PA; ++q) av[q] = 0.0f;
+                }
+            };
+            load(0);
+            for (int k0 = 0; k0 < kf; k0 += TK) process(k0, kf, load);
         }
     }
     if (b < B) {
         const float m = mrun[mycol];
+        if (FUSE) {
+            float v[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t o = (obase + acc_row(r, lane)) * B + b;
-            const float e = epre[r];
-            float v = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
-            if (accumulate) v += eflows[o];
-            eflows[o] = v;
+            for (int r = 0; r < 16; ++r) {
+                const float e = epre[r];
+                v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
+            }
+            const int* crow = ch_nodes + (int64_t)j * Kc;
+            for (int k = 0; k < Kc; ++k) {
+                const int code = crow[k];
+                if (code == 0) continue;
+                const int64_t cbase = (int64_t)(code >> 1) + m0 + wm * 32;
+                if ((code & 1) && mod_children) {
+                    float mv[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mv[r] = nmars[(cbase + acc_row(r, lane)) * B + b];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                            (mv[r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t o = (obase + acc_row(r, lane)) * B + b;
+                const float e = epre[r];
+                float v = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
+                if (accumulate) v += eflows[o];
+                eflows[o] = v;
+            }
         }
     }
 }
@@ -490,51 +566,88 @@ int log2_exact(int v) {
     return l;
 }
 
+template <typename T, int TM, int TN>
+void launch_fwd_tile(bool run, unsigned g, float* node_mars, const float* emars, const float* params, const int* nids,
+                     const int* cids, const int* pids, const int* nch, int K, int64_t B, int mtiles, int btiles,
+                     hipStream_t stream) {
+    if (run)
+        sum_fwd_mfma<T, TM, TN, true><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                             mtiles, btiles);
+    else
+        sum_fwd_mfma<T, TM, TN, false><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                              mtiles, btiles);
+}
+
 template <typename T>
 int launch_fwd(float* node_mars, const float* emars, const float* params, const int* nids, const int* cids,
-               const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B, hipStream_t stream) {
+               const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B, int child_run,
+               hipStream_t stream) {
     const int TM = bs >= 64 ? 64 : 32, TN = bs >= 64 ? 64 : 128;
     const int mtiles = bs / TM;
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles));
+    const bool run = child_run >= TK && child_run % TK == 0;
     if (bs >= 64)
-        sum_fwd_mfma<T, 64, 64><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K,
-                                                                     B, mtiles, (int)btiles);
+        launch_fwd_tile<T, 64, 64>(run, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles,
+                                   (int)btiles, stream);
     else
-        sum_fwd_mfma<T, 32, 128><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch,
-                                                                      K, B, mtiles, (int)btiles);
+        launch_fwd_tile<T, 32, 128>(run, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles,
+                                    (int)btiles, stream);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
+}
+
+template <typename T, int TM, int TN, bool FUSE>
+void launch_ele_tile(bool one_block, unsigned g, float* eflows, const float* nflows, const float* nmars,
+                     const float* emars, const float* params, const int* chids, const int* parids,
+                     const int* parpids, const int* npar, int KP, int lbs, int64_t B, int mtiles, int bt, bool mod,
+                     bool accum, float* child_flows, const int* ch_nodes, int Kc, bool mod_children,
+                     hipStream_t stream) {
+    if (one_block)
+        sum_ele_mfma<T, TM, TN, true, FUSE><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
+                                                                   parpids, npar, KP, lbs, B, mtiles, bt, mod, accum,
+                                                                   child_flows, ch_nodes, Kc, mod_children);
+    else
+        sum_ele_mfma<T, TM, TN, false, FUSE><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids,
+                                                                    parids, parpids, npar, KP, lbs, B, mtiles, bt, mod,
+                                                                    accum, child_flows, ch_nodes, Kc, mod_children);
 }
 
 template <typename T>
 int launch_ele(float* eflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* chids, const int* parids, const int* parpids, const int* npar, int n, int KP, int cbs, int bs,
-               int64_t B, bool mod, bool accum, hipStream_t stream) {
+               int64_t B, bool mod, bool accum, float* child_flows, const int* ch_nodes, int Kc, bool mod_children,
+               hipStream_t stream) {
     const int lbs = log2_exact(bs);
-    const int TM = cbs >= 64 ? 64 : 32, TN = cbs >= 64 ? 64 : 128;
+    const int TM = cbs >= 64 ? 64 : 32;
+    const int TN = cbs >= 64 ? 64 : 128;
     const int mtiles = cbs / TM;
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n * mtiles * btiles;
     PJAMD_REQUIRE(lbs >= 0 && cbs % TM == 0 && grid_fits(tiles));
     const unsigned g = (unsigned)tiles;
     const int bt = (int)btiles;
+    const bool ob = bs >= TK;
+    const bool fuse = ch_nodes != nullptr;
     if (cbs >= 64) {
-        if (bs >= TK)
-            sum_ele_mfma<T, 64, 64, true><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
-                                                                 parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+        if (fuse)
+            launch_ele_tile<T, 64, 64, true>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
+                                             KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
+                                             mod_children, stream);
         else
-            sum_ele_mfma<T, 64, 64, false><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
-                                                                  parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+            launch_ele_tile<T, 64, 64, false>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
+                                              KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
+                                              mod_children, stream);
     } else {
-        if (bs >= TK)
-            sum_ele_mfma<T, 32, 128, true><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,
-                                                                  parpids, npar, KP, lbs, B, mtiles, bt, mod, accum);
+        if (fuse)
+            launch_ele_tile<T, 32, 128, true>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
+                                              KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
+                                              mod_children, stream);
         else
-            sum_ele_mfma<T, 32, 128, false><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids,
-                                                                   parids, parpids, npar, KP, lbs, B, mtiles, bt, mod,
-                                                                   accum);
+            launch_ele_tile<T, 32, 128, false>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids,
+                                               npar, KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
+                                               mod_children, stream);
     }
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
@@ -593,13 +706,16 @@ int launch_par_batched(PF* pflows, const float* nflows, const float* nmars, cons
 // precision: 0 = FP32 operands, 1 = BF16, 2 = FP16 (FP32 accumulation in every case).
 int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const float* params, const int* nids,
                            const int* cids, const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B,
-                           int precision, hipStream_t stream) {
+                           int child_run, int precision, hipStream_t stream) {
     if (bs < 32) return PJAMD_EUNSUPPORTED;
     if (precision == 1)
-        return launch_fwd<__bf16>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, stream);
+        return launch_fwd<__bf16>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B,
+                                  child_run, stream);
     if (precision == 2)
-        return launch_fwd<_Float16>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, stream);
-    return launch_fwd<float>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, stream);
+        return launch_fwd<_Float16>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B,
+                                    child_run, stream);
+    return launch_fwd<float>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, child_run,
+                             stream);
 }
 
 int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,
@@ -609,12 +725,33 @@ int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float*
     if (cbs < 32) return PJAMD_EUNSUPPORTED;
     if (precision == 1)
         return launch_ele<__bf16>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs, B,
-                                  mod, accum, stream);
+                                  mod, accum, nullptr, nullptr, 0, false, stream);
     if (precision == 2)
         return launch_ele<_Float16>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs,
-                                    B, mod, accum, stream);
+                                    B, mod, accum, nullptr, nullptr, 0, false, stream);
     return launch_ele<float>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs, B,
-                             mod, accum, stream);
+                             mod, accum, nullptr, nullptr, 0, false, stream);
+}
+
+PJAMD_EXPORT int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars,
+                                                    const float* element_mars, const float* params, const int* chids,
+                                                    const int* parids, const int* parpids, const int* n_pars,
+                                                    const int* ch_nodes, int Kc, int n_cblocks, int KP, int cbs,
+                                                    int bs, int64_t B, int flags, int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && Kc >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 2 &&
+                  ch_nodes != nullptr);
+    if (n_cblocks == 0) return PJAMD_OK;
+    if (cbs < 32) return PJAMD_EUNSUPPORTED;
+    const bool mod = flags & 1, modc = (flags & 2) != 0;
+    if (precision == 1)
+        return launch_ele<__bf16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
+                                  n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, stream);
+    if (precision == 2)
+        return launch_ele<_Float16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids,
+                                    n_pars, n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc,
+                                    stream);
+    return launch_ele<float>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
+                             n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, stream);
 }
 
 // FP16 operands can overflow in the parameter-flow product (exp(e + s) is unbounded above), so

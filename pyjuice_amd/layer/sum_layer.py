@@ -106,6 +106,28 @@ class SumLayer(Layer, nn.Module):
     def __repr__(self):
         return f"SumLayer(nid_range=({self._layer_nid_range[0]}, {self._layer_nid_range[1]}), num_nodes={self.num_nodes}, num_edges={self.num_edges})"
 
+    def _child_run(self, p: int) -> int:
+        """Largest power of two R (<= 1024) such that every R-aligned group of real entries of
+        partition p's index rows names consecutive element rows whose parameters are bs apart
+        (cids[j, k0 + i] = cids[j, k0] + i, pids[j, k0 + i] = pids[j, k0] + i * bs); 0 if none.
+        The compiler lays out each child block this way, so R is at least the child block size
+        when all child blocks of the partition share it."""
+        c = self.partitioned_cids[p].detach().cpu().numpy().astype(np.int64)
+        q = self.partitioned_pids[p].detach().cpu().numpy().astype(np.int64)
+        n = self._fw_n_chs[p].detach().cpu().numpy().astype(np.int64)
+        K = c.shape[1]
+        k = np.arange(K)
+        valid = k[None, :] < n[:, None]
+        for r in (1024, 512, 256, 128, 64, 32, 16, 8, 4, 2):
+            if r > 2 * K and r > 2:
+                continue
+            base = (k // r) * r
+            off = k - base
+            ok = ((c - c[:, base]) == off[None, :]) & ((q - q[:, base]) == off[None, :] * self.block_size)
+            if bool(np.all(ok | ~valid)):
+                return r
+        return 0
+
     def to(self, device):
         nn.Module.to(self, device)
         device = torch.device(device)
@@ -116,6 +138,7 @@ class SumLayer(Layer, nn.Module):
         self._dev = None
         if device.type == "cuda":
             fw = []
+            self._fw_child_run = []
             for p in range(self.num_fw_partitions):
                 n = self.partitioned_nids[p]
                 if n.numel() == 0:
@@ -124,6 +147,7 @@ class SumLayer(Layer, nn.Module):
                 fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
                            _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device)),
                            int(self._fw_n_chs[p].sum())))
+                self._fw_child_run.append(self._child_run(p))
             bk = []
             for q in range(self.num_bk_partitions):
                 c = self.partitioned_chids[q]
@@ -148,23 +172,28 @@ class SumLayer(Layer, nn.Module):
             prec = 0
         B = node_mars.size(1)
         bs = self.block_size
-        for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
+        for (nids, cids, pids, pfids, nch, tot), run in zip(self._dev["fw"], self._fw_child_run):
             # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
             # per sample, parameters read once
             algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
             N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
-                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, prec,
+                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, prec,
                    N.stream(), algo=algo)
 
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
                  element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,
                  allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
                  negate_pflows: bool = False, accumulate_ch_flows: bool = False, allow_neg_flows: bool = False,
-                 force_use_fp32: bool = False, precision: Optional[str] = None, **kwargs):
+                 force_use_fp32: bool = False, precision: Optional[str] = None, premodified: bool = False,
+                 child_tables: Optional[Sequence[torch.Tensor]] = None, **kwargs):
         """Element flows and (optionally) parameter flows of the layer (``sum_layer.py:264-384``).
 
         ``allow_modify_flows`` rewrites the layer's node flows in place as ``log(flow) - mar``
         (``sum_layer.py:1381-1392``); the element / parameter-flow kernels read either form.
+        ``premodified``: the flows already arrive in that form (written by the fused kernel of the
+        layer above).  ``child_tables`` (one per backward partition, see
+        ``pjamd_sum_backward_ele_to_children``): write the flows of the product layer's children
+        directly into ``node_flows`` instead of element flows.
         """
         assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
         assert not logspace_flows and not allow_neg_flows, "unsupported flow mode for the HIP sum layers"
@@ -175,12 +204,24 @@ class SumLayer(Layer, nn.Module):
         B = node_flows.size(1)
         s = N.stream()
         bs = self.block_size
-        if allow_modify_flows:
+        if allow_modify_flows and not premodified:
             for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
                 N.call("pjamd_sum_modify_flows", N.ptr(node_flows), N.ptr(node_mars), N.ptr(nids), nids.numel(),
                        bs, B, s, algo=(12 * nids.numel() * bs * B, 0))
         mod = 1 if allow_modify_flows else 0
-        for chids, parids, parpids, npar, cbs, totp in self._dev["bk"]:
+        for q, (chids, parids, parpids, npar, cbs, totp) in enumerate(self._dev["bk"]):
+            if child_tables is not None:
+                tab = child_tables[q]
+                # parent rows (and marginals unless modified) + element rows read, one write per
+                # child row (+ its marginal when stored as log(flow) - mar), parameters once
+                n_ch = int(tab.size(0)) * cbs * int(tab.size(1))
+                algo = (4 * (totp * bs * (1 if mod else 2) + chids.numel() * cbs + n_ch * (2 if mod else 1)) * B
+                        + 4 * totp * bs * cbs, 2 * totp * bs * cbs * B)
+                N.call("pjamd_sum_backward_ele_to_children", N.ptr(node_flows), N.ptr(node_mars),
+                       N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids),
+                       N.ptr(npar), N.ptr(tab), tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
+                       mod | (2 if allow_modify_flows else 0), prec, s, algo=algo)
+                continue
             algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
                     2 * totp * bs * cbs * B)
             N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),

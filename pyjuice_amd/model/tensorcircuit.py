@@ -94,6 +94,7 @@ class TensorCircuit(nn.Module):
         self._em_dev = None
         self.modify_flows_in_place = True
         self._par_batches = {"tables": dict(), "layers": set()}
+        self._fused = {"tables": dict(), "prod_groups": set(), "premod": set()}
 
     # ------------------------------------------------------------------ device placement
 
@@ -123,7 +124,87 @@ class TensorCircuit(nn.Module):
                          for t, n, ch in self.parflow_fusing_kwargs],
             }
             self._par_batches = self._build_par_batches(device)
+            self._fused = self._build_fused_backward(device)
         return self
+
+    def _build_fused_backward(self, device):
+        """Sum layers whose element-flow kernel also performs the product backward below them
+        (``pjamd_sum_backward_ele_to_children``).  Checked here on the index tables: every child
+        of the product group below has exactly one parent in the whole circuit (then a child's
+        flow is its parent product's flow, written once, no accumulation); the sum layer's child
+        blocks (cbs >= 32) are exactly those product blocks, of that size.  A sum layer all of
+        whose nodes receive their flows this way is ``premod``: with ``allow_modify_flows`` the
+        fused kernel already stores its flows as log(flow) - mar, so its own modify pass is
+        skipped."""
+        groups = self.inner_layer_groups
+        cnt = np.zeros(self.num_nodes, dtype=np.int64)
+        children = dict()   # product group id -> {element block start: (child block starts, bs)}
+        for gid, g in enumerate(groups):
+            if not g.is_prod():
+                continue
+            d = dict()
+            for layer in g:
+                bs = layer.block_size
+                for n, c in zip(layer.partitioned_nids, layer.partitioned_cids):
+                    n = n.detach().cpu().numpy().astype(np.int64)
+                    c = c.detach().cpu().numpy().astype(np.int64)
+                    real = c[c != 0]
+                    if real.size:
+                        np.add.at(cnt, (real[:, None] + np.arange(bs)[None, :]).reshape(-1), 1)
+                    for e, ch in zip(n.tolist(), c):
+                        d[int(e)] = (ch[ch != 0], bs)
+            children[gid] = d
+
+        raw_tables, fused_prod = dict(), set()
+        covered = np.zeros(self.num_nodes, dtype=bool)
+        for gid, g in enumerate(groups):
+            if g.is_prod() or gid == 0 or not groups[gid - 1].is_prod():
+                continue
+            d = children[gid - 1]
+            ok = all(bool(np.all(cnt[(ch[:, None] + np.arange(bs)[None, :]).reshape(-1)] == 1))
+                     for ch, bs in d.values())
+            per_layer, seen = dict(), set()
+            for layer in g:
+                if not ok:
+                    break
+                tabs = []
+                for chids, parids, parpids, npar, cbs, totp in layer._dev["bk"]:
+                    starts = chids.detach().cpu().numpy().astype(np.int64).tolist()
+                    if cbs < 32 or any(c not in d or d[c][1] != cbs for c in starts):
+                        ok = False
+                        break
+                    seen.update(starts)
+                    rows = [d[c][0] for c in starts]
+                    T = np.zeros([len(rows), max(max(len(x) for x in rows), 1)], dtype=np.int64)
+                    for i, x in enumerate(rows):
+                        T[i, :len(x)] = x
+                    tabs.append(T)
+                per_layer[id(layer)] = tabs
+            if not ok or seen != set(d.keys()):
+                continue
+            raw_tables.update(per_layer)
+            fused_prod.add(gid - 1)
+            for ch, bs in d.values():
+                covered[(ch[:, None] + np.arange(bs)[None, :]).reshape(-1)] = True
+
+        premod, premod_row = set(), np.zeros(self.num_nodes, dtype=bool)
+        for gid, g in enumerate(groups):
+            if g.is_prod():
+                continue
+            for layer in g:
+                a, b = layer.nid_range
+                if covered[a:b].all():
+                    premod.add(id(layer))
+                    premod_row[a:b] = True
+        tables = dict()
+        for lid, tabs in raw_tables.items():
+            out = []
+            for T in tabs:
+                code = 2 * T + (premod_row[T] & (T != 0)).astype(np.int64)
+                assert code.max() < 2 ** 31
+                out.append(torch.from_numpy(code).to(torch.int32).to(device).contiguous())
+            tables[lid] = out
+        return {"tables": tables, "prod_groups": fused_prod, "premod": premod}
 
     def _build_par_batches(self, device):
         """Descriptor tables of the batched parameter-flow launch: every sum layer with block size
@@ -337,17 +418,25 @@ class TensorCircuit(nn.Module):
         # ``modify_flows_in_place = False`` the kernels compute it on the fly instead.
         # Parameter flows of sum layers with block size >= 32 come from one batched launch after
         # the loop (the flows and element marginals they read stay untouched until then).
+        # Where the product layer below a sum layer only fans flows out to single-parent children,
+        # the sum layer's element-flow kernel writes those children's node flows itself (and, for
+        # sum children, already as log(flow) - mar), so that product backward and the children's
+        # modify pass are not launched.
         modify = allow_modify_flows and self.modify_flows_in_place
         batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu") else set()
+        fz = self._fused if precision != "fp32_valu" else {"tables": dict(), "prod_groups": set(), "premod": set()}
         for lid in range(len(self.inner_layer_groups) - 1, -1, -1):
             g = self.inner_layer_groups[lid]
             if g.is_prod():
-                g.backward(self.node_flows, self.element_flows)
+                if lid not in fz["prod_groups"]:
+                    g.backward(self.node_flows, self.element_flows)
                 continue
             for layer in g:
                 pf = self.param_flows if compute_param_flows and id(layer) not in batched else None
                 layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
-                               param_flows=pf, allow_modify_flows=modify, precision=precision)
+                               param_flows=pf, allow_modify_flows=modify, precision=precision,
+                               premodified=modify and id(layer) in fz["premod"],
+                               child_tables=fz["tables"].get(id(layer)))
         if batched:
             self._run_par_batched(modify, precision)
 

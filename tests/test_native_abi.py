@@ -80,8 +80,8 @@ def test_argument_checks_fail_before_launch(lib):
     null = ctypes.c_void_p(0)
     # negative block count / zero batch: rejected on the host with PJAMD_STATUS_EINVAL
     assert lib.pjamd_prod_forward(null, null, null, null, -1, 2, 4, 8, 0, null) == -1
-    assert lib.pjamd_sum_forward(null, null, null, null, null, null, null, 1, 4, 32, 0, 1, null) == -1
-    assert lib.pjamd_sum_forward(null, null, null, null, null, null, null, 1, 4, 32, 8, 7, null) == -1
+    assert lib.pjamd_sum_forward(null, null, null, null, null, null, null, 1, 4, 32, 0, 0, 1, null) == -1
+    assert lib.pjamd_sum_forward(null, null, null, null, null, null, null, 1, 4, 32, 8, 0, 7, null) == -1
     assert lib.pjamd_sum_backward_par_batched(null, null, null, null, null, null, null, null, null, null, null,
                                               null, 4, 48, 8, 0, 1, null) == -1
     # empty work is a no-op that succeeds without touching the device
