@@ -1,7 +1,7 @@
 /*
  * pyjuice_amd — C ABI of the MI355X (gfx950) probabilistic-circuit kernels.
  *
- * One shared library, libpyjuice_amd.so, built from pyjuice_amd/csrc/*.hip.  Every entry point
+ * One shared library, libpyjuice_amd.so, built from the HIP sources in pyjuice_amd/csrc.  Every entry point
  * is a plain C function over device pointers and sizes; it enqueues its kernels on `stream`
  * and returns without synchronising.  Return value: 0 on success, a positive hipError_t when
  * a launch failed, PJAMD_EINVAL (-1) for invalid sizes / argument combinations and
@@ -80,12 +80,13 @@ int pjamd_data_to_vb(int* out, int src_code, const void* src, int64_t B, int64_t
 /* ----------------------------------------------------------------------------- input layers */
 
 /* node_mars[node_offset + n][b] = log p_n(data[vids[n]][b]) for the num_nodes nodes of one
- * input layer; s_pids[n] is node n's first parameter, ncats[n] its category count.
+ * input layer; s_pids[n] is node n's first parameter, ncats[n] its category count and kmax the
+ * largest category count of the layer (Categorical; ignored for Bernoulli).
  * Replaces InputLayer.forward (src/pyjuice/layer/input_layer.py:283-404) with
  * Categorical.fw_mar_fn (categorical.py:83-92) / Bernoulli.fw_mar_fn (bernoulli.py:65-71). */
 int pjamd_input_forward(int kind, const float* params, float* node_mars, const int* data, const int* vids,
                         const int64_t* s_pids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
-                        pjamd_stream_t stream);
+                        int kmax, pjamd_stream_t stream);
 
 /* pflows[s_pfids[n] + data[vids[n]][b]] += node_flows[node_offset + n][b] summed over b
  * (Categorical; Bernoulli adds the flows of samples with x = 1 to slot 0 and of x = 0 to

@@ -6,74 +6,129 @@ using namespace pjamd;
 
 namespace {
 
-constexpr int kFwdRows = 4;  // input nodes per thread in the forward pass
+constexpr int kLdsFloats = 4096; // largest category count with LDS flow histograms
 
 // node_mars[(off + i) * B + b] = log p_i(x[v_i, b]); out-of-range categories have p = 0.
+// kRowThreads threads per node row, 4 consecutive batch columns each (16-byte loads of the
+// categories, 16-byte stores); the parameter gathers hit the node's row (kmax floats) in L1 / L2.
+constexpr int kRowThreads = 128;
+
 template <int KIND>
 __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict__ params, float* __restrict__ node_mars,
                                                         const int* __restrict__ data, const int* __restrict__ vids,
                                                         const int64_t* __restrict__ s_pids,
                                                         const int* __restrict__ ncats, int64_t num_nodes,
                                                         int64_t node_offset, int64_t B) {
-    const int64_t b = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    const int64_t i0 = (int64_t)blockIdx.x * kFwdRows;
-    int last_v = -1;
-    int x = 0;
-#pragma unroll
-    for (int r = 0; r < kFwdRows; ++r) {
-        const int64_t i = i0 + r;
-        if (i >= num_nodes) break;
-        const int v = vids[i];
-        if (v != last_v) {
-            x = data[(int64_t)v * B + b];
-            last_v = v;
-        }
-        float lp;
-        if (KIND == 0) {
-            const int K = ncats[i];
-            lp = (x >= 0 && x < K) ? __logf(params[s_pids[i] + x]) : kNegInf;
+    const int64_t i = (int64_t)blockIdx.x * (256 / kRowThreads) + threadIdx.x / kRowThreads;
+    if (i >= num_nodes) return;
+    const int sub = threadIdx.x % kRowThreads;
+    const int* xrow = data + (int64_t)vids[i] * B;
+    float* out = node_mars + (node_offset + i) * B;
+    const float* th = params + s_pids[i];
+    const int K = (KIND == 0) ? ncats[i] : 2;
+    const float p1 = (KIND == 1) ? th[0] : 0.0f;
+    for (int64_t b = (int64_t)sub * 4; b < B; b += kRowThreads * 4) {
+        if (b + 4 <= B) {
+            const int4 x = *reinterpret_cast<const int4*>(xrow + b);
+            float4 o;
+            if (KIND == 0) {
+                o.x = (x.x >= 0 && x.x < K) ? __logf(th[x.x]) : kNegInf;
+                o.y = (x.y >= 0 && x.y < K) ? __logf(th[x.y]) : kNegInf;
+                o.z = (x.z >= 0 && x.z < K) ? __logf(th[x.z]) : kNegInf;
+                o.w = (x.w >= 0 && x.w < K) ? __logf(th[x.w]) : kNegInf;
+            } else {
+                o.x = __logf(x.x > 0 ? p1 : 1.0f - p1);
+                o.y = __logf(x.y > 0 ? p1 : 1.0f - p1);
+                o.z = __logf(x.z > 0 ? p1 : 1.0f - p1);
+                o.w = __logf(x.w > 0 ? p1 : 1.0f - p1);
+            }
+            *reinterpret_cast<float4*>(out + b) = o;
         } else {
-            const float p = params[s_pids[i]];
-            lp = __logf(x > 0 ? p : 1.0f - p);
+            for (int64_t c = b; c < B; ++c) {
+                const int xx = xrow[c];
+                if (KIND == 0) out[c] = (xx >= 0 && xx < K) ? __logf(th[xx]) : kNegInf;
+                else out[c] = __logf(xx > 0 ? p1 : 1.0f - p1);
+            }
         }
-        node_mars[(node_offset + i) * B + b] = lp;
     }
 }
 
-// Categorical flow backward: each workgroup owns `nt` consecutive input nodes and builds
-// their [nt, kmax] flow histograms in LDS, then adds them to the parameter flows.
+// Forward, generic form (any B and alignment): one thread per (node, batch column).
+__global__ __launch_bounds__(256) void cat_fwd_global_kernel(const float* __restrict__ params,
+                                                             float* __restrict__ node_mars,
+                                                             const int* __restrict__ data, const int* __restrict__ vids,
+                                                             const int64_t* __restrict__ s_pids,
+                                                             const int* __restrict__ ncats, int64_t node_offset,
+                                                             int64_t B, int kind) {
+    const int64_t b = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int64_t i = blockIdx.x;
+    const int x = data[(int64_t)vids[i] * B + b];
+    float v;
+    if (kind == 0) {
+        v = (x >= 0 && x < ncats[i]) ? __logf(params[s_pids[i] + x]) : kNegInf;
+    } else {
+        const float p = params[s_pids[i]];
+        v = __logf(x > 0 ? p : 1.0f - p);
+    }
+    node_mars[(node_offset + i) * B + b] = v;
+}
+
+// Categorical flow backward: a workgroup owns two nodes (kRowThreads threads each, 4 consecutive
+// batch columns per thread with 16-byte loads) and builds their flow histograms in LDS; then
+// each node's kmax slots are added to the parameter flows, all slot reads issued before any
+// write (the slots of a node are distinct), or atomically when slots are shared between nodes.
 template <typename PF>
 __global__ __launch_bounds__(256) void cat_bwd_lds_kernel(PF* __restrict__ pflows, const float* __restrict__ node_flows,
                                                           const int* __restrict__ data, const int* __restrict__ vids,
                                                           const int64_t* __restrict__ s_pfids,
                                                           const int* __restrict__ ncats, int64_t num_nodes,
-                                                          int64_t node_offset, int64_t B, int nt, int kmax,
-                                                          bool atomic_out) {
+                                                          int64_t node_offset, int64_t B, int kmax, bool atomic_out) {
     extern __shared__ float hist[];
-    const int64_t i0 = (int64_t)blockIdx.x * nt;
-    const int n_here = (int)((num_nodes - i0) < nt ? (num_nodes - i0) : nt);
-    for (int t = threadIdx.x; t < nt * kmax; t += blockDim.x) hist[t] = 0.0f;
+    constexpr int NR = 256 / kRowThreads;
+    const int r = threadIdx.x / kRowThreads, sub = threadIdx.x % kRowThreads;
+    const int64_t i = (int64_t)blockIdx.x * NR + r;
+    const bool on = i < num_nodes;
+    float* h = hist + r * kmax;
+    for (int t = sub; t < kmax; t += kRowThreads) h[t] = 0.0f;
     __syncthreads();
-    for (int r = 0; r < n_here; ++r) {
-        const int64_t i = i0 + r;
+    const int K = on ? ncats[i] : 0;
+    if (on) {
         const int* xrow = data + (int64_t)vids[i] * B;
         const float* frow = node_flows + (node_offset + i) * B;
-        const int K = ncats[i];
-        for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
-            const int x = xrow[b];
-            const float f = frow[b];
-            if (x >= 0 && x < K && f != 0.0f) atomicAdd(&hist[r * kmax + x], f);
+        for (int64_t b = (int64_t)sub * 4; b < B; b += kRowThreads * 4) {
+            const int4 x = *reinterpret_cast<const int4*>(xrow + b);
+            const float4 f = *reinterpret_cast<const float4*>(frow + b);
+            if (x.x >= 0 && x.x < K && f.x != 0.0f) atomicAdd(&h[x.x], f.x);
+            if (x.y >= 0 && x.y < K && f.y != 0.0f) atomicAdd(&h[x.y], f.y);
+            if (x.z >= 0 && x.z < K && f.z != 0.0f) atomicAdd(&h[x.z], f.z);
+            if (x.w >= 0 && x.w < K && f.w != 0.0f) atomicAdd(&h[x.w], f.w);
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < n_here * kmax; t += blockDim.x) {
-        const int r = t / kmax;
-        const int k = t - r * kmax;
-        const int64_t i = i0 + r;
-        if (k < ncats[i]) {
-            const float h = hist[t];
-            if (h != 0.0f) pf_add<PF>(&pflows[s_pfids[i] + k], (PF)h, atomic_out);
+    if (!on) return;
+    constexpr int kPer = 8;   // slots per thread per pass (loads first)
+    const int64_t f0 = s_pfids[i];
+    for (int k0 = 0; k0 < K; k0 += kRowThreads * kPer) {
+        float hv[kPer];
+        int kk[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int k = k0 + q * kRowThreads + sub;
+            kk[q] = (k < K && h[k] != 0.0f) ? k : -1;
+            hv[q] = kk[q] >= 0 ? h[k] : 0.0f;
+        }
+        if (atomic_out) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (kk[q] >= 0) atomicAdd(&pflows[f0 + kk[q]], (PF)hv[q]);
+        } else {
+            PF old[kPer];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) old[q] = kk[q] >= 0 ? pflows[f0 + kk[q]] : (PF)0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (kk[q] >= 0) pflows[f0 + kk[q]] = old[q] + (PF)hv[q];
         }
     }
 }
@@ -200,14 +255,28 @@ __global__ __launch_bounds__(256) void input_sample_kernel(int kind, OUT* __rest
 
 PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_mars, const int* data,
                                      const int* vids, const int64_t* s_pids, const int* ncats, int64_t num_nodes,
-                                     int64_t node_offset, int64_t B, hipStream_t stream) {
+                                     int64_t node_offset, int64_t B, int kmax, hipStream_t stream) {
     PJAMD_REQUIRE(num_nodes >= 0 && B > 0 && (kind == 0 || kind == 1));
     if (num_nodes == 0) return PJAMD_OK;
-    dim3 grid(cdiv(num_nodes, kFwdRows), cdiv(B, 256));
+    PJAMD_REQUIRE(kind == 1 || kmax > 0);
+    // 16-byte rows: B a multiple of 4 and 16-byte aligned buffers (int32 data rows, float rows)
+    const bool vec = (B % 4 == 0) && ((((uintptr_t)data) | ((uintptr_t)node_mars)) % 16 == 0) &&
+                     ((node_offset * B) % 4 == 0);
+    if (!vec) {
+        dim3 grid((unsigned)num_nodes, cdiv(B, 256));
+        cat_fwd_global_kernel<<<grid, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats, node_offset, B,
+                                                        kind);
+        PJAMD_LAUNCH_CHECK();
+        return PJAMD_OK;
+    }
+    const int64_t blocks = cdiv(num_nodes, 256 / kRowThreads);
+    PJAMD_REQUIRE(blocks < ((int64_t)1 << 31));
     if (kind == 0)
-        input_fwd_kernel<0><<<grid, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats, num_nodes, node_offset, B);
+        input_fwd_kernel<0><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats,
+                                                                  num_nodes, node_offset, B);
     else
-        input_fwd_kernel<1><<<grid, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats, num_nodes, node_offset, B);
+        input_fwd_kernel<1><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats,
+                                                                  num_nodes, node_offset, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -219,13 +288,11 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
     if (kind == 1) {
         bern_bwd_kernel<PF><<<(unsigned)num_nodes, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids,
                                                                     node_offset, B, atomic_out);
-    } else if (kmax <= 8192) {
-        int nt = 8192 / kmax;
-        if (nt > 16) nt = 16;
-        if (nt < 1) nt = 1;
-        size_t lds = (size_t)nt * kmax * sizeof(float);
-        cat_bwd_lds_kernel<PF><<<cdiv(num_nodes, nt), 256, lds, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
-                                                                        num_nodes, node_offset, B, nt, kmax, atomic_out);
+    } else if (kmax <= kLdsFloats && B % 4 == 0 && ((((uintptr_t)data) | ((uintptr_t)node_flows)) % 16 == 0)) {
+        constexpr int NR = 256 / kRowThreads;
+        const size_t lds = (size_t)NR * kmax * sizeof(float);
+        cat_bwd_lds_kernel<PF><<<(unsigned)cdiv(num_nodes, NR), 256, lds, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kmax, atomic_out);
     } else {
         dim3 grid((unsigned)num_nodes, cdiv(B, 256));
         cat_bwd_global_kernel<PF><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,

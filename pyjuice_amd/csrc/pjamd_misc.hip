@@ -27,40 +27,86 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
     }
 }
 
-// Mini-batch EM of the sum parameters.  One wavefront handles (node block, up to 64 nodes);
-// record = [par_start, pflow_start, node_start, block_size, n_children] and the parameter of
-// child c of node i sits at par_start + c * block_size + i (same for flows).
+// Mini-batch EM of the sum parameters.  Record = [par_start, pflow_start, node_start,
+// block_size, n_children]; the parameter of child c of node i sits at par_start + c * block_size
+// + i (same for flows).  A workgroup of four wavefronts handles (record, up to 64 nodes): lanes
+// run along the nodes (ti = min(bs, 64) nodes, G = 64 / ti lanes per node) and the wavefronts
+// split the children.  When every lane owns at most kRegChildren children (block size >= 64 and
+// at most 256 children) the flows and parameters stay in registers between the normaliser and
+// the update, so each is read from memory once; otherwise the update re-reads them.
+constexpr int kEmWaves = 4;
+constexpr int kRegChildren = 64;
+
 template <typename PF>
-__global__ __launch_bounds__(64) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
-                                                    const int* __restrict__ recs, int n_recs, double step, double pc,
-                                                    bool keep_zero) {
+__global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
+                                                     const int* __restrict__ recs, int n_recs, double step, double pc,
+                                                     bool keep_zero) {
+    __shared__ double part[kEmWaves][64];
     const int rec = blockIdx.x;
     const int* R = recs + (int64_t)rec * 5;
     const int64_t par0 = R[0], pf0 = R[1];
     const int bs = R[3], nch = R[4];
-    const int ti = bs < 64 ? bs : 64;       // nodes handled by this wavefront
-    const int G = 64 / ti;                  // lanes cooperating on one node
-    const int lane = threadIdx.x;
-    const int i = blockIdx.y * 64 + (lane % ti);
-    const int g = lane / ti;
     if (blockIdx.y * 64 >= bs) return;
+    const int ti = bs < 64 ? bs : 64;       // nodes of this workgroup
+    const int G = 64 / ti;                  // lanes per node inside a wavefront
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.y * 64 + (lane % ti);
+    const int g = lane / ti + G * w;        // this lane's child offset
+    const int stride = G * kEmWaves;
+    const bool lane_ok = (lane / ti) < G;   // lanes past G * ti (when 64 % ti != 0) idle
     const double npc = pc / (double)nch;
+    const bool in_regs = (ti == 64) && (nch <= kEmWaves * kRegChildren);
+
+    double fr[kRegChildren];
+    float tr[kRegChildren];
     double cum = 0.0;
-    for (int c = g; c < nch; c += G) {
-        const int64_t off = (int64_t)c * bs + i;
-        const double f = (double)pflows[pf0 + off];
-        if (keep_zero) cum += (params[par0 + off] < 1e-12f) ? 0.0 : f + npc;
-        else cum += f;
+    if (in_regs) {
+#pragma unroll
+        for (int q = 0; q < kRegChildren; ++q) {
+            const int c = g + q * stride;
+            const int64_t off = (int64_t)c * bs + i;
+            fr[q] = (c < nch) ? (double)pflows[pf0 + off] : 0.0;
+            tr[q] = (c < nch) ? params[par0 + off] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kRegChildren; ++q) {
+            const int c = g + q * stride;
+            if (c < nch) cum += keep_zero ? ((tr[q] < 1e-12f) ? 0.0 : fr[q] + npc) : fr[q];
+        }
+    } else if (lane_ok) {
+        for (int c = g; c < nch; c += stride) {
+            const int64_t off = (int64_t)c * bs + i;
+            const double f = (double)pflows[pf0 + off];
+            cum += keep_zero ? ((params[par0 + off] < 1e-12f) ? 0.0 : f + npc) : f;
+        }
     }
+    // reduce over the G lane groups of a wavefront, then over the wavefronts
     for (int o = ti; o < 64; o <<= 1) cum += __shfl_xor(cum, o, 64);
-    const double denom = keep_zero ? cum : cum + pc;
-    for (int c = g; c < nch; c += G) {
-        const int64_t off = (int64_t)c * bs + i;
-        const double th = (double)params[par0 + off];
-        const double nv = ((double)pflows[pf0 + off] + npc) / denom;
-        double upd = (1.0 - step) * th + step * nv;
-        if (keep_zero && th < 1e-12) upd = 0.0;
-        params[par0 + off] = (float)upd;
+    part[w][lane] = cum;
+    __syncthreads();
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < kEmWaves; ++q) tot += part[q][lane % ti];
+    const double denom = keep_zero ? tot : tot + pc;
+    if (in_regs) {
+#pragma unroll
+        for (int q = 0; q < kRegChildren; ++q) {
+            const int c = g + q * stride;
+            if (c < nch) {
+                const double th = (double)tr[q];
+                double upd = (1.0 - step) * th + step * ((fr[q] + npc) / denom);
+                if (keep_zero && th < 1e-12) upd = 0.0;
+                params[par0 + (int64_t)c * bs + i] = (float)upd;
+            }
+        }
+    } else if (lane_ok) {
+        for (int c = g; c < nch; c += stride) {
+            const int64_t off = (int64_t)c * bs + i;
+            const double th = (double)params[par0 + off];
+            double upd = (1.0 - step) * th + step * (((double)pflows[pf0 + off] + npc) / denom);
+            if (keep_zero && th < 1e-12) upd = 0.0;
+            params[par0 + off] = (float)upd;
+        }
     }
 }
 
@@ -155,11 +201,11 @@ PJAMD_EXPORT int pjamd_sum_em(float* params, const void* pflows, int pf64, const
     if (n_recs == 0) return PJAMD_OK;
     dim3 grid((unsigned)n_recs, cdiv(max_bs, 64));
     if (pf64)
-        sum_em_kernel<double><<<grid, 64, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step, pseudocount,
-                                                       keep_zero != 0);
+        sum_em_kernel<double><<<grid, 256, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step, pseudocount,
+                                                        keep_zero != 0);
     else
-        sum_em_kernel<float><<<grid, 64, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step, pseudocount,
-                                                      keep_zero != 0);
+        sum_em_kernel<float><<<grid, 256, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step, pseudocount,
+                                                       keep_zero != 0);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }

@@ -127,26 +127,48 @@ __global__ __launch_bounds__(kThreads) void sum_fwd_tiled(float* __restrict__ no
     }
 }
 
-// Forward, any bs (used for bs < 16): one thread per (node, batch column), two passes over K.
+// Forward, any bs (used for bs < 16): one workgroup per (node, 64 batch columns); its four
+// wavefronts split the children (k = w, w + 4, ...), lanes run along the batch.  Pass 1 takes
+// the column max, pass 2 the sum of theta * exp(e - max); the loop bodies are unrolled so that
+// many child rows are in flight per wavefront (a node with hundreds of children, such as a
+// circuit's root, would otherwise be a chain of dependent loads).
+constexpr int kSmallWaves = kThreads / 64;
+
 __global__ __launch_bounds__(kThreads) void sum_fwd_rowwise(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                             const float* __restrict__ params, const int* __restrict__ nids,
                                                             const int* __restrict__ cids, const int* __restrict__ pids,
                                                             const int* __restrict__ nch, int K, int bs, int64_t B) {
-    const int64_t b = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (b >= B) return;
+    __shared__ float red[kSmallWaves][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t b = (int64_t)blockIdx.y * 64 + lane;
+    const bool ok = b < B;
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
     const int n = nch[j];
     const int* crow = cids + (int64_t)j * K;
     const int* prow = pids + (int64_t)j * K;
     float m = kNegInf;
-    for (int k = 0; k < n; ++k) m = fmaxf(m, emars[(int64_t)crow[k] * B + b]);
-    float out = kNegInf;
+#pragma unroll 8
+    for (int k = w; k < n; k += kSmallWaves) m = fmaxf(m, ok ? emars[(int64_t)crow[k] * B + b] : kNegInf);
+    red[w][lane] = m;
+    __syncthreads();
+    m = red[0][lane];
+#pragma unroll
+    for (int q = 1; q < kSmallWaves; ++q) m = fmaxf(m, red[q][lane]);
+    __syncthreads();
+    float s = 0.0f;
     if (m != kNegInf) {
-        float s = 0.0f;
-        for (int k = 0; k < n; ++k) s = fmaf(params[(int64_t)prow[k] + i], __expf(emars[(int64_t)crow[k] * B + b] - m), s);
-        out = (s > 0.0f) ? __logf(s) + m : kNegInf;
+#pragma unroll 8
+        for (int k = w; k < n; k += kSmallWaves)
+            s = fmaf(params[(int64_t)prow[k] + i], __expf((ok ? emars[(int64_t)crow[k] * B + b] : kNegInf) - m), s);
     }
-    node_mars[((int64_t)nids[j] + i) * B + b] = out;
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && ok) {
+        s = red[0][lane];
+#pragma unroll
+        for (int q = 1; q < kSmallWaves; ++q) s += red[q][lane];
+        node_mars[((int64_t)nids[j] + i) * B + b] = (s > 0.0f && m != kNegInf) ? __logf(s) + m : kNegInf;
+    }
 }
 
 // node_flows <- log(node_flows) - node_mars on the layer's nodes (-inf where the marginal is -inf).
@@ -427,7 +449,8 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_tiled(PF* __restrict__ p
     }
 }
 
-// Parameter flows for bs < 16: one thread per (node, child); loops over the batch.
+// Parameter flows for bs < 16: one wavefront per (node, child), lanes along the batch (loop
+// unrolled so that several batch chunks are in flight), then a wavefront reduction.
 template <typename PF>
 __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                                                 const float* __restrict__ nmars, const float* __restrict__ emars,
@@ -436,19 +459,24 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__
                                                                 const int* __restrict__ pfids, const int* __restrict__ nch,
                                                                 int K, int bs, int64_t B, bool modified, bool atomic_out,
                                                                 float sign) {
-    const int k = blockIdx.y * blockDim.x + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.y * kSmallWaves + w;
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
     if (k >= nch[j]) return;
     const int64_t nrow = (int64_t)nids[j] + i;
     const int64_t erow = cids[(int64_t)j * K + k];
     float s = 0.0f;
-    for (int64_t b = 0; b < B; ++b) {
+#pragma unroll 4
+    for (int64_t b = lane; b < B; b += 64) {
         const float l = log_flow_div_mar(nflows, nmars, nrow * B + b, modified);
         const float e = emars[erow * B + b];
         if (l != kNegInf && e != kNegInf) s += __expf(l + e);
     }
-    const float v = sign * params[(int64_t)pids[(int64_t)j * K + k] + i] * s;
-    pf_add<PF>(&pflows[(int64_t)pfids[(int64_t)j * K + k] + i], (PF)v, atomic_out);
+    s = warp_sum(s);
+    if (lane == 0) {
+        const float v = sign * params[(int64_t)pids[(int64_t)j * K + k] + i] * s;
+        pf_add<PF>(&pflows[(int64_t)pfids[(int64_t)j * K + k] + i], (PF)v, atomic_out);
+    }
 }
 
 }  // namespace
@@ -486,7 +514,7 @@ PJAMD_EXPORT int pjamd_sum_forward(float* node_mars, const float* element_mars, 
         sum_fwd_tiled<16><<<dim3((unsigned)n_blocks, gy), kThreads, 0, stream>>>(node_mars, element_mars, params, nids,
                                                                                 cids, pids, nch, K, bs, B);
     } else {
-        sum_fwd_rowwise<<<dim3((unsigned)((int64_t)n_blocks * bs), cdiv(B, kThreads)), kThreads, 0, stream>>>(
+        sum_fwd_rowwise<<<dim3((unsigned)((int64_t)n_blocks * bs), cdiv(B, 64)), kThreads, 0, stream>>>(
             node_mars, element_mars, params, nids, cids, pids, nch, K, bs, B);
     }
     PJAMD_LAUNCH_CHECK();
@@ -554,7 +582,7 @@ static int sum_bwd_par_impl(PF* pflows, const float* node_flows, const float* no
             pflows, node_flows, node_mars, element_mars, params, nids, cids, pids, pfids, nch, K, bs, B, mod,
             atomic_out, sign);
     } else {
-        sum_bwd_par_rowwise<PF><<<dim3((unsigned)((int64_t)n_blocks * bs), cdiv(K, kThreads)), kThreads, 0, stream>>>(
+        sum_bwd_par_rowwise<PF><<<dim3((unsigned)((int64_t)n_blocks * bs), cdiv(K, kSmallWaves)), kThreads, 0, stream>>>(
             pflows, node_flows, node_mars, element_mars, params, nids, cids, pids, pfids, nch, K, bs, B, mod,
             atomic_out, sign);
     }

@@ -275,8 +275,8 @@ class InputLayer(Layer, nn.Module):
         t = self._dev
         B = node_mars.size(1)
         N.call("pjamd_input_forward", self._kind(), N.ptr(p), N.ptr(node_mars), N.ptr(data), N.ptr(t["vids"]),
-               N.ptr(t["s_pids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0], B, N.stream(),
-               algo=(4 * (self.num_nodes + len(self.layer_scope)) * B, 0))
+               N.ptr(t["s_pids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0], B, t["kmax"],
+               N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B + 4 * self.num_parameters, 0))
 
     def backward(self, data: torch.Tensor, node_flows: torch.Tensor, node_mars: torch.Tensor,
                  params: Optional[Dict] = None, logspace_flows: bool = False, **kwargs):
