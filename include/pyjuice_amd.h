@@ -90,13 +90,17 @@ int pjamd_input_forward(int kind, const float* params, float* node_mars, const i
 
 /* pflows[s_pfids[n] + data[vids[n]][b]] += node_flows[node_offset + n][b] summed over b
  * (Categorical; Bernoulli adds the flows of samples with x = 1 to slot 0 and of x = 0 to
- * slot 1).  pf64 selects a double accumulator.  kmax_or_tied = (largest category count of the
- * layer << 1) | 1 if several nodes share parameter-flow slots (then the adds are atomic).  Replaces InputLayer.backward
+ * slot 1).  pf64 selects a double accumulator.  layout = (largest category count of the layer
+ * << 3) | 4 if the accumulator is zero (slots are stored, not read; not with 1) | 2 if every
+ * tile of 8 consecutive nodes (from node 0) shares one variable and one category count (lets
+ * the kernel sort the batch once per tile instead of adding per sample) | 1 if several nodes
+ * share parameter-flow slots (then the adds are atomic).
+ * Replaces InputLayer.backward
  * (input_layer.py:406-571) with Categorical.bk_flow_fn (categorical.py:94-101) /
  * Bernoulli.bk_flow_fn (bernoulli.py:73-77). */
 int pjamd_input_backward(int kind, void* pflows, const float* node_flows, int pf64, const int* data, const int* vids,
                          const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
-                         int kmax_or_tied, pjamd_stream_t stream);
+                         int layout, pjamd_stream_t stream);
 
 /* Mini-batch EM of the num_src source nodes src[] of an input layer:
  * theta <- (1 - step) theta + step * (flows + pseudocount / K) / (sum flows + pseudocount),
@@ -179,7 +183,8 @@ int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars
 /* param_flows[pfids[j][k] + i] += sum_b params[pids[j][k] + i] * node_flows[nids[j] + i][b] *
  * exp(element_mars[cids[j][k]][b] - node_mars[nids[j] + i][b]) for one forward partition.
  * flags bit 0: flows hold log(flow) - mar; bit 1: slots are shared (atomic adds); bit 2: negate;
- * bit 3: param_flows is double.  Replaces SumLayer._backward_block_sparse_par_flows
+ * bit 3: param_flows is double; bit 4: the accumulator is zero and every slot has one writer, so
+ * slots are stored instead of read and added (not with bit 1).  Replaces SumLayer._backward_block_sparse_par_flows
  * (sum_layer.py:2284-2412) and _backward_sparse_par_flows (2904-3023). */
 int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const float* node_mars,
                            const float* element_mars, const float* params, const int* nids, const int* cids,
@@ -196,7 +201,8 @@ int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const flo
  * block offset, index-row offset, first element row of the layer, shared-slot flag};
  * tile_desc[t] = p for each of the n_tiles tiles.  tile_rows is 64 (block sizes >= 64, child
  * tiles of 64) or 32 (block size 32, child tiles of 128).  element_mars is the buffer that
- * holds every layer's element rows.  flags as pjamd_sum_backward_par (bit 1 unused).
+ * holds every layer's element rows.  flags as pjamd_sum_backward_par (bit 1 unused; bit 4 applies
+ * to the partitions without the shared-slot flag).
  * Same per-partition results as pjamd_sum_backward_par; the reference runs one launch per
  * layer inside SumLayer.backward (sum_layer.py:264-384) under TensorCircuit.backward
  * (tensorcircuit.py:344-538). */

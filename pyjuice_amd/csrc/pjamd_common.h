@@ -34,12 +34,19 @@ constexpr float kNegInf = -INFINITY;
 
 __device__ __forceinline__ float neg_inf() { return -INFINITY; }
 
-// Accumulate into a parameter-flow slot of type T (float or double), either with a plain
-// read-modify-write (the slot is owned by exactly one thread of the launch) or atomically.
+// How a kernel deposits a parameter flow into its slot (float or double):
+//   kPfAdd       plain read-modify-write (the slot is owned by exactly one thread of the launch)
+//   kPfAtomic    atomic add (slots shared within the launch)
+//   kPfOverwrite plain store: the slot is owned by one thread and the accumulator is known to be
+//                zero, so it need not be read (the first backward after the flows were cleared)
+enum { kPfAdd = 0, kPfAtomic = 1, kPfOverwrite = 2 };
+
 template <typename T>
-__device__ __forceinline__ void pf_add(T* p, T v, bool atomic) {
-    if (atomic) {
+__device__ __forceinline__ void pf_add(T* p, T v, int mode) {
+    if (mode == kPfAtomic) {
         atomicAdd(p, v);
+    } else if (mode == kPfOverwrite) {
+        *p = v;
     } else {
         *p += v;
     }

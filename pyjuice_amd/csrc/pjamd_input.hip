@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void cat_bwd_lds_kernel(PF* __restrict__ pflow
                                                           const int* __restrict__ data, const int* __restrict__ vids,
                                                           const int64_t* __restrict__ s_pfids,
                                                           const int* __restrict__ ncats, int64_t num_nodes,
-                                                          int64_t node_offset, int64_t B, int kmax, bool atomic_out) {
+                                                          int64_t node_offset, int64_t B, int kmax, int pf_mode) {
     extern __shared__ float hist[];
     constexpr int NR = 256 / kRowThreads;
     const int r = threadIdx.x / kRowThreads, sub = threadIdx.x % kRowThreads;
@@ -115,10 +115,15 @@ __global__ __launch_bounds__(256) void cat_bwd_lds_kernel(PF* __restrict__ pflow
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int k = k0 + q * kRowThreads + sub;
-            kk[q] = (k < K && h[k] != 0.0f) ? k : -1;
+            // a zero bin is skipped unless the slot is to be overwritten
+            kk[q] = (k < K && (h[k] != 0.0f || pf_mode == kPfOverwrite)) ? k : -1;
             hv[q] = kk[q] >= 0 ? h[k] : 0.0f;
         }
-        if (atomic_out) {
+        if (pf_mode == kPfOverwrite) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (kk[q] >= 0) pflows[f0 + kk[q]] = (PF)hv[q];
+        } else if (pf_mode == kPfAtomic) {
 #pragma unroll
             for (int q = 0; q < kPer; ++q)
                 if (kk[q] >= 0) atomicAdd(&pflows[f0 + kk[q]], (PF)hv[q]);
@@ -131,6 +136,151 @@ __global__ __launch_bounds__(256) void cat_bwd_lds_kernel(PF* __restrict__ pflow
                 if (kk[q] >= 0) pflows[f0 + kk[q]] = old[q] + (PF)hv[q];
         }
     }
+}
+
+// Categorical flow backward without atomics per sample, for tiles of kSortTile nodes that
+// share one variable and one category count K <= kSortK (the caller selects this kernel when
+// every tile does; a tile that does not falls back to per-sample global atomics here).  Per batch chunk of kSortB columns the
+// workgroup counting-sorts the variable's categories once (positions grouped by category), stages
+// the tile's flow rows in LDS, and each thread sums, for its categories, the segment of every
+// node.  The per-(node, category) sums stay in registers across chunks and are added to the
+// parameter flows at the end (reads before writes, or atomics when slots are shared).
+constexpr int kSortTile = 8;
+constexpr int kSortK = 1024;
+constexpr int kSortB = 1024;
+
+template <typename PF>
+__global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pflows,
+                                                             const float* __restrict__ node_flows,
+                                                             const int* __restrict__ data, const int* __restrict__ vids,
+                                                             const int64_t* __restrict__ s_pfids,
+                                                             const int* __restrict__ ncats, int64_t num_nodes,
+                                                             int64_t node_offset, int64_t B, int pf_mode) {
+    __shared__ int cnt[kSortK + 1];
+    __shared__ int cur[kSortK];
+    __shared__ short perm[kSortB];
+    __shared__ float fl[kSortTile][kSortB];
+    __shared__ int uniform;
+    constexpr int KPT = kSortK / 256;   // categories per thread
+    const int tid = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * kSortTile;
+    const int n_here = (int)((num_nodes - i0) < kSortTile ? (num_nodes - i0) : kSortTile);
+    const int v = vids[i0];
+    const int K = ncats[i0];
+    if (tid == 0) {
+        int ok = K <= kSortK;
+        for (int r = 1; r < n_here; ++r) ok &= (vids[i0 + r] == v) && (ncats[i0 + r] == K);
+        uniform = ok;
+    }
+    __syncthreads();
+    if (!uniform) {
+        if (pf_mode == kPfOverwrite) {
+            for (int r = 0; r < n_here; ++r)
+                for (int k = tid; k < ncats[i0 + r]; k += 256) pflows[s_pfids[i0 + r] + k] = (PF)0;
+            __syncthreads();
+        }
+        for (int r = 0; r < n_here; ++r) {
+            const int64_t i = i0 + r;
+            const int* xr = data + (int64_t)vids[i] * B;
+            const float* fr = node_flows + (node_offset + i) * B;
+            for (int64_t b = tid; b < B; b += 256) {
+                const int x = xr[b];
+                if (x >= 0 && x < ncats[i] && fr[b] != 0.0f) atomicAdd(&pflows[s_pfids[i] + x], (PF)fr[b]);
+            }
+        }
+        return;
+    }
+    const int* xrow = data + (int64_t)v * B;
+    float sums[kSortTile][KPT];
+#pragma unroll
+    for (int r = 0; r < kSortTile; ++r)
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) sums[r][q] = 0.0f;
+
+    for (int64_t c0 = 0; c0 < B; c0 += kSortB) {
+        const int nb = (int)((B - c0) < kSortB ? (B - c0) : kSortB);
+        for (int k = tid; k <= K; k += 256) cnt[k] = 0;
+        __syncthreads();
+        // category counts and the tile's flow rows of this chunk
+        for (int t = tid; t < nb; t += 256) {
+            const int x = xrow[c0 + t];
+            if (x >= 0 && x < K) atomicAdd(&cnt[x + 1], 1);
+        }
+        for (int t = tid; t < n_here * nb; t += 256) {
+            const int r = t / nb, c = t - r * nb;
+            fl[r][c] = node_flows[(node_offset + i0 + r) * B + c0 + c];
+        }
+        __syncthreads();
+        // inclusive scan of cnt[0..K] by one wavefront (cnt[k] becomes the first position of
+        // category k, since category x was counted at x + 1)
+        if (tid < 64) {
+            const int per = (K + 1 + 63) / 64;
+            const int base = tid * per;
+            int local = 0;
+            for (int q = 0; q < per; ++q)
+                if (base + q <= K) local += cnt[base + q];
+            int incl = local;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o, 64);
+                if (tid >= o) incl += y;
+            }
+            int run = incl - local;
+            for (int q = 0; q < per; ++q)
+                if (base + q <= K) { run += cnt[base + q]; cnt[base + q] = run; }
+        }
+        __syncthreads();
+        for (int k = tid; k < K; k += 256) cur[k] = cnt[k];
+        __syncthreads();
+        for (int t = tid; t < nb; t += 256) {
+            const int x = xrow[c0 + t];
+            if (x >= 0 && x < K) perm[atomicAdd(&cur[x], 1)] = (short)t;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            const int k = tid + q * 256;
+            if (k < K) {
+                const int p0 = cnt[k], p1 = cnt[k + 1];
+                for (int p = p0; p < p1; ++p) {
+                    const int c = perm[p];
+#pragma unroll
+                    for (int r = 0; r < kSortTile; ++r) sums[r][q] += fl[r][c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // parameter flows: reads first, then writes (or plain stores when the accumulator is zero)
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+        const int k = tid + q * 256;
+        if (k >= K) continue;
+        if (pf_mode == kPfOverwrite) {
+#pragma unroll
+            for (int r = 0; r < kSortTile; ++r)
+                if (r < n_here) pflows[s_pfids[i0 + r] + k] = (PF)sums[r][q];
+        } else if (pf_mode == kPfAtomic) {
+#pragma unroll
+            for (int r = 0; r < kSortTile; ++r)
+                if (r < n_here && sums[r][q] != 0.0f) atomicAdd(&pflows[s_pfids[i0 + r] + k], (PF)sums[r][q]);
+        } else {
+            PF old[kSortTile];
+#pragma unroll
+            for (int r = 0; r < kSortTile; ++r) old[r] = (r < n_here) ? pflows[s_pfids[i0 + r] + k] : (PF)0;
+#pragma unroll
+            for (int r = 0; r < kSortTile; ++r)
+                if (r < n_here) pflows[s_pfids[i0 + r] + k] = old[r] + (PF)sums[r][q];
+        }
+    }
+}
+
+// Clears the parameter-flow slots of every node (used before the atomic path when the caller
+// asked for the flows to be overwritten).
+template <typename PF>
+__global__ __launch_bounds__(256) void cat_zero_slots_kernel(PF* __restrict__ pflows, const int64_t* __restrict__ s_pfids,
+                                                             const int* __restrict__ ncats) {
+    const int64_t i = blockIdx.x;
+    for (int k = threadIdx.x; k < ncats[i]; k += blockDim.x) pflows[s_pfids[i] + k] = (PF)0;
 }
 
 // Categorical flow backward for very large category counts: direct global atomics.
@@ -153,7 +303,7 @@ template <typename PF>
 __global__ __launch_bounds__(256) void bern_bwd_kernel(PF* __restrict__ pflows, const float* __restrict__ node_flows,
                                                        const int* __restrict__ data, const int* __restrict__ vids,
                                                        const int64_t* __restrict__ s_pfids, int64_t node_offset,
-                                                       int64_t B, bool atomic_out) {
+                                                       int64_t B, int atomic_out) {
     __shared__ float red[2][4];
     const int64_t i = blockIdx.x;
     float s1 = 0.0f, s0 = 0.0f;
@@ -284,16 +434,21 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
 template <typename PF>
 static int input_backward_impl(int kind, PF* pflows, const float* node_flows, const int* data, const int* vids,
                                const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset,
-                               int64_t B, bool atomic_out, int kmax, hipStream_t stream) {
+                               int64_t B, int pf_mode, bool sorted_tiles, int kmax, hipStream_t stream) {
     if (kind == 1) {
         bern_bwd_kernel<PF><<<(unsigned)num_nodes, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids,
-                                                                    node_offset, B, atomic_out);
+                                                                    node_offset, B, pf_mode);
+    } else if (sorted_tiles && kmax <= kSortK) {
+        cat_bwd_sorted_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile), 256, 0, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);
     } else if (kmax <= kLdsFloats && B % 4 == 0 && ((((uintptr_t)data) | ((uintptr_t)node_flows)) % 16 == 0)) {
         constexpr int NR = 256 / kRowThreads;
         const size_t lds = (size_t)NR * kmax * sizeof(float);
         cat_bwd_lds_kernel<PF><<<(unsigned)cdiv(num_nodes, NR), 256, lds, stream>>>(
-            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kmax, atomic_out);
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kmax, pf_mode);
     } else {
+        if (pf_mode == kPfOverwrite)
+            cat_zero_slots_kernel<PF><<<(unsigned)num_nodes, 256, 0, stream>>>(pflows, s_pfids, ncats);
         dim3 grid((unsigned)num_nodes, cdiv(B, 256));
         cat_bwd_global_kernel<PF><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
                                                            node_offset, B);
@@ -302,21 +457,25 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
     return PJAMD_OK;
 }
 
-// `kmax_or_tied`: bit 0 = some flow ranges are shared by several input vectors (atomic adds);
-// bits 1.. = largest category count of the layer (Categorical) .
+// `layout`: bit 0 = some flow ranges are shared by several input vectors (atomic adds); bit 1 =
+// every tile of kSortTile consecutive nodes shares one variable and category count; bit 2 = the
+// accumulator is zero, store instead of add (not with bit 0); bits 3.. = largest category count
+// of the layer (Categorical).
 PJAMD_EXPORT int pjamd_input_backward(int kind, void* pflows, const float* node_flows, int pf64, const int* data,
                                       const int* vids, const int64_t* s_pfids, const int* ncats, int64_t num_nodes,
-                                      int64_t node_offset, int64_t B, int kmax_or_tied, hipStream_t stream) {
-    PJAMD_REQUIRE(num_nodes >= 0 && B > 0 && (kind == 0 || kind == 1));
+                                      int64_t node_offset, int64_t B, int layout, hipStream_t stream) {
+    PJAMD_REQUIRE(num_nodes >= 0 && B > 0 && (kind == 0 || kind == 1) && layout >= 0);
     if (num_nodes == 0) return PJAMD_OK;
-    const bool atomic_out = (kmax_or_tied & 1) != 0;
-    const int kmax = kmax_or_tied >> 1;
+    PJAMD_REQUIRE(!((layout & 1) && (layout & 4)));
+    const int pf_mode = (layout & 1) ? kPfAtomic : (layout & 4) ? kPfOverwrite : kPfAdd;
+    const bool sorted_tiles = (layout & 2) != 0;
+    const int kmax = layout >> 3;
     PJAMD_REQUIRE(kind == 1 || kmax > 0);
     if (pf64)
         return input_backward_impl<double>(kind, (double*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
-                                           node_offset, B, atomic_out, kmax, stream);
+                                           node_offset, B, pf_mode, sorted_tiles, kmax, stream);
     return input_backward_impl<float>(kind, (float*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
-                                      node_offset, B, atomic_out, kmax, stream);
+                                      node_offset, B, pf_mode, sorted_tiles, kmax, stream);
 }
 
 PJAMD_EXPORT int pjamd_input_em(int kind, float* params, const void* pflows, int pf64, const int* src,

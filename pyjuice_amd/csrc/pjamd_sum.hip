@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_tiled(PF* __restrict__ p
                                                               const float* __restrict__ params, const int* __restrict__ nids,
                                                               const int* __restrict__ cids, const int* __restrict__ pids,
                                                               const int* __restrict__ pfids, const int* __restrict__ nch,
-                                                              int K, int bs, int64_t B, bool modified, bool atomic_out,
+                                                              int K, int bs, int64_t B, bool modified, int atomic_out,
                                                               float sign) {
     constexpr int TB = 32;
     constexpr int RM = TM / 16, RN = TN / 16;
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__
                                                                 const float* __restrict__ params, const int* __restrict__ nids,
                                                                 const int* __restrict__ cids, const int* __restrict__ pids,
                                                                 const int* __restrict__ pfids, const int* __restrict__ nch,
-                                                                int K, int bs, int64_t B, bool modified, bool atomic_out,
+                                                                int K, int bs, int64_t B, bool modified, int atomic_out,
                                                                 float sign) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.y * kSmallWaves + w;
@@ -492,7 +492,7 @@ int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float*
                                 int precision, hipStream_t stream);
 int pjamd_sum_backward_par_mfma(void* pflows, bool pf64, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* nids, const int* cids, const int* pids, const int* pfids,
-                                const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, bool shared, float sign,
+                                const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, int pf_mode, float sign,
                                 int precision, hipStream_t stream);
 
 PJAMD_EXPORT int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
@@ -567,7 +567,7 @@ PJAMD_EXPORT int pjamd_sum_backward_ele(float* element_flows, const float* node_
 template <typename PF>
 static int sum_bwd_par_impl(PF* pflows, const float* node_flows, const float* node_mars, const float* element_mars,
                             const float* params, const int* nids, const int* cids, const int* pids, const int* pfids,
-                            const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, bool atomic_out,
+                            const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, int atomic_out,
                             float sign, hipStream_t stream) {
     if (bs >= 64) {
         sum_bwd_par_tiled<64, 64, PF><<<dim3((unsigned)((int64_t)n_blocks * (bs / 64)), cdiv(K, 64)), kThreads, 0, stream>>>(
@@ -591,14 +591,17 @@ static int sum_bwd_par_impl(PF* pflows, const float* node_flows, const float* no
 }
 
 // flags: bit 0 = node flows hold log(flow) - mar; bit 1 = flow slots shared inside the launch
-// (atomic adds); bit 2 = subtract instead of add; bit 3 = parameter flows are FP64.
+// (atomic adds); bit 2 = subtract instead of add; bit 3 = parameter flows are FP64; bit 4 = the
+// accumulator is zero: store instead of read-modify-write (not with bit 1).
 PJAMD_EXPORT int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const float* node_mars,
                                         const float* element_mars, const float* params, const int* nids, const int* cids,
                                         const int* pids, const int* pfids, const int* nch, int n_blocks, int K, int bs,
                                         int64_t B, int flags, int precision, hipStream_t stream) {
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_blocks == 0) return PJAMD_OK;
-    const bool mod = flags & 1, atomic_out = (flags & 2) != 0;
+    PJAMD_REQUIRE(!((flags & 2) && (flags & 16)));
+    const bool mod = flags & 1;
+    const int atomic_out = (flags & 2) ? kPfAtomic : (flags & 16) ? kPfOverwrite : kPfAdd;
     const float sign = (flags & 4) ? -1.0f : 1.0f;
     if (precision != 3 && bs >= 32)
         return pjamd_sum_backward_par_mfma(param_flows, (flags & 8) != 0, node_flows, node_mars, element_mars, params,

@@ -415,7 +415,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
                                          const int* __restrict__ cids, const int* __restrict__ pids,
                                          const int* __restrict__ pfids, const int* __restrict__ nch, int K, int64_t B,
                                          int j, int m0, int kt0, int64_t bstart, int64_t bend, bool modified,
-                                         bool atomic_out, float sign) {
+                                         int atomic_out, float sign) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
     constexpr int LD = TK + MF<T>::PAD;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
                                                     const int* __restrict__ cids, const int* __restrict__ pids,
                                                     const int* __restrict__ pfids, const int* __restrict__ nch, int K,
                                                     int64_t B, int64_t bchunk, int mtiles, int ktiles, int splits,
-                                                    bool modified, bool atomic_out, float sign) {
+                                                    bool modified, int atomic_out, float sign) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int m0 = (t % mtiles) * TM;
     const int kt0 = ((t / mtiles) % ktiles) * TN;
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, 
                                                        const int* __restrict__ pfids, const int* __restrict__ nch,
                                                        const int* __restrict__ desc,
                                                        const int* __restrict__ tile_desc, int64_t B, bool modified,
-                                                       float sign) {
+                                                       bool overwrite, float sign) {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int* D = desc + (int64_t)tile_desc[t] * PJAMD_PAR_DESC_FIELDS;
     const int local = t - D[0];
@@ -534,8 +534,9 @@ __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, 
     const int m0 = (local % mtiles) * TM;
     const int kt0 = ((local / mtiles) % ktiles) * TN;
     const int j = local / (mtiles * ktiles);
+    const int mode = D[7] ? kPfAtomic : (overwrite ? kPfOverwrite : kPfAdd);
     par_tile<T, TM, TN, PF>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io, pids + io,
-                            pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, D[7] != 0, sign);
+                            pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign);
 }
 
 // One 32 x 32 x KI product with the operand / accumulator lane layouts the kernels above assume:
@@ -656,7 +657,7 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
 template <typename T, typename PF>
 int launch_par(PF* pflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* nids, const int* cids, const int* pids, const int* pfids, const int* nch, int n_blocks, int K,
-               int bs, int64_t B, bool mod, bool shared, float sign, hipStream_t stream) {
+               int bs, int64_t B, bool mod, int pf_mode, float sign, hipStream_t stream) {
     const int TM = bs >= 64 ? 64 : 32, TN = bs >= 64 ? 64 : 128;
     const int mtiles = bs / TM, ktiles = (K + TN - 1) / TN;
     const int64_t tiles = (int64_t)n_blocks * mtiles * ktiles;
@@ -666,11 +667,11 @@ int launch_par(PF* pflows, const float* nflows, const float* nmars, const float*
     const int64_t max_splits = (B + TK - 1) / TK;
     if (splits > max_splits) splits = max_splits;
     if (splits > 4) splits = 4;
-    if (splits < 1) splits = 1;
+    if (splits < 1 || pf_mode == kPfOverwrite) splits = 1;   // a stored slot must have one writer
     int64_t bchunk = (B + splits - 1) / splits;
     bchunk = ((bchunk + TK - 1) / TK) * TK;
     splits = (B + bchunk - 1) / bchunk;
-    const bool atomic_out = shared || splits > 1;
+    const int atomic_out = (pf_mode == kPfAtomic || splits > 1) ? kPfAtomic : pf_mode;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles * splits));
     const unsigned grid = (unsigned)(tiles * splits);
     if (bs >= 64)
@@ -689,14 +690,14 @@ template <typename T, typename PF>
 int launch_par_batched(PF* pflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                        const int* nids, const int* cids, const int* pids, const int* pfids, const int* nch,
                        const int* desc, const int* tile_desc, int n_tiles, int tile_rows, int64_t B, bool mod,
-                       float sign, hipStream_t stream) {
+                       bool overwrite, float sign, hipStream_t stream) {
     PJAMD_REQUIRE(grid_fits(n_tiles));
     if (tile_rows == 64)
         sum_par_batched<T, 64, 64, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
-            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, sign);
+            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign);
     else
         sum_par_batched<T, 32, 128, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
-            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, sign);
+            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -758,7 +759,7 @@ PJAMD_EXPORT int pjamd_sum_backward_ele_to_children(float* node_flows, const flo
 // precision 2 runs this contraction in BF16.
 int pjamd_sum_backward_par_mfma(void* pflows, bool pf64, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* nids, const int* cids, const int* pids, const int* pfids,
-                                const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, bool shared, float sign,
+                                const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, int shared, float sign,
                                 int precision, hipStream_t stream) {
     if (bs < 32) return PJAMD_EUNSUPPORTED;
     if (precision == 0) {
@@ -795,20 +796,21 @@ PJAMD_EXPORT int pjamd_sum_backward_par_batched(void* param_flows, const float* 
     const bool mod = flags & 1;
     const float sign = (flags & 4) ? -1.0f : 1.0f;
     const bool pf64 = (flags & 8) != 0;
+    const bool overwrite = (flags & 16) != 0;
     if (precision == 0) {
         if (pf64)
             return launch_par_batched<float, double>((double*)param_flows, node_flows, node_mars, element_mars, params,
                                                      nids, cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows,
-                                                     B, mod, sign, stream);
+                                                     B, mod, overwrite, sign, stream);
         return launch_par_batched<float, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
                                                 cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod,
-                                                sign, stream);
+                                                overwrite, sign, stream);
     }
     if (pf64)
         return launch_par_batched<__bf16, double>((double*)param_flows, node_flows, node_mars, element_mars, params,
                                                   nids, cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B,
-                                                  mod, sign, stream);
+                                                  mod, overwrite, sign, stream);
     return launch_par_batched<__bf16, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
-                                             cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod, sign,
-                                             stream);
+                                             cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod,
+                                             overwrite, sign, stream);
 }

@@ -135,7 +135,7 @@ class InputLayer(Layer, nn.Module):
         self.params = nn.Parameter(torch.empty([max(self.num_parameters, 1)], dtype=torch.float32),
                                    requires_grad=False)
         self.num_vars_per_node = nvars
-        self.param_flows = None
+        self._pf, self._pf_fresh = None, False
         self.pflow_dtype = torch.float32
         self._used_external_params = False
         self._param_batch_size = 1
@@ -243,21 +243,48 @@ class InputLayer(Layer, nn.Module):
             "kmax": int(self._ncats.max().item()) if self.num_nodes > 0 else 1,
             # Several vectors accumulate into one flow range when tied vectors share it.
             "shared_pf": any(ns.is_tied() for ns in self.nodes),
+            "sorted_tiles": self._uniform_tiles(8),
         }
         assert self.num_vars_per_node == 1, "multivariate input distributions are not supported by the HIP layers"
+
+    def _uniform_tiles(self, tile: int) -> bool:
+        """True if every tile of ``tile`` consecutive nodes (from node 0) has one variable and one
+        category count (the backward kernel then sorts each tile's batch once)."""
+        v = self.vids[:, 0].detach().cpu().numpy()
+        k = self._ncats.detach().cpu().numpy()
+        n = v.shape[0]
+        if n == 0:
+            return False
+        pad = (-n) % tile
+        vp = np.concatenate([v, np.repeat(v[-1:], pad)]).reshape(-1, tile)
+        kp = np.concatenate([k, np.repeat(k[-1:], pad)]).reshape(-1, tile)
+        return bool(np.all(vp == vp[:, :1]) and np.all(kp == kp[:, :1]))
 
     def _kind(self):
         return self.nodes[0].dist.native_kind
 
+    @property
+    def param_flows(self):
+        """Parameter-flow accumulator of the layer.  After ``init_param_flows(0.0)`` it is only
+        marked zero (the next backward stores instead of adding); reading it first clears it."""
+        if self._pf_fresh and self._pf is not None:
+            self._pf.zero_()
+            self._pf_fresh = False
+        return self._pf
+
+    @param_flows.setter
+    def param_flows(self, value):
+        self._pf = value
+        self._pf_fresh = False
+
     def init_param_flows(self, flows_memory: float = 1.0, dtype: Optional[torch.dtype] = None):
         dtype = dtype if dtype is not None else self.pflow_dtype
-        if self.param_flows is None or self.param_flows.dtype != dtype or \
-                self.param_flows.device != self.params.device:
+        if self._pf is None or self._pf.dtype != dtype or self._pf.device != self.params.device:
             self.param_flows = torch.zeros([self.num_param_flows], device=self.params.device, dtype=dtype)
             self.pflow_dtype = dtype
         elif flows_memory < 1.0:
             if flows_memory == 0.0:
-                self.param_flows.zero_()
+                self._pf_fresh = True     # cleared lazily (see the param_flows property)
             else:
                 self.param_flows.mul_(flows_memory)
         return None
@@ -282,13 +309,18 @@ class InputLayer(Layer, nn.Module):
                  params: Optional[Dict] = None, logspace_flows: bool = False, **kwargs):
         """``param_flows[s_pfid_i + x] += node_flows[off + i, b]`` (``input_layer.py:406-571``)."""
         assert not logspace_flows, "logspace flows are not supported by the HIP input layers"
-        if self.param_flows is None:
+        if self._pf is None:
             self.init_param_flows(0.0)
         t = self._dev
         B = node_flows.size(1)
-        pf64 = 1 if self.param_flows.dtype == torch.float64 else 0
-        flags = (t["kmax"] << 1) | (1 if t["shared_pf"] else 0)
-        N.call("pjamd_input_backward", self._kind(), N.ptr(self.param_flows), N.ptr(node_flows), pf64, N.ptr(data),
+        # store instead of add into a buffer marked zero when every slot has one owner
+        overwrite = self._pf_fresh and not t["shared_pf"]
+        pf = self._pf if overwrite else self.param_flows
+        pf64 = 1 if pf.dtype == torch.float64 else 0
+        flags = (t["kmax"] << 3) | (4 if overwrite else 0) | (2 if t["sorted_tiles"] else 0) | \
+            (1 if t["shared_pf"] else 0)
+        self._pf_fresh = False
+        N.call("pjamd_input_backward", self._kind(), N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data),
                N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0],
                B, flags, N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B
                                             + 2 * self.param_flows.element_size() * self.num_param_flows, 0))

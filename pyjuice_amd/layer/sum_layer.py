@@ -185,7 +185,8 @@ class SumLayer(Layer, nn.Module):
                  allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
                  negate_pflows: bool = False, accumulate_ch_flows: bool = False, allow_neg_flows: bool = False,
                  force_use_fp32: bool = False, precision: Optional[str] = None, premodified: bool = False,
-                 child_tables: Optional[Sequence[torch.Tensor]] = None, **kwargs):
+                 child_tables: Optional[Sequence[torch.Tensor]] = None, overwrite_pflows: bool = False,
+                 **kwargs):
         """Element flows and (optionally) parameter flows of the layer (``sum_layer.py:264-384``).
 
         ``allow_modify_flows`` rewrites the layer's node flows in place as ``log(flow) - mar``
@@ -231,6 +232,8 @@ class SumLayer(Layer, nn.Module):
         if param_flows is not None:
             pf64 = 1 if param_flows.dtype == torch.float64 else 0
             flags = mod | (2 if self._pf_shared else 0) | (4 if negate_pflows else 0) | (8 * pf64)
+            if overwrite_pflows and not self._pf_shared:
+                flags |= 16
             pfw = 8 if pf64 else 4
             for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
                 algo = (4 * (nids.numel() * bs * (1 if mod else 2) + tot) * B + (4 + 2 * pfw) * tot * bs,

@@ -80,7 +80,7 @@ class TensorCircuit(nn.Module):
         self.element_mars = None
         self.node_flows = None
         self.element_flows = None
-        self.param_flows = None
+        self._pf, self._pf_fresh, self._pf_overwrite_ok = None, False, False
 
         self._init_layers(layer_sparsity_tol=layer_sparsity_tol, max_num_partitions=max_num_partitions,
                           max_tied_ns_per_parflow_block=max_tied_ns_per_parflow_block, verbose=verbose)
@@ -95,6 +95,23 @@ class TensorCircuit(nn.Module):
         self.modify_flows_in_place = True
         self._par_batches = {"tables": dict(), "layers": set()}
         self._fused = {"tables": dict(), "prod_groups": set(), "premod": set()}
+
+    # ------------------------------------------------------------------ parameter-flow buffer
+
+    @property
+    def param_flows(self):
+        """Sum parameter-flow accumulator.  After ``init_param_flows(0.0)`` the buffer is only
+        marked zero (the next flow backward stores into the slots it owns instead of adding);
+        reading it before that clears it."""
+        if self._pf_fresh and self._pf is not None:
+            self._pf.zero_()
+            self._pf_fresh = False
+        return self._pf
+
+    @param_flows.setter
+    def param_flows(self, value):
+        self._pf = value
+        self._pf_fresh = False
 
     # ------------------------------------------------------------------ device placement
 
@@ -125,7 +142,29 @@ class TensorCircuit(nn.Module):
             }
             self._par_batches = self._build_par_batches(device)
             self._fused = self._build_fused_backward(device)
+            self._pf_overwrite_ok = self._sum_flows_overwritable()
         return self
+
+    def _sum_flows_overwritable(self) -> bool:
+        """True if one flow backward writes every sum parameter-flow slot exactly once with no
+        atomics: no tied sharing inside or across layers, every slot owned by one (node, child)
+        edge.  Then the first backward after the flows were cleared may store instead of add."""
+        if any(t["desc"].view(-1, 8)[:, 7].any().item() for t in self._par_batches["tables"].values()):
+            return False
+        cover = np.zeros(self.num_param_flows, dtype=np.int64)
+        for g in self.inner_layer_groups:
+            if g.is_prod():
+                continue
+            for layer in g:
+                if layer._pf_shared:
+                    return False
+                for nids, cids, pids, pfids, nch, tot in layer._dev["fw"]:
+                    f = pfids.detach().cpu().numpy().astype(np.int64)
+                    n = nch.detach().cpu().numpy().astype(np.int64)
+                    real = np.arange(f.shape[1])[None, :] < n[:, None]
+                    slots = (f[real][:, None] + np.arange(layer.block_size)[None, :]).reshape(-1)
+                    np.add.at(cover, slots, 1)
+        return bool(np.all(cover == 1))
 
     def _build_fused_backward(self, device):
         """Sum layers whose element-flow kernel also performs the product backward below them
@@ -275,10 +314,10 @@ class TensorCircuit(nn.Module):
                 batched_layers.update(id(layer) for layer in g if layer.block_size >= 32)
         return {"tables": batches, "layers": batched_layers}
 
-    def _run_par_batched(self, modify: bool, precision: str):
-        pf = self.param_flows
+    def _run_par_batched(self, modify: bool, precision: str, overwrite: bool):
+        pf = self._pf
         pf64 = pf.dtype == torch.float64
-        flags = (1 if modify else 0) | (8 if pf64 else 0)
+        flags = (1 if modify else 0) | (8 if pf64 else 0) | (16 if overwrite else 0)
         prec = PRECISIONS[precision]
         B = self.node_flows.size(1)
         for tm, t in self._par_batches["tables"].items():
@@ -410,7 +449,7 @@ class TensorCircuit(nn.Module):
         torch.cuda.synchronize()
         return g
 
-    def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision):
+    def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision, overwrite=False):
         # Element marginals of every layer are still resident (one slab per product group), so no
         # product layer is recomputed here (the reference recomputes, tensorcircuit.py:446-447).
         # ``allow_modify_flows`` permits the in-place log(flow) - mar transform of the sum nodes'
@@ -432,13 +471,13 @@ class TensorCircuit(nn.Module):
                     g.backward(self.node_flows, self.element_flows)
                 continue
             for layer in g:
-                pf = self.param_flows if compute_param_flows and id(layer) not in batched else None
+                pf = self._pf if compute_param_flows and id(layer) not in batched else None
                 layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
                                param_flows=pf, allow_modify_flows=modify, precision=precision,
                                premodified=modify and id(layer) in fz["premod"],
-                               child_tables=fz["tables"].get(id(layer)))
+                               child_tables=fz["tables"].get(id(layer)), overwrite_pflows=overwrite)
         if batched:
-            self._run_par_batched(modify, precision)
+            self._run_par_batched(modify, precision, overwrite)
 
     def backward(self, inputs: Optional[torch.Tensor] = None, ll_weights: Optional[torch.Tensor] = None,
                  compute_param_flows: bool = True, flows_memory: float = 1.0,
@@ -476,9 +515,15 @@ class TensorCircuit(nn.Module):
                 self._cum_flow += (r1 - r0) * B if ll_weights is None else float(ll_weights.sum().item())
                 self.init_param_flows(flows_memory=flows_memory)
 
+            # The first backward after init_param_flows(0.0) stores the sum parameter flows instead
+            # of adding them when every slot has exactly one writer; otherwise the buffer is
+            # cleared first.
+            overwrite = compute_param_flows and self._pf_fresh and self._pf_overwrite_ok
+            if compute_param_flows and not overwrite:
+                _ = self.param_flows
             sig = (1, id(self.node_flows), id(self.element_flows), id(self.node_mars), id(self.element_mars),
-                   id(self.params), id(self.param_flows), B, allow_modify_flows, compute_param_flows, prec)
-            run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec)
+                   id(self.params), id(self._pf), B, allow_modify_flows, compute_param_flows, prec, overwrite)
+            run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec, overwrite)
             if record_cudagraph and sig not in self._recorded_cuda_graphs:
                 run()  # this call's result; the capture below does not execute kernels
                 self._recorded_cuda_graphs[sig] = self._capture(run)
@@ -486,6 +531,8 @@ class TensorCircuit(nn.Module):
                 self._recorded_cuda_graphs[sig].replay()
             else:
                 run()
+            if compute_param_flows:
+                self._pf_fresh = False
 
             if input_layer_fn is None:
                 data = self._data_vb if inputs is None else self._to_vb(inputs)
@@ -539,11 +586,11 @@ class TensorCircuit(nn.Module):
         assert 0.0 <= flows_memory <= 1.0, "`flows_memory` should be in [0.0, 1.0]"
         assert batch_size is None, "per-sample parameter flows are not supported"
         dev = self.device
-        if self.param_flows is None or self.param_flows.device != dev or self.param_flows.dtype != self.pflow_dtype:
+        if self._pf is None or self._pf.device != dev or self._pf.dtype != self.pflow_dtype:
             self.param_flows = torch.zeros([self.num_param_flows], device=dev, dtype=self.pflow_dtype)
         elif flows_memory != 1.0:
             if flows_memory == 0.0:
-                self.param_flows.zero_()
+                self._pf_fresh = True     # cleared lazily (see the param_flows property)
             else:
                 self.param_flows.mul_(flows_memory)
         for layer in self.input_layer_group:

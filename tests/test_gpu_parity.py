@@ -166,3 +166,28 @@ def test_mfma_lane_layout(precision, ki, device):
     N.call("pjamd_debug_mfma_probe", precision, N.ptr(A), N.ptr(Bm), N.ptr(C), N.stream())
     torch.cuda.synchronize()
     assert torch.equal(C.cpu(), (A.cpu() @ Bm.cpu()))
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs64", "hmm_small", "simple_model"])
+def test_param_flow_accumulation(name, device):
+    """init_param_flows(0) marks the accumulators zero: the next backward stores into the slots it
+    owns; later backwards add.  Two identical backwards after clearing give twice one backward's
+    parameter flows (sum and input layers), and reading the flows right after clearing gives 0."""
+    U.say(f"[pflow accumulate] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision="bf16", **kw)
+    pc.to(device)
+    X = torch.from_numpy(U.random_data(cards, 32, seed=4)).to(device)
+    pc(X)
+    pc.backward(X, flows_memory=0.0)
+    one = pc.param_flows.detach().clone()
+    one_in = [l.param_flows.detach().clone() for l in pc.input_layer_group]
+    pc.init_param_flows(0.0)
+    assert float(pc.param_flows.abs().max()) == 0.0
+    assert all(float(l.param_flows.abs().max()) == 0.0 for l in pc.input_layer_group)
+    pc.init_param_flows(0.0)
+    pc.backward(X, flows_memory=1.0)
+    pc.backward(X, flows_memory=1.0)
+    assert U.rel_err(pc.param_flows.cpu().numpy(), 2 * one.cpu().numpy()) <= 1e-5
+    for l, o in zip(pc.input_layer_group, one_in):
+        assert U.rel_err(l.param_flows.cpu().numpy(), 2 * o.cpu().numpy()) <= 1e-5
