@@ -8,9 +8,9 @@ from . import utils, graph, nodes
 from .nodes import (inputs, multiply, summate, set_block_size, structural_properties, distributions,
                     CircuitNodes, InputNodes, ProdNodes, SumNodes)
 from .layer import InputLayer, ProdLayer, SumLayer, LayerGroup
-from .model import TensorCircuit, compile
+from .model import TensorCircuit, compile, log_likelihood
 from . import structures, optim, queries, transformations
 
-__all__ = ["compile", "TensorCircuit", "inputs", "multiply", "summate", "set_block_size",
+__all__ = ["compile", "TensorCircuit", "log_likelihood", "inputs", "multiply", "summate", "set_block_size",
            "structural_properties", "distributions", "structures", "optim", "queries", "transformations",
            "InputLayer", "ProdLayer", "SumLayer", "LayerGroup"]

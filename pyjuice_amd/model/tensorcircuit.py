@@ -140,10 +140,38 @@ class TensorCircuit(nn.Module):
                 "fuse": [(int(t), int(n), torch.tensor(ch, dtype=torch.int64, device=device))
                          for t, n, ch in self.parflow_fusing_kwargs],
             }
+            self._mask_unreached_products()
             self._par_batches = self._build_par_batches(device)
             self._fused = self._build_fused_backward(device)
             self._pf_overwrite_ok = self._sum_flows_overwritable()
         return self
+
+    def _mask_unreached_products(self):
+        """A product block that no sum edge of the layer above points to (sparse block
+        connections, e.g. PD's) receives no flow, and the sum layer's element-flow pass does not
+        write its rows of the shared element-flow buffer.  Its entries in the product layers'
+        backward tables are set to padding (0), so the product backward adds nothing for it
+        instead of reading rows left over from another layer."""
+        groups = self.inner_layer_groups
+        for gid, g in enumerate(groups):
+            if not g.is_prod() or gid + 1 >= len(groups):
+                continue
+            covered = set()
+            for sl in groups[gid + 1]:
+                for chids in sl.partitioned_chids:
+                    covered.update(chids.detach().cpu().numpy().astype(np.int64).tolist())
+            for layer in g:
+                if layer._dev is None:
+                    continue
+                bk = []
+                for u, parids, words in layer._dev["bk"]:
+                    p = parids.detach().cpu().numpy()
+                    keep = np.isin(p, np.fromiter(covered, dtype=np.int64, count=len(covered))) | (p == 0)
+                    if not keep.all():
+                        p = np.where(keep, p, 0).astype(np.int32)
+                        parids = torch.from_numpy(p).to(parids.device).contiguous()
+                    bk.append((u, parids, words))
+                layer._dev["bk"] = bk
 
     def _sum_flows_overwritable(self) -> bool:
         """True if one flow backward writes every sum parameter-flow slot exactly once with no

@@ -53,6 +53,14 @@ def pd_small(seed=0):
     return ns, [6] * 16
 
 
+def pd_sparse(seed=0):
+    """PD whose sparse block connections leave some product blocks without a parent edge."""
+    torch.manual_seed(seed)
+    ns = juice.structures.PD(data_shape=(8, 8), num_latents=16, split_intervals=(2, 2),
+                             structure_type="sum_dominated", input_dist=dists.Categorical(num_cats=4))
+    return ns, [4] * 64
+
+
 def hmm_small(seed=0, T=6, L=32, V=10):
     torch.manual_seed(seed)
     ns = juice.structures.HMM(seq_length=T, num_latents=L, num_emits=V, homogeneous=True)

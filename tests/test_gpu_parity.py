@@ -43,11 +43,14 @@ def _circuit(name):
     if name == "hmm_small":
         ns, cards = U.hmm_small()
         return ns, cards, dict()
+    if name == "pd_sparse":
+        ns, cards = U.pd_sparse()
+        return ns, cards, dict()
     raise KeyError(name)
 
 
 CIRCUITS = ["simple_model", "tying_block1", "hclt_bern16", "hclt_cat_bs32", "hclt_cat_bs64", "hclt_cat_bs128",
-            "pd_small", "hmm_small"]
+            "pd_small", "pd_sparse", "hmm_small"]
 
 # (LL nats/var, node-mar tol per scope variable, node-mar offset, relative flow / parameter-flow
 # tol, EM abs tol).  A single BF16 sum node can be off by ~2^-8 (operand rounding), hence the
@@ -191,3 +194,24 @@ def test_param_flow_accumulation(name, device):
     assert U.rel_err(pc.param_flows.cpu().numpy(), 2 * one.cpu().numpy()) <= 1e-5
     for l, o in zip(pc.input_layer_group, one_in):
         assert U.rel_err(l.param_flows.cpu().numpy(), 2 * o.cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["pd_sparse", "hclt_cat_bs32", "hmm_small"])
+@pytest.mark.parametrize("precision", ["bf16", "fp32_valu"])
+def test_backward_is_repeatable(name, precision, device):
+    """Two identical flow backwards give the same parameter flows (up to the order of FP64 adds):
+    no pass may read buffer rows that this backward did not write."""
+    U.say(f"[repeatable] {name} {precision}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = torch.from_numpy(U.random_data(cards, 48, seed=6)).to(device)
+    outs = []
+    for _ in range(2):
+        pc(X)
+        pc.backward(X, flows_memory=0.0)
+        outs.append((pc.param_flows.detach().cpu().numpy().copy(),
+                     [l.param_flows.detach().cpu().numpy().copy() for l in pc.input_layer_group]))
+    assert U.rel_err(outs[0][0], outs[1][0]) <= 1e-6
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert U.rel_err(a, b) <= 1e-6

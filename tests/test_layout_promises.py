@@ -39,3 +39,30 @@ def test_child_run_promise(maker, min_run):
                     runs.append(r)
     if min_run:
         assert runs and min(runs) >= min_run
+
+
+def test_unreached_products_are_masked():
+    """PD's sparse block connections leave product blocks without any parent edge; the product
+    backward tables must not read their (unwritten) element-flow rows."""
+    import torch
+    from pyjuice_amd.model import TensorCircuit
+    pc = TensorCircuit(U.pd_sparse()[0])
+    G = pc.inner_layer_groups
+    for gid, g in enumerate(G):
+        if g.is_prod():
+            for layer in g:
+                layer._dev = {"bk": [(u, p.to(torch.int32), 0) for u, p in
+                                     zip(layer.partitioned_u_cids, layer.partitioned_parids)]}
+    before = sum(int((p != 0).sum()) for g in G if g.is_prod() for l in g for _, p, _ in l._dev["bk"])
+    pc._mask_unreached_products()
+    after = sum(int((p != 0).sum()) for g in G if g.is_prod() for l in g for _, p, _ in l._dev["bk"])
+    assert after < before        # this circuit does have unreached product blocks
+    for gid in range(0, len(G) - 1, 2):
+        covered = set()
+        for sl in G[gid + 1]:
+            for c in sl.partitioned_chids:
+                covered.update(c.tolist())
+        for layer in G[gid]:
+            for _, p, _ in layer._dev["bk"]:
+                vals = set(p.flatten().tolist()) - {0}
+                assert vals <= covered
