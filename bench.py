@@ -33,10 +33,9 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0
 # profiles/hclt784_L256_B512_bf16_hbm_traffic.txt.  Reported as roofline.traffic for the dominant
 # kernel (counters cannot be read from inside the timed process).
 TRAFFIC_MB_PER_LAUNCH = {
-    "pjamd_sum_backward_ele": 20.76, "pjamd_sum_forward": 15.51, "pjamd_sum_backward_par_batched": 1330.06,
-    "pjamd_prod_forward": 16.39, "pjamd_prod_backward": 33.62, "pjamd_input_backward": 1090.38,
-    "pjamd_input_forward": 843.47, "pjamd_sum_em": 854.16, "pjamd_input_em": 847.78,
-    "pjamd_sum_modify_flows": 16.14,
+    "pjamd_sum_backward_ele_to_children": 34.13, "pjamd_sum_forward": 15.15,
+    "pjamd_sum_backward_par_batched": 1141.55, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 981.71,
+    "pjamd_input_forward": 652.21, "pjamd_sum_em": 570.03, "pjamd_input_em": 847.78,
 }
 HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
                "pjamd_input_em", "pjamd_sum_em", "pjamd_sum_modify_flows", "pjamd_pflow_fuse")

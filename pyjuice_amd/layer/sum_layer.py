@@ -185,16 +185,16 @@ class SumLayer(Layer, nn.Module):
                  allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
                  negate_pflows: bool = False, accumulate_ch_flows: bool = False, allow_neg_flows: bool = False,
                  force_use_fp32: bool = False, precision: Optional[str] = None, premodified: bool = False,
-                 child_tables: Optional[Sequence[torch.Tensor]] = None, overwrite_pflows: bool = False,
+                 child_tables: Optional[Sequence[tuple]] = None, overwrite_pflows: bool = False,
                  **kwargs):
         """Element flows and (optionally) parameter flows of the layer (``sum_layer.py:264-384``).
 
         ``allow_modify_flows`` rewrites the layer's node flows in place as ``log(flow) - mar``
         (``sum_layer.py:1381-1392``); the element / parameter-flow kernels read either form.
         ``premodified``: the flows already arrive in that form (written by the fused kernel of the
-        layer above).  ``child_tables`` (one per backward partition, see
-        ``pjamd_sum_backward_ele_to_children``): write the flows of the product layer's children
-        directly into ``node_flows`` instead of element flows.
+        layer above).  ``child_tables`` (one ``(table, child blocks, of which modified)`` per
+        backward partition, see ``pjamd_sum_backward_ele_to_children``): write the flows of the
+        product layer's children directly into ``node_flows`` instead of element flows.
         """
         assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
         assert not logspace_flows and not allow_neg_flows, "unsupported flow mode for the HIP sum layers"
@@ -212,11 +212,11 @@ class SumLayer(Layer, nn.Module):
         mod = 1 if allow_modify_flows else 0
         for q, (chids, parids, parpids, npar, cbs, totp) in enumerate(self._dev["bk"]):
             if child_tables is not None:
-                tab = child_tables[q]
+                tab, n_blk, n_mod = child_tables[q]
                 # parent rows (and marginals unless modified) + element rows read, one write per
                 # child row (+ its marginal when stored as log(flow) - mar), parameters once
-                n_ch = int(tab.size(0)) * cbs * int(tab.size(1))
-                algo = (4 * (totp * bs * (1 if mod else 2) + chids.numel() * cbs + n_ch * (2 if mod else 1)) * B
+                n_rd = n_mod if mod else 0
+                algo = (4 * (totp * bs * (1 if mod else 2) + chids.numel() * cbs + (n_blk + n_rd) * cbs) * B
                         + 4 * totp * bs * cbs, 2 * totp * bs * cbs * B)
                 N.call("pjamd_sum_backward_ele_to_children", N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids),

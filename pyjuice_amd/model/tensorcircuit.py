@@ -269,7 +269,9 @@ class TensorCircuit(nn.Module):
             for T in tabs:
                 code = 2 * T + (premod_row[T] & (T != 0)).astype(np.int64)
                 assert code.max() < 2 ** 31
-                out.append(torch.from_numpy(code).to(torch.int32).to(device).contiguous())
+                # (table, child blocks written, of which stored as log(flow) - mar)
+                out.append((torch.from_numpy(code).to(torch.int32).to(device).contiguous(),
+                            int((code != 0).sum()), int((code & 1).sum())))
             tables[lid] = out
         return {"tables": tables, "prod_groups": fused_prod, "premod": premod}
 
