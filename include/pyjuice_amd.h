@@ -136,14 +136,20 @@ int pjamd_prod_backward(float* node_flows, const float* element_flows, const int
 
 /* ----------------------------------------------------------------------------- sum layers */
 
+/* Layout promise bit of pjamd_sum_forward's child_run argument. */
+#define PJAMD_PARAMS_ALIGNED (1 << 30)
+
 /* node_mars[nids[j] + i][b] = log sum_{k < n_chs[j]} params[pids[j][k] + i] *
  * exp(element_mars[cids[j][k]][b]) for one forward partition (n_blocks node blocks of bs nodes,
  * index rows of width K).  child_run is a layout promise (0 = none): every child_run-aligned
  * group of index entries names consecutive element rows, cids[j][k0 + i] = cids[j][k0] + i, with
  * parameters bs apart, pids[j][k0 + i] = pids[j][k0] + i * bs; the compiler's index rows keep it
  * for child blocks of that size (PyJuice's block-sparse kernels read them the same way through
- * cids_start / cids_increment).  Replaces SumLayer.forward (src/pyjuice/layer/sum_layer.py:220-262)
- * with _forward_block_sparse (sum_layer.py:802-966) and _forward_sparse (1106-1214). */
+ * cids_start / cids_increment).  child_run may carry PJAMD_PARAMS_ALIGNED: every pids entry of
+ * the partition's real children is a multiple of 4 (16-byte-aligned parameter rows, read with
+ * 16-byte loads when B % 4 == 0 too).  Replaces SumLayer.forward
+ * (src/pyjuice/layer/sum_layer.py:220-262) with _forward_block_sparse (sum_layer.py:802-966) and
+ * _forward_sparse (1106-1214). */
 int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* params, const int* nids,
                       const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
                       int child_run, int precision, pjamd_stream_t stream);
@@ -157,7 +163,8 @@ int pjamd_sum_modify_flows(float* node_flows, const float* node_mars, const int*
 /* element_flows[chids[j] + c][b] (+)= exp(element_mars[..][b]) * sum_{k < n_pars[j], i < bs}
  * params[parpids[j][k] + c * bs + i] * node_flows[parids[j][k] + i][b] / exp(node_mars[..][b])
  * for the n_cblocks child blocks (cbs rows) of one backward partition.  flags bit 0: node flows
- * already hold log(flow) - mar; bit 1: add to element_flows.  Replaces
+ * already hold log(flow) - mar; bit 1: add to element_flows; bit 2: every parpids entry of the
+ * real parents is a multiple of 4 (16-byte-aligned parameter rows).  Replaces
  * SumLayer._backward_block_sparse_ele_flows (sum_layer.py:1891-2048) and
  * _backward_sparse_ele_flows (2668-2791). */
 int pjamd_sum_backward_ele(float* element_flows, const float* node_flows, const float* node_mars,
@@ -172,7 +179,8 @@ int pjamd_sum_backward_ele(float* element_flows, const float* node_flows, const 
  * the product block's size (cbs).  ch_nodes is int32 [n_cblocks][Kc]: for the product block
  * chids[j], 2 * (first node of its k-th child block) + 1 if that child's flows are to be stored as
  * log(flow) - mar, 0 = padding.  flags bit 0: the parents' flows hold log(flow) - mar; bit 1:
- * honour the log(flow) - mar marks of ch_nodes.  Replaces SumLayer._backward_block_sparse_ele_flows
+ * honour the log(flow) - mar marks of ch_nodes; bit 2: as pjamd_sum_backward_ele.  Replaces
+ * SumLayer._backward_block_sparse_ele_flows
  * (sum_layer.py:1891-2048) followed by ProdLayer.backward (prod_layer.py:206-261) and, for the
  * children, SumLayer._bk_triton_modify_flow (sum_layer.py:1439-1515). */
 int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars, const float* element_mars,
@@ -211,6 +219,48 @@ int pjamd_sum_backward_par_batched(void* param_flows, const float* node_flows, c
                                    const int* pids, const int* pfids, const int* n_chs, const int* desc,
                                    const int* tile_desc, int n_tiles, int tile_rows, int64_t B, int flags,
                                    int precision, pjamd_stream_t stream);
+
+/* ----------------------------------------------------------------------------- layer chains */
+
+/* Number of int64 fields per operation of a chain, and the longest reduction (children of a
+ * node block, or parent rows of a child block) a chain operation stages: PJAMD_CHAIN_MAX_K with
+ * BF16 / FP16 operands, half of it with FP32 operands. */
+#define PJAMD_CHAIN_OP_FIELDS 16
+#define PJAMD_CHAIN_MAX_K 1024
+
+/* A run of consecutive inner layers in one launch, split over the batch: workgroup g computes
+ * columns [32 g, 32 g + 32) of every operation in order (batch columns never interact), with a
+ * workgroup barrier between operations.  ops is int64 [n_ops][PJAMD_CHAIN_OP_FIELDS] in device
+ * memory, idx the int32 index tables the operations point into:
+ *   {kind, n_blocks, K, bs, cbs, element row offset, a, b, c, d, e, Kc, flags, 0, 0, 0}
+ *   kind 0, product partition: a = nids, b = cids (as pjamd_prod_forward, rows of the element
+ *           slab that starts at the element row offset);
+ *   kind 1, sum partition forward: a = nids, b = cids, c = pids, d = n_chs (as
+ *           pjamd_sum_forward; bs a power of two >= 32; flags bit 2: every aligned group of 8
+ *           entries of a cids / pids row is a run, cids consecutive and pids bs apart).
+ * An operation stages n_blocks * K index entries, and the reductions of its blocks (children per
+ * block, or parent rows per child block for kind 2) each rounded up to 64, in at most the staged
+ * length; at most 64 blocks.
+ * Results equal the per-layer entry points' up to the order of floating-point additions.
+ * Replaces the forward loop over the inner layer groups of TensorCircuit.forward
+ * (tensorcircuit.py:269-283) for the layers of the run. */
+int pjamd_chain_forward(float* node_mars, float* element_mars, const float* params, const int64_t* ops, int n_ops,
+                        const int* idx, int64_t B, int precision, pjamd_stream_t stream);
+
+/* The backward of a run of layers, top-down, same split and layout as pjamd_chain_forward:
+ *   kind 2, backward partition of a sum layer fused with the product layer below it: a = chids,
+ *           b = parids, c = parpids, d = n_pars, e = ch_nodes with row width Kc, cbs the child
+ *           block size (as pjamd_sum_backward_ele_to_children; flags bit 0: the parents' flows
+ *           hold log(flow) - mar; bit 1: honour the log(flow) - mar marks of ch_nodes; bit 2:
+ *           every parpids entry is a multiple of 4 and bs is, so parameter rows are 16-byte
+ *           aligned);
+ *   kind 3, modify pass of a sum partition: a = nids, bs (as pjamd_sum_modify_flows).
+ * Replaces the backward loop over the inner layer groups of TensorCircuit.backward
+ * (tensorcircuit.py:436-470) for the layers of the run (their parameter flows come from
+ * pjamd_sum_backward_par_batched afterwards). */
+int pjamd_chain_backward(float* node_flows, const float* node_mars, const float* element_mars, const float* params,
+                         const int64_t* ops, int n_ops, const int* idx, int64_t B, int precision,
+                         pjamd_stream_t stream);
 
 /* ----------------------------------------------------------------------------- EM */
 

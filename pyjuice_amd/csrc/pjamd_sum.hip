@@ -489,7 +489,7 @@ int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const fl
 int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* chids, const int* parids, const int* parpids,
                                 const int* npar, int n, int KP, int cbs, int bs, int64_t B, bool mod, bool accum,
-                                int precision, hipStream_t stream);
+                                bool aligned, int precision, hipStream_t stream);
 int pjamd_sum_backward_par_mfma(void* pflows, bool pf64, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* nids, const int* cids, const int* pids, const int* pfids,
                                 const int* nch, int n_blocks, int K, int bs, int64_t B, bool mod, int pf_mode, float sign,
@@ -538,10 +538,11 @@ PJAMD_EXPORT int pjamd_sum_backward_ele(float* element_flows, const float* node_
                                         int cbs, int bs, int64_t B, int flags, int precision, hipStream_t stream) {
     PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && cbs >= 1 && bs >= 1 && B > 0 && precision >= 0 && precision <= 3);
     if (n_cblocks == 0) return PJAMD_OK;
-    const bool mod = flags & 1, accum = (flags & 2) != 0;
+    const bool mod = flags & 1, accum = (flags & 2) != 0, aligned = (flags & 4) != 0;
     if (precision != 3 && cbs >= 32)
         return pjamd_sum_backward_ele_mfma(element_flows, node_flows, node_mars, element_mars, params, chids, parids,
-                                           parpids, npar, n_cblocks, KP, cbs, bs, B, mod, accum, precision, stream);
+                                           parpids, npar, n_cblocks, KP, cbs, bs, B, mod, accum, aligned, precision,
+                                           stream);
     const unsigned gy = cdiv(B, kTN);
     if (cbs >= 64) {
         sum_bwd_ele_tiled<64><<<dim3((unsigned)((int64_t)n_cblocks * (cbs / 64)), gy), kThreads, 0, stream>>>(

@@ -15,56 +15,14 @@
 // Grids are one-dimensional over logical tiles.  The hardware hands workgroup i to XCD i % 8;
 // xcd_tile() gives each XCD a contiguous range of logical tiles so that the tiles of one node
 // block (which share parameters and child rows) meet in the same L2.
-#include "pjamd_common.h"
+#include "pjamd_mfma.h"
 
 using namespace pjamd;
 
 namespace {
 
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
-
 constexpr int TK = 128;
 constexpr int kXcds = 8;
-
-template <typename T>
-struct MF;
-
-template <>
-struct MF<__bf16> {
-    static constexpr int KPL = 8;  // reduction values per lane per instruction (instruction K = 16)
-    static constexpr int PAD = 8;  // LDS row stride 272 B: 16-byte aligned, 4-bank shift per row
-    static __device__ __forceinline__ floatx16 mma(const __bf16* a, const __bf16* b, floatx16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(a),
-                                                       *reinterpret_cast<const bf16x8*>(b), c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
-};
-
-template <>
-struct MF<_Float16> {
-    static constexpr int KPL = 8;
-    static constexpr int PAD = 8;
-    static __device__ __forceinline__ floatx16 mma(const _Float16* a, const _Float16* b, floatx16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const halfx8*>(a),
-                                                      *reinterpret_cast<const halfx8*>(b), c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ _Float16 cvt(float x) { return (_Float16)x; }
-};
-
-template <>
-struct MF<float> {
-    static constexpr int KPL = 1;  // instruction K = 2
-    static constexpr int PAD = 1;  // LDS row stride 129 words: lanes of a column read distinct banks
-    static __device__ __forceinline__ floatx16 mma(const float* a, const float* b, floatx16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x2f32(*a, *b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ float cvt(float x) { return x; }
-};
-
-// Row of accumulator register r for this lane inside a 32 x 32 tile (the column is lane & 31).
-__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
 template <typename T, int LD>
 __device__ __forceinline__ floatx16 stage_mma(const T (*As)[LD], const T (*Bs)[LD], int wm, int wn, int lane,
@@ -85,13 +43,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + s;
 }
 
-__device__ __forceinline__ float lfdm_at(const float* nflows, const float* nmars, int64_t o, bool modified) {
-    if (modified) return nflows[o];
-    const float m = nmars[o];
-    return (m == kNegInf) ? kNegInf : __logf(nflows[o]) - m;
-}
-
 constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per chunk
+constexpr int kPreChildren = 4;  // child blocks per product block the FUSE = 2 kernels prefetch
 
 // ---------------------------------------------------------------------------------------------
 // Forward.  Tile = (node block j, TM parent rows) x TN batch columns; logical tile order is
@@ -103,7 +56,10 @@ constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per ch
 // (pids[k0 + i] = pids[k0] + i bs), which is how the compiler lays out a child block of size
 // >= TK.  A stage then needs two indices instead of TK, read straight from memory; otherwise
 // the index rows are staged through LDS.
-template <typename T, int TM, int TN, bool RUN>
+//
+// RUN = 2 (grids smaller than the chip): the loads of the first two stages are issued together,
+// so a two-stage reduction waits for one memory round trip instead of two.
+template <typename T, int TM, int TN, int RUN>
 __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ nids,
                                                     const int* __restrict__ cids, const int* __restrict__ pids,
@@ -170,24 +126,36 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
     };
 
     if (RUN) {
-        auto load = [&](int k0) {
+        auto load_into = [&](int k0, float* evp, float* avp) {
             const int64_t c0 = crow[k0], p0 = prow[k0];
             const float* ep = emars + (c0 + bk0) * B + (bok ? bb : 0);
             const float* pp = params + p0 + (int64_t)ak0 * bs + m0 + arow;
             if (k0 + TK <= n && bok) {   // full stage: no per-element predicates
 #pragma unroll
-                for (int q = 0; q < PB; ++q) ev[q] = ep[(int64_t)q * B];
+                for (int q = 0; q < PB; ++q) evp[q] = ep[(int64_t)q * B];
 #pragma unroll
-                for (int q = 0; q < PA; ++q) av[q] = pp[(int64_t)q * bs];
+                for (int q = 0; q < PA; ++q) avp[q] = pp[(int64_t)q * bs];
             } else {
 #pragma unroll
-                for (int q = 0; q < PB; ++q) ev[q] = (k0 + bk0 + q < n && bok) ? ep[(int64_t)q * B] : kNegInf;
+                for (int q = 0; q < PB; ++q) evp[q] = (k0 + bk0 + q < n && bok) ? ep[(int64_t)q * B] : kNegInf;
 #pragma unroll
-                for (int q = 0; q < PA; ++q) av[q] = (k0 + ak0 + q < n) ? pp[(int64_t)q * bs] : 0.0f;
+                for (int q = 0; q < PA; ++q) avp[q] = (k0 + ak0 + q < n) ? pp[(int64_t)q * bs] : 0.0f;
             }
         };
-        load(0);
-        for (int k0 = 0; k0 < n; k0 += TK) process(k0, n, load);
+        float ev2[RUN == 2 ? PB : 1], av2[RUN == 2 ? PA : 1];
+        load_into(0, ev, av);
+        if (RUN == 2 && TK < n) load_into(TK, ev2, av2);
+        auto next = [&](int k1) {
+            if (RUN == 2 && k1 == TK) {
+#pragma unroll
+                for (int q = 0; q < PB; ++q) ev[q] = ev2[q];
+#pragma unroll
+                for (int q = 0; q < PA; ++q) av[q] = av2[q];
+            } else {
+                load_into(k1, ev, av);
+            }
+        };
+        for (int k0 = 0; k0 < n; k0 += TK) process(k0, n, next);
     } else {
         // Index rows are staged by their padded width K, so the first chunk's loads do not wait
         // for the child count n.
@@ -240,7 +208,12 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 // child node blocks have the product block's size.  ch_nodes[j][k] = 2 * (first node of child
 // block k of the product block j) + 1 if that child's flows are to be stored as
 // log(flow) - mar (and mod_children is set), 0 = padding.
-template <typename T, int TM, int TN, bool ONE_BLOCK, bool FUSE>
+//
+// FUSE = 2 (at most kPreChildren child blocks per product block; chosen for grids smaller than the
+// chip, where the kernel time is its chain of dependent memory round trips): the child codes and
+// the marginals of the marked children are fetched at kernel start, beside the element marginals,
+// instead of after the reduction.
+template <typename T, int TM, int TN, bool ONE_BLOCK, int FUSE>
 __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ chids,
@@ -282,8 +255,28 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     const int64_t b = b0 + mycol;
     const int64_t obase = (int64_t)chids[j] + m0 + wm * 32;
     float epre[16];
+    int cpre[FUSE == 2 ? kPreChildren : 1];
+    float mpre[FUSE == 2 ? kPreChildren : 1][16];
+    auto prefetch_epilogue = [&]() {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
+        for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
+        if (FUSE == 2) {
+            const int* crow = ch_nodes + (int64_t)j * Kc;
+            const int64_t bc = b < B ? b : B - 1;   // clamped column: unconditional loads
+#pragma unroll
+            for (int u = 0; u < kPreChildren; ++u) {
+                cpre[u] = u < Kc ? crow[u] : 0;
+                if ((cpre[u] & 1) && mod_children) {
+                    const int64_t cb = (int64_t)(cpre[u] >> 1) + m0 + wm * 32;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mpre[u][r] = nmars[(cb + acc_row(r, lane)) * B + bc];
+                }
+            }
+        }
+    };
+    // FUSE = 2 issues the reduction's loads first (all of them when it has two stages) and the
+    // epilogue's after them; the others fetch the epilogue's operands first.
+    if (FUSE != 2) prefetch_epilogue();
     // One stage of the reduction over the flattened parent index [k0, k0 + TK).
     auto process = [&](int k0, int kf, auto&& load) {
         float lm = kNegInf;
@@ -318,18 +311,33 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
         // bs >= TK (both powers of two): a stage lies inside one parent block, so its rows and
         // parameters are contiguous runs and need only that block's two indices.
         const int kf = np << lbs;
-        auto load = [&](int k0) {
+        auto load_into = [&](int k0, float* lvp, float* avp) {
             const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
             const int64_t rbase = ((int64_t)parrow[kb] + k_in + bk0) * B + bb;
 #pragma unroll
             for (int q = 0; q < PB; ++q)
-                lv[q] = bok ? lfdm_at(nflows, nmars, rbase + (int64_t)q * B, modified) : kNegInf;
+                lvp[q] = bok ? lfdm_at(nflows, nmars, rbase + (int64_t)q * B, modified) : kNegInf;
             const int64_t base = (int64_t)pprow[kb] + (int64_t)(m0 + ar0) * bs + k_in + ak;
 #pragma unroll
-            for (int q = 0; q < PA; ++q) av[q] = params[base + (int64_t)q * RA * bs];
+            for (int q = 0; q < PA; ++q) avp[q] = params[base + (int64_t)q * RA * bs];
         };
-        load(0);
-        for (int k0 = 0; k0 < kf; k0 += TK) process(k0, kf, load);
+        float lv2[FUSE == 2 ? PB : 1], av2[FUSE == 2 ? PA : 1];
+        load_into(0, lv, av);
+        if (FUSE == 2) {
+            if (TK < kf) load_into(TK, lv2, av2);
+            prefetch_epilogue();
+        }
+        auto next = [&](int k1) {
+            if (FUSE == 2 && k1 == TK) {
+#pragma unroll
+                for (int q = 0; q < PB; ++q) lv[q] = lv2[q];
+#pragma unroll
+                for (int q = 0; q < PA; ++q) av[q] = av2[q];
+            } else {
+                load_into(k1, lv, av);
+            }
+        };
+        for (int k0 = 0; k0 < kf; k0 += TK) process(k0, kf, next);
     } else {
         for (int pb = 0; pb < np; pb += KCAP) {
             const int cnp = (np - pb < KCAP) ? np - pb : KCAP;
@@ -366,7 +374,28 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
     }
     if (b < B) {
         const float m = mrun[mycol];
-        if (FUSE) {
+        if (FUSE == 2) {
+            float v[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float e = epre[r];
+                v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
+            }
+#pragma unroll
+            for (int u = 0; u < kPreChildren; ++u) {
+                if (cpre[u] == 0) continue;
+                const int64_t cbase = (int64_t)(cpre[u] >> 1) + m0 + wm * 32;
+                if ((cpre[u] & 1) && mod_children) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                            (mpre[u][r] == kNegInf) ? kNegInf : __logf(v[r]) - mpre[u][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+                }
+            }
+        } else if (FUSE) {
             float v[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -399,6 +428,307 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
                 float v = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
                 if (accumulate) v += eflows[o];
                 eflows[o] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 16-byte-load forms of the two hot reductions (BF16 / FP16 operands, 64 x 64 tiles, TK = 128).
+// Each thread moves a stage's operands with 16 float4 loads instead of 64 dword loads, so a
+// wavefront stays well inside its 63 outstanding vector loads and a stage costs one memory round
+// trip.  Requirements (checked by the launchers): B % 4 == 0; every parameter index of the
+// partition a multiple of 4 (16-byte-aligned parameter rows); forward: the child-run layout with
+// runs >= TK; element flows: parent blocks of >= TK rows (a stage inside one parent block).
+//
+// Element / flow rows (TK rows x 64 columns): thread = (row group g = tid / 16 of 8 rows,
+// column quad q = tid % 16); its 8 float4 give 4 columns x 8 consecutive rows, which are one
+// 16-byte LDS store per column of the [column][k] B tile.
+constexpr int kVecRows = 8;                 // rows per thread of a row tile
+constexpr int kVecGroups = 256 / 16;        // row groups (16 column quads per 64 columns)
+
+template <typename T>
+struct Pack8 {
+    static __device__ __forceinline__ void store(T* dst, const float* v) {
+        typedef T T8 __attribute__((ext_vector_type(8)));
+        T8 x;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = MF<T>::cvt(v[i]);
+        *reinterpret_cast<T8*>(dst) = x;
+    }
+};
+
+// Column max of a 128 x 64 stage held as v[8 rows][4 columns] per thread, folded into the running
+// max; returns this thread's 4 column maxima (new running max).
+__device__ __forceinline__ void vec_stage_max(const float4 (&v)[kVecRows], float (&red)[kVecGroups][64],
+                                              float* mrun, float* msc, int g, int q, float (&mn)[4]) {
+    float m[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+    for (int r = 0; r < kVecRows; ++r) {
+        m[0] = fmaxf(m[0], v[r].x);
+        m[1] = fmaxf(m[1], v[r].y);
+        m[2] = fmaxf(m[2], v[r].z);
+        m[3] = fmaxf(m[3], v[r].w);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[g][4 * q + c] = m[c];
+    __syncthreads();
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        float x = red[0][tid];
+#pragma unroll
+        for (int i = 1; i < kVecGroups; ++i) x = fmaxf(x, red[i][tid]);
+        const float mo = mrun[tid];
+        const float mnew = fmaxf(mo, x);
+        msc[tid] = (mnew == kNegInf) ? 1.0f : __expf(mo - mnew);
+        mrun[tid] = mnew;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) mn[c] = mrun[4 * q + c];
+}
+
+// B tile from a stage: Bs[4q + c][8g .. 8g + 7] = exp(v[r][c] - mn[c]).
+template <typename T, int LD>
+__device__ __forceinline__ void vec_stage_b(const float4 (&v)[kVecRows], const float (&mn)[4], T (*Bs)[LD], int g,
+                                            int q) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float e[8];
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const float x = c == 0 ? v[r].x : c == 1 ? v[r].y : c == 2 ? v[r].z : v[r].w;
+            e[r] = mn[c] == kNegInf ? 0.0f : __expf(x - mn[c]);
+        }
+        Pack8<T>::store(&Bs[4 * q + c][8 * g], e);
+    }
+}
+
+// Forward with 16-byte loads (RUN layout).  PRE: the first two stages' loads are issued together
+// (grids smaller than the chip).
+template <typename T, bool PRE>
+__global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars, const float* __restrict__ emars,
+                                                   const float* __restrict__ params, const int* __restrict__ nids,
+                                                   const int* __restrict__ cids, const int* __restrict__ pids,
+                                                   const int* __restrict__ nch, int K, int64_t B, int mtiles,
+                                                   int btiles) {
+    constexpr int TM = 64, TN = 64, LD = TK + MF<T>::PAD;
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[kVecGroups][64], mrun[TN], msc[TN];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = mtiles * TM;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % 2, wn = w / 2;
+    const int n = nch[j];
+    const int* crow = cids + (int64_t)j * K;
+    const int* prow = pids + (int64_t)j * K;
+    const int g = tid / 16, q = tid % 16;           // element rows: row group, column quad
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;                        // B % 4 == 0: a quad is all in or all out
+    const int64_t bqc = bok ? bq : B - 4;
+    const int iq = tid % 16, kg = tid / 16;         // parameters: row quad, child group of 8
+    const int mycol = wn * 32 + (lane & 31);
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+
+    float4 ev[kVecRows], av[kVecRows], ev2[PRE ? kVecRows : 1], av2[PRE ? kVecRows : 1];
+    auto load = [&](int k0, float4* e, float4* a) {
+        const int64_t c0 = crow[k0], p0 = prow[k0];
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int k = 8 * g + r;
+            const int kc = (k0 + k < n) ? k : 0;
+            e[r] = *reinterpret_cast<const float4*>(emars + (c0 + kc) * B + bqc);
+            if (!(k0 + k < n && bok)) e[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        }
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int k = 8 * kg + r;
+            const int kc = (k0 + k < n) ? k : 0;
+            a[r] = *reinterpret_cast<const float4*>(params + p0 + (int64_t)kc * bs + m0 + 4 * iq);
+            if (!(k0 + k < n)) a[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    if (n > 0) load(0, ev, av);
+    if (PRE && TK < n) load(TK, ev2, av2);
+    for (int k0 = 0; k0 < n; k0 += TK) {
+        float mn[4];
+        vec_stage_max(ev, red, mrun, msc, g, q, mn);
+        vec_stage_b<T, LD>(ev, mn, Bs, g, q);
+        // As[4 iq + c][8 kg .. 8 kg + 7]
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x[8];
+#pragma unroll
+            for (int r = 0; r < kVecRows; ++r)
+                x[r] = c == 0 ? av[r].x : c == 1 ? av[r].y : c == 2 ? av[r].z : av[r].w;
+            Pack8<T>::store(&As[4 * iq + c][8 * kg], x);
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < n) {
+            if (PRE && k0 == 0) {
+#pragma unroll
+                for (int r = 0; r < kVecRows; ++r) {
+                    ev[r] = ev2[r];
+                    av[r] = av2[r];
+                }
+            } else {
+                load(k0 + TK, ev, av);
+            }
+        }
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    const int64_t b = b0 + mycol;
+    if (b < B) {
+        const float m = mrun[mycol];
+        const int64_t base = (int64_t)nids[j] + m0 + wm * 32;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float v = acc[r];
+            node_mars[(base + acc_row(r, lane)) * B + b] = (v > 0.0f && m != kNegInf) ? __logf(v) + m : kNegInf;
+        }
+    }
+}
+
+// Element flows with 16-byte loads (parent blocks of >= TK rows).  FUSE as sum_ele_mfma (0 or 1);
+// PRE as sum_fwd_vec.
+template <typename T, int FUSE, bool PRE>
+__global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, const float* nflows,
+                                                   const float* __restrict__ nmars, const float* __restrict__ emars,
+                                                   const float* __restrict__ params, const int* __restrict__ chids,
+                                                   const int* __restrict__ parids, const int* __restrict__ parpids,
+                                                   const int* __restrict__ npar, int KP, int lbs, int64_t B,
+                                                   int mtiles, int btiles, bool modified, bool accumulate,
+                                                   float* child_flows, const int* __restrict__ ch_nodes, int Kc,
+                                                   bool mod_children) {
+    constexpr int TM = 64, TN = 64, LD = TK + MF<T>::PAD;
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[kVecGroups][64], mrun[TN], msc[TN];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = 1 << lbs;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % 2, wn = w / 2;
+    const int np = npar[j];
+    const int* parrow = parids + (int64_t)j * KP;
+    const int* pprow = parpids + (int64_t)j * KP;
+    const int g = tid / 16, q = tid % 16;            // parent rows: row group, column quad
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;
+    const int64_t bqc = bok ? bq : B - 4;
+    const int kq = tid % 32, ig = tid / 32;          // parameters: parent quad, child row group
+    const int mycol = wn * 32 + (lane & 31);
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+
+    const int kf = np << lbs;
+    float4 lv[kVecRows], av[kVecRows], lv2[PRE ? kVecRows : 1], av2[PRE ? kVecRows : 1];
+    auto load = [&](int k0, float4* l, float4* a) {
+        const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
+        const int64_t rbase = (int64_t)parrow[kb] + k_in + 8 * g;
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int64_t o = (rbase + r) * B + bqc;
+            float4 f = *reinterpret_cast<const float4*>(nflows + o);
+            if (!modified) {
+                const float4 m = *reinterpret_cast<const float4*>(nmars + o);
+                f.x = m.x == kNegInf ? kNegInf : __logf(f.x) - m.x;
+                f.y = m.y == kNegInf ? kNegInf : __logf(f.y) - m.y;
+                f.z = m.z == kNegInf ? kNegInf : __logf(f.z) - m.z;
+                f.w = m.w == kNegInf ? kNegInf : __logf(f.w) - m.w;
+            }
+            l[r] = bok ? f : make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        }
+        const float* pp = params + (int64_t)pprow[kb] + (int64_t)(m0 + ig) * bs + k_in + 4 * kq;
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) a[r] = *reinterpret_cast<const float4*>(pp + (int64_t)(8 * r) * bs);
+    };
+    if (kf > 0) load(0, lv, av);
+    if (PRE && TK < kf) load(TK, lv2, av2);
+    // element marginals for the epilogue
+    const int64_t b = b0 + mycol;
+    const int64_t obase = (int64_t)chids[j] + m0 + wm * 32;
+    float epre[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
+    for (int k0 = 0; k0 < kf; k0 += TK) {
+        float mn[4];
+        vec_stage_max(lv, red, mrun, msc, g, q, mn);
+        vec_stage_b<T, LD>(lv, mn, Bs, g, q);
+        // As[ig + 8 r][4 kq .. 4 kq + 3]
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            typedef T T4 __attribute__((ext_vector_type(4)));
+            T4 x;
+            x[0] = MF<T>::cvt(av[r].x);
+            x[1] = MF<T>::cvt(av[r].y);
+            x[2] = MF<T>::cvt(av[r].z);
+            x[3] = MF<T>::cvt(av[r].w);
+            *reinterpret_cast<T4*>(&As[ig + 8 * r][4 * kq]) = x;
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < kf) {
+            if (PRE && k0 == 0) {
+#pragma unroll
+                for (int r = 0; r < kVecRows; ++r) {
+                    lv[r] = lv2[r];
+                    av[r] = av2[r];
+                }
+            } else {
+                load(k0 + TK, lv, av);
+            }
+        }
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    if (b < B) {
+        const float m = mrun[mycol];
+        float v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float e = epre[r];
+            v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
+        }
+        if (FUSE) {
+            const int* crow = ch_nodes + (int64_t)j * Kc;
+            for (int k = 0; k < Kc; ++k) {
+                const int code = crow[k];
+                if (code == 0) continue;
+                const int64_t cbase = (int64_t)(code >> 1) + m0 + wm * 32;
+                if ((code & 1) && mod_children) {
+                    float mv[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mv[r] = nmars[(cbase + acc_row(r, lane)) * B + b];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                            (mv[r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t o = (obase + acc_row(r, lane)) * B + b;
+                eflows[o] = accumulate ? eflows[o] + v[r] : v[r];
             }
         }
     }
@@ -571,12 +901,15 @@ template <typename T, int TM, int TN>
 void launch_fwd_tile(bool run, unsigned g, float* node_mars, const float* emars, const float* params, const int* nids,
                      const int* cids, const int* pids, const int* nch, int K, int64_t B, int mtiles, int btiles,
                      hipStream_t stream) {
-    if (run)
-        sum_fwd_mfma<T, TM, TN, true><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
-                                                             mtiles, btiles);
+    if (run && g < 256)
+        sum_fwd_mfma<T, TM, TN, 2><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                          mtiles, btiles);
+    else if (run)
+        sum_fwd_mfma<T, TM, TN, 1><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                          mtiles, btiles);
     else
-        sum_fwd_mfma<T, TM, TN, false><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
-                                                              mtiles, btiles);
+        sum_fwd_mfma<T, TM, TN, 0><<<g, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                          mtiles, btiles);
 }
 
 template <typename T>
@@ -588,8 +921,23 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles));
+    const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
+    child_run &= ~PJAMD_PARAMS_ALIGNED;
     const bool run = child_run >= TK && child_run % TK == 0;
-    if (bs >= 64)
+    bool vec = false;
+    if constexpr (sizeof(T) == 2) {
+        if (run && aligned && bs >= 64 && B % 4 == 0) {
+            vec = true;
+            if (tiles < 256)
+                sum_fwd_vec<T, true><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids,
+                                                                         nch, K, B, mtiles, (int)btiles);
+            else
+                sum_fwd_vec<T, false><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids,
+                                                                          nch, K, B, mtiles, (int)btiles);
+        }
+    }
+    if (vec) {
+    } else if (bs >= 64)
         launch_fwd_tile<T, 64, 64>(run, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles,
                                    (int)btiles, stream);
     else
@@ -599,7 +947,7 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     return PJAMD_OK;
 }
 
-template <typename T, int TM, int TN, bool FUSE>
+template <typename T, int TM, int TN, int FUSE>
 void launch_ele_tile(bool one_block, unsigned g, float* eflows, const float* nflows, const float* nmars,
                      const float* emars, const float* params, const int* chids, const int* parids,
                      const int* parpids, const int* npar, int KP, int lbs, int64_t B, int mtiles, int bt, bool mod,
@@ -619,7 +967,7 @@ template <typename T>
 int launch_ele(float* eflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* chids, const int* parids, const int* parpids, const int* npar, int n, int KP, int cbs, int bs,
                int64_t B, bool mod, bool accum, float* child_flows, const int* ch_nodes, int Kc, bool mod_children,
-               hipStream_t stream) {
+               bool aligned, hipStream_t stream) {
     const int lbs = log2_exact(bs);
     const int TM = cbs >= 64 ? 64 : 32;
     const int TN = cbs >= 64 ? 64 : 128;
@@ -630,26 +978,42 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
     const unsigned g = (unsigned)tiles;
     const int bt = (int)btiles;
     const bool ob = bs >= TK;
-    const bool fuse = ch_nodes != nullptr;
-    if (cbs >= 64) {
-        if (fuse)
-            launch_ele_tile<T, 64, 64, true>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
-                                             KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
-                                             mod_children, stream);
-        else
-            launch_ele_tile<T, 64, 64, false>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
-                                              KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
-                                              mod_children, stream);
-    } else {
-        if (fuse)
-            launch_ele_tile<T, 32, 128, true>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar,
-                                              KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
-                                              mod_children, stream);
-        else
-            launch_ele_tile<T, 32, 128, false>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids,
-                                               npar, KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,
-                                               mod_children, stream);
+    if constexpr (sizeof(T) == 2) {
+        if (aligned && ob && cbs >= 64 && B % 4 == 0) {
+#define PJAMD_ELE_VEC(F_, P_)                                                                                     \
+    sum_ele_vec<T, F_, P_><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, \
+                                                  KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,  \
+                                                  mod_children)
+            const bool pre = g < 256;
+            if (ch_nodes != nullptr) {
+                if (pre) PJAMD_ELE_VEC(1, true);
+                else PJAMD_ELE_VEC(1, false);
+            } else {
+                if (pre) PJAMD_ELE_VEC(0, true);
+                else PJAMD_ELE_VEC(0, false);
+            }
+#undef PJAMD_ELE_VEC
+            PJAMD_LAUNCH_CHECK();
+            return PJAMD_OK;
+        }
     }
+    // 0: element flows; 1: children scatter; 2: children scatter with prefetched children (grids
+    // smaller than the chip's 256 CUs, where every dependent round trip is paid in full)
+    const int fuse = ch_nodes == nullptr ? 0 : ((Kc <= kPreChildren && g < 256 && ob) ? 2 : 1);
+#define PJAMD_ELE_TILE(TM_, TN_, F_)                                                                              \
+    launch_ele_tile<T, TM_, TN_, F_>(ob, g, eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, \
+                                     KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc, mod_children, \
+                                     stream)
+    if (cbs >= 64) {
+        if (fuse == 2) PJAMD_ELE_TILE(64, 64, 2);
+        else if (fuse == 1) PJAMD_ELE_TILE(64, 64, 1);
+        else PJAMD_ELE_TILE(64, 64, 0);
+    } else {
+        if (fuse == 2) PJAMD_ELE_TILE(32, 128, 2);
+        else if (fuse == 1) PJAMD_ELE_TILE(32, 128, 1);
+        else PJAMD_ELE_TILE(32, 128, 0);
+    }
+#undef PJAMD_ELE_TILE
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -722,16 +1086,16 @@ int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const fl
 int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,
                                 const float* params, const int* chids, const int* parids, const int* parpids,
                                 const int* npar, int n, int KP, int cbs, int bs, int64_t B, bool mod, bool accum,
-                                int precision, hipStream_t stream) {
+                                bool aligned, int precision, hipStream_t stream) {
     if (cbs < 32) return PJAMD_EUNSUPPORTED;
     if (precision == 1)
         return launch_ele<__bf16>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs, B,
-                                  mod, accum, nullptr, nullptr, 0, false, stream);
+                                  mod, accum, nullptr, nullptr, 0, false, aligned, stream);
     if (precision == 2)
         return launch_ele<_Float16>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs,
-                                    B, mod, accum, nullptr, nullptr, 0, false, stream);
+                                    B, mod, accum, nullptr, nullptr, 0, false, aligned, stream);
     return launch_ele<float>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, n, KP, cbs, bs, B,
-                             mod, accum, nullptr, nullptr, 0, false, stream);
+                             mod, accum, nullptr, nullptr, 0, false, aligned, stream);
 }
 
 PJAMD_EXPORT int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars,
@@ -743,16 +1107,16 @@ PJAMD_EXPORT int pjamd_sum_backward_ele_to_children(float* node_flows, const flo
                   ch_nodes != nullptr);
     if (n_cblocks == 0) return PJAMD_OK;
     if (cbs < 32) return PJAMD_EUNSUPPORTED;
-    const bool mod = flags & 1, modc = (flags & 2) != 0;
+    const bool mod = flags & 1, modc = (flags & 2) != 0, aligned = (flags & 4) != 0;
     if (precision == 1)
         return launch_ele<__bf16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
-                                  n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, stream);
+                                  n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream);
     if (precision == 2)
         return launch_ele<_Float16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids,
                                     n_pars, n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc,
-                                    stream);
+                                    aligned, stream);
     return launch_ele<float>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
-                             n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, stream);
+                             n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream);
 }
 
 // FP16 operands can overflow in the parameter-flow product (exp(e + s) is unbounded above), so

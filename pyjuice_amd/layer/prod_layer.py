@@ -84,16 +84,20 @@ class ProdLayer(Layer, nn.Module):
         self.device = device
         self._dev = None
         if device.type == "cuda":
-            i32 = lambda t: t.to(torch.int32).contiguous()
-            # algorithmic FP32 words per sample: rows gathered + rows written (+ read for accumulation)
-            bs = self.block_size
-            self._dev = {
-                "fw": [(i32(n), i32(c), int((c != 0).sum()) * bs + n.numel() * bs)
-                       for n, c in zip(self.partitioned_nids, self.partitioned_cids) if n.numel() > 0],
-                "bk": [(i32(n), i32(c), int((c != 0).sum()) * bs + 2 * n.numel() * bs)
-                       for n, c in zip(self.partitioned_u_cids, self.partitioned_parids) if n.numel() > 0],
-            }
+            self.build_device_tables()
         return self
+
+    def build_device_tables(self):
+        """int32 launch tables (``_dev``) on the layer's device, with the algorithmic FP32 words per
+        sample of each partition: rows gathered + rows written (+ read for accumulation)."""
+        i32 = lambda t: t.to(torch.int32).contiguous()
+        bs = self.block_size
+        self._dev = {
+            "fw": [(i32(n), i32(c), int((c != 0).sum()) * bs + n.numel() * bs)
+                   for n, c in zip(self.partitioned_nids, self.partitioned_cids) if n.numel() > 0],
+            "bk": [(i32(n), i32(c), int((c != 0).sum()) * bs + 2 * n.numel() * bs)
+                   for n, c in zip(self.partitioned_u_cids, self.partitioned_parids) if n.numel() > 0],
+        }
 
     def forward(self, node_mars: torch.Tensor, element_mars: torch.Tensor, _for_backward: bool = False, **kwargs):
         """``element_mars[nid + j] = sum_k node_mars[cid_k + j]`` (``prod_layer.py:160-204``)."""

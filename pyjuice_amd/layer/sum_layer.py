@@ -27,6 +27,16 @@ def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
     return torch.unique(torch.from_numpy(next_power_of_2(x)))
 
 
+PARAMS_ALIGNED = 1 << 30   # PJAMD_PARAMS_ALIGNED
+
+
+def _params_aligned(idx: torch.Tensor, counts: torch.Tensor, bs: int) -> bool:
+    """Every real entry of an index table (the first counts[j] of row j) is a multiple of 4, and
+    so is the block size: parameter rows start 16-byte aligned."""
+    real = torch.arange(idx.size(1), device=idx.device)[None, :] < counts.to(idx.device)[:, None]
+    return bs % 4 == 0 and bool(((idx[real] % 4) == 0).all())
+
+
 def _i32(t: torch.Tensor) -> torch.Tensor:
     if t.numel() > 0:
         assert int(t.max()) < 2 ** 31, "index exceeds the int32 range of the HIP kernels"
@@ -137,27 +147,36 @@ class SumLayer(Layer, nn.Module):
         self.device = device
         self._dev = None
         if device.type == "cuda":
-            fw = []
-            self._fw_child_run = []
-            for p in range(self.num_fw_partitions):
-                n = self.partitioned_nids[p]
-                if n.numel() == 0:
-                    continue
-                # Real children of block j occupy columns [0, n_chs_j); the rest is padding.
-                fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
-                           _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device)),
-                           int(self._fw_n_chs[p].sum())))
-                self._fw_child_run.append(self._child_run(p))
-            bk = []
-            for q in range(self.num_bk_partitions):
-                c = self.partitioned_chids[q]
-                if c.numel() == 0:
-                    continue
-                bk.append((_i32(c), _i32(self.partitioned_parids[q]), _i32(self.partitioned_parpids[q]),
-                           _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q]),
-                           int(self._bk_n_pars[q].sum())))
-            self._dev = {"fw": fw, "bk": bk}
+            self.build_device_tables()
         return self
+
+    def build_device_tables(self):
+        """int32 launch tables (``_dev``) of the forward and backward partitions on the layer's
+        device, with each forward partition's child-run promise."""
+        device = self.device
+        fw = []
+        self._fw_child_run = []
+        self._fw_aligned, self._bk_aligned = [], []
+        for p in range(self.num_fw_partitions):
+            n = self.partitioned_nids[p]
+            if n.numel() == 0:
+                continue
+            # Real children of block j occupy columns [0, n_chs_j); the rest is padding.
+            fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
+                       _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device)),
+                       int(self._fw_n_chs[p].sum())))
+            self._fw_child_run.append(self._child_run(p))
+            self._fw_aligned.append(_params_aligned(self.partitioned_pids[p], self._fw_n_chs[p], self.block_size))
+        bk = []
+        for q in range(self.num_bk_partitions):
+            c = self.partitioned_chids[q]
+            if c.numel() == 0:
+                continue
+            bk.append((_i32(c), _i32(self.partitioned_parids[q]), _i32(self.partitioned_parpids[q]),
+                       _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q]),
+                       int(self._bk_n_pars[q].sum())))
+            self._bk_aligned.append(_params_aligned(self.partitioned_parpids[q], self._bk_n_pars[q], self.block_size))
+        self._dev = {"fw": fw, "bk": bk}
 
     # ------------------------------------------------------------------ passes
 
@@ -172,7 +191,9 @@ class SumLayer(Layer, nn.Module):
             prec = 0
         B = node_mars.size(1)
         bs = self.block_size
-        for (nids, cids, pids, pfids, nch, tot), run in zip(self._dev["fw"], self._fw_child_run):
+        for (nids, cids, pids, pfids, nch, tot), run, al in zip(self._dev["fw"], self._fw_child_run,
+                                                                self._fw_aligned):
+            run = run | (PARAMS_ALIGNED if al else 0)
             # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
             # per sample, parameters read once
             algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
@@ -211,6 +232,7 @@ class SumLayer(Layer, nn.Module):
                        bs, B, s, algo=(12 * nids.numel() * bs * B, 0))
         mod = 1 if allow_modify_flows else 0
         for q, (chids, parids, parpids, npar, cbs, totp) in enumerate(self._dev["bk"]):
+            al = 4 if self._bk_aligned[q] else 0
             if child_tables is not None:
                 tab, n_blk, n_mod = child_tables[q]
                 # parent rows (and marginals unless modified) + element rows read, one write per
@@ -221,13 +243,13 @@ class SumLayer(Layer, nn.Module):
                 N.call("pjamd_sum_backward_ele_to_children", N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids),
                        N.ptr(npar), N.ptr(tab), tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
-                       mod | (2 if allow_modify_flows else 0), prec, s, algo=algo)
+                       mod | (2 if allow_modify_flows else 0) | al, prec, s, algo=algo)
                 continue
             algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
                     2 * totp * bs * cbs * B)
             N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
                    N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar),
-                   chids.numel(), parids.size(1), cbs, bs, B, mod | (2 if accumulate_ch_flows else 0),
+                   chids.numel(), parids.size(1), cbs, bs, B, mod | (2 if accumulate_ch_flows else 0) | al,
                    prec, s, algo=algo)
         if param_flows is not None:
             pf64 = 1 if param_flows.dtype == torch.float64 else 0

@@ -25,6 +25,7 @@ from pyjuice_amd.layer import InputLayer, LayerGroup, ProdLayer, SumLayer
 from pyjuice_amd.nodes import CircuitNodes, multiply, summate
 from pyjuice_amd.nodes.nodes import dfs_postorder
 from pyjuice_amd.utils import BitSet
+from . import chain as _chain
 from .backend import compile_cum_par_flows_fn, compile_par_update_fn, normalize_parameters
 
 __all__ = ["TensorCircuit", "compile"]
@@ -95,6 +96,13 @@ class TensorCircuit(nn.Module):
         self.modify_flows_in_place = True
         self._par_batches = {"tables": dict(), "layers": set()}
         self._fused = {"tables": dict(), "prod_groups": set(), "premod": set()}
+        # Runs of narrow layers in one launch each (pyjuice_amd.model.chain): on/off, and the
+        # largest sum-parameter footprint of a (product, sum) group pair that may join a run.
+        # Off by default: on HCLT-784 (256 latents) a run's single workgroup per 32 columns is
+        # bound by its loads in flight at HBM latency and loses to one tiled launch per layer.
+        self.use_chains = False
+        self.chain_max_param_bytes = 1 << 20
+        self._chain_cache = dict()
 
     # ------------------------------------------------------------------ parameter-flow buffer
 
@@ -140,11 +148,35 @@ class TensorCircuit(nn.Module):
                 "fuse": [(int(t), int(n), torch.tensor(ch, dtype=torch.int64, device=device))
                          for t, n, ch in self.parflow_fusing_kwargs],
             }
-            self._mask_unreached_products()
-            self._par_batches = self._build_par_batches(device)
-            self._fused = self._build_fused_backward(device)
-            self._pf_overwrite_ok = self._sum_flows_overwritable()
+            self._build_launch_plans(device)
+        self._chain_cache = dict()
         return self
+
+    def _build_launch_plans(self, device):
+        """Launch tables of the whole circuit on ``device`` (built on the HIP device by ``to``; on the
+        CPU only to inspect them): per-layer int32 tables, masked product backward tables, batched
+        parameter-flow descriptors, fused product backward tables."""
+        for g in self.inner_layer_groups:
+            for layer in g:
+                if layer._dev is None:
+                    layer.build_device_tables()
+        self._mask_unreached_products()
+        self._par_batches = self._build_par_batches(device)
+        self._fused = self._build_fused_backward(device)
+        self._pf_overwrite_ok = self._sum_flows_overwritable()
+        self._chain_cache = dict()
+
+    def _chains(self, direction: str, precision: str, modify: bool = True, compute_param_flows: bool = True):
+        """Chain runs (``pyjuice_amd.model.chain.plan``) of one pass, cached per configuration."""
+        if not self.use_chains or precision not in ("bf16", "fp16", "fp32"):
+            return dict()
+        key = (direction, "fp32" if precision == "fp32" else "16", modify, compute_param_flows,
+               self.chain_max_param_bytes)
+        runs = self._chain_cache.get(key)
+        if runs is None:
+            runs = _chain.plan(self, direction, precision, modify, compute_param_flows, self.chain_max_param_bytes)
+            self._chain_cache[key] = runs
+        return runs
 
     def _mask_unreached_products(self):
         """A product block that no sum edge of the layer above points to (sparse block
@@ -411,12 +443,25 @@ class TensorCircuit(nn.Module):
         return self.element_mars.narrow(0, off, self._group_elems[group_id // 2])
 
     def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
-        for gid, g in enumerate(self.inner_layer_groups):
+        runs = self._chains("fwd", precision)
+        groups = self.inner_layer_groups
+        gid = 0
+        while gid < len(groups):
+            run = runs.get(gid)
+            if run is not None:
+                B = self.node_mars.size(1)
+                N.call("pjamd_chain_forward", N.ptr(self.node_mars), N.ptr(self.element_mars), N.ptr(self.params),
+                       N.ptr(run.ops), run.n_ops, N.ptr(run.idx), B, PRECISIONS[precision], N.stream(),
+                       algo=run.algo(B))
+                gid = run.hi
+                continue
+            g = groups[gid]
             if g.is_prod():
                 g(self.node_mars, self._slab(gid))
             else:
                 g(self.node_mars, self._slab(gid), self.params, force_use_bf16=force_use_bf16,
                   force_use_fp32=force_use_fp32, precision=precision)
+            gid += 1
 
     def forward(self, inputs: torch.Tensor, input_layer_fn: Optional[Union[str, Callable]] = None,
                 cache: Optional[dict] = None, return_cache: bool = False, record_cudagraph: bool = False,
@@ -494,15 +539,26 @@ class TensorCircuit(nn.Module):
         modify = allow_modify_flows and self.modify_flows_in_place
         batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu") else set()
         fz = self._fused if precision != "fp32_valu" else {"tables": dict(), "prod_groups": set(), "premod": set()}
-        for lid in range(len(self.inner_layer_groups) - 1, -1, -1):
-            g = self.inner_layer_groups[lid]
+        runs = self._chains("bwd", precision, modify, compute_param_flows)
+        lid = len(self.inner_layer_groups) - 1
+        while lid >= 0:
+            run = runs.get(lid)
+            if run is not None:
+                B = self.node_flows.size(1)
+                N.call("pjamd_chain_backward", N.ptr(self.node_flows), N.ptr(self.node_mars),
+                       N.ptr(self.element_mars), N.ptr(self.params), N.ptr(run.ops), run.n_ops, N.ptr(run.idx), B,
+                       PRECISIONS[precision], N.stream(), algo=run.algo(B))
+                lid = run.lo - 1
+                continue
+            gid, lid = lid, lid - 1
+            g = self.inner_layer_groups[gid]
             if g.is_prod():
-                if lid not in fz["prod_groups"]:
+                if gid not in fz["prod_groups"]:
                     g.backward(self.node_flows, self.element_flows)
                 continue
             for layer in g:
                 pf = self._pf if compute_param_flows and id(layer) not in batched else None
-                layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(lid), self.params,
+                layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(gid), self.params,
                                param_flows=pf, allow_modify_flows=modify, precision=precision,
                                premodified=modify and id(layer) in fz["premod"],
                                child_tables=fz["tables"].get(id(layer)), overwrite_pflows=overwrite)

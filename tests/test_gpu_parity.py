@@ -215,3 +215,37 @@ def test_backward_is_repeatable(name, precision, device):
     assert U.rel_err(outs[0][0], outs[1][0]) <= 1e-6
     for a, b in zip(outs[0][1], outs[1][1]):
         assert U.rel_err(a, b) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs32", "hclt_cat_bs64", "hclt_cat_bs128", "hmm_small", "pd_small"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "fp32"])
+@pytest.mark.parametrize("B", [64, 45])
+def test_chain_runs_match_layerwise(name, precision, B, device):
+    """Runs of narrow layers in one launch (pjamd_chain_forward / _backward) against one launch per
+    layer on the same circuit and data: log-likelihoods, node flows of the modify-free backward,
+    sum and input parameter flows.  Both apply the same operand rounding; only the order of the
+    FP32 accumulations differs."""
+    U.say(f"[chain vs layerwise] {name} {precision} B={B}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = torch.from_numpy(U.random_data(cards, B, seed=8)).to(device)
+    pc.use_chains = True
+    if name != "pd_small":
+        assert pc._chains("fwd", precision) and pc._chains("bwd", precision), "no run planned"
+    out = dict()
+    for chains in (False, True):
+        pc.use_chains = chains
+        lls = pc(X).detach().double().cpu().numpy()
+        pc.backward(X, flows_memory=0.0, allow_modify_flows=False)
+        nf = pc.node_flows.detach().double().cpu().numpy()
+        pc.backward(X, flows_memory=0.0)
+        out[chains] = (lls, nf, pc.param_flows.detach().double().cpu().numpy(),
+                       [l.param_flows.detach().double().cpu().numpy() for l in pc.input_layer_group])
+    tol = 1e-5 if precision == "fp32" else 2e-4
+    (l0, f0, p0, i0), (l1, f1, p1, i1) = out[False], out[True]
+    assert np.abs(l0 - l1).max() <= tol * len(cards)
+    assert U.rel_err(f1, f0) <= 10 * tol
+    assert U.rel_err(p1, p0) <= 10 * tol
+    for a, b in zip(i1, i0):
+        assert U.rel_err(a, b) <= 10 * tol
