@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
 // Parameter flows of one tile = (node block j, TM parent rows from m0) x (TN children from kt0),
 // reduced over the batch range [bstart, bend).  Both operand tiles are loaded with lanes along
 // the batch.  Shared by the per-layer and the batched kernels below.
-template <typename T, int TM, int TN, typename PF>
+template <typename T, int TM, int TN, typename PF, bool VEC>
 __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                          const float* __restrict__ nmars, const float* __restrict__ emars,
                                          const float* __restrict__ params, const int* __restrict__ nids,
@@ -753,7 +753,8 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
     constexpr int PX = TM / RG, PY = TN / RG;
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
-    __shared__ float xred[RG][TK];
+    constexpr int RGV = 256 / (TK / 4);  // VEC: rows per pass (lanes run along the batch, 4 values each)
+    __shared__ float xred[VEC ? RGV : RG][TK];
     __shared__ float sh[TK];
     __shared__ int64_t sy[TN];          // element row offsets of the tile's children (-1: padding)
 
@@ -764,9 +765,89 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
     const int64_t nrow0 = (int64_t)nids[j] + m0;
     const int* crow = cids + (int64_t)j * K;
 
-    const int xb = tid % TK, r0 = tid / TK;
     if (tid < TN) sy[tid] = (kt0 + tid < n) ? (int64_t)crow[kt0 + tid] * B : -1;
     __syncthreads();
+    floatx16 acc = {0};
+
+    if constexpr (VEC) {
+        // 16-byte loads (B % 4 == 0, batch range bounds multiples of 4): thread = (row group
+        // rg = tid / 32, batch quad xq = tid % 32); 16 float4 per stage instead of 64 dwords.
+        constexpr int PXV = TM / RGV, PYV = TN / RGV;
+        const int xq = tid % 32, rg = tid / 32;
+        float4 xv[PXV], yv[PYV];
+        const float4 ninf = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        auto load = [&](int64_t b0) {
+            const int64_t b = b0 + 4 * xq;
+            const bool ok = b < bend;
+            const int64_t bc = ok ? b : bstart;
+#pragma unroll
+            for (int q = 0; q < PXV; ++q) {
+                const int64_t o = (nrow0 + rg + (int64_t)q * RGV) * B + bc;
+                float4 f = *reinterpret_cast<const float4*>(nflows + o);
+                if (!modified) {
+                    const float4 m = *reinterpret_cast<const float4*>(nmars + o);
+                    f.x = m.x == kNegInf ? kNegInf : __logf(f.x) - m.x;
+                    f.y = m.y == kNegInf ? kNegInf : __logf(f.y) - m.y;
+                    f.z = m.z == kNegInf ? kNegInf : __logf(f.z) - m.z;
+                    f.w = m.w == kNegInf ? kNegInf : __logf(f.w) - m.w;
+                }
+                xv[q] = ok ? f : ninf;
+            }
+#pragma unroll
+            for (int q = 0; q < PYV; ++q) {
+                const int64_t yb = sy[rg + q * RGV];
+                const float4 y = *reinterpret_cast<const float4*>(emars + (yb >= 0 ? yb : 0) + bc);
+                yv[q] = (ok && yb >= 0) ? y : ninf;
+            }
+        };
+        load(bstart);
+        for (int64_t b0 = bstart; b0 < bend; b0 += TK) {
+            float m4[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+            for (int q = 0; q < PXV; ++q) {
+                m4[0] = fmaxf(m4[0], xv[q].x);
+                m4[1] = fmaxf(m4[1], xv[q].y);
+                m4[2] = fmaxf(m4[2], xv[q].z);
+                m4[3] = fmaxf(m4[3], xv[q].w);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) xred[rg][4 * xq + c] = m4[c];
+            __syncthreads();
+            if (tid < TK) {
+                float v = xred[0][tid];
+#pragma unroll
+                for (int g = 1; g < RGV; ++g) v = fmaxf(v, xred[g][tid]);
+                sh[tid] = v;
+            }
+            __syncthreads();
+            float s4[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s4[c] = sh[4 * xq + c];
+            typedef T T4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int q = 0; q < PXV; ++q) {
+                const float x[4] = {xv[q].x, xv[q].y, xv[q].z, xv[q].w};
+                T4 o;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    o[c] = MF<T>::cvt((s4[c] == kNegInf || x[c] == kNegInf) ? 0.0f : __expf(x[c] - s4[c]));
+                *reinterpret_cast<T4*>(&As[rg + q * RGV][4 * xq]) = o;
+            }
+#pragma unroll
+            for (int q = 0; q < PYV; ++q) {
+                const float y[4] = {yv[q].x, yv[q].y, yv[q].z, yv[q].w};
+                T4 o;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    o[c] = MF<T>::cvt((s4[c] == kNegInf || y[c] == kNegInf) ? 0.0f : __expf(y[c] + s4[c]));
+                *reinterpret_cast<T4*>(&Bs[rg + q * RGV][4 * xq]) = o;
+            }
+            __syncthreads();
+            if (b0 + TK < bend) load(b0 + TK);
+            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        }
+    } else {
+    const int xb = tid % TK, r0 = tid / TK;
     const int64_t xbase = (nrow0 + r0) * B;
 
     float xv[PX], yv[PY];
@@ -783,7 +864,6 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
         }
     };
 
-    floatx16 acc = {0};
     load(bstart);
     for (int64_t b0 = bstart; b0 < bend; b0 += TK) {
         float lm = kNegInf;
@@ -809,6 +889,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
         if (b0 + TK < bend) load(b0 + TK);
         acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
     }
+    }
     const int k = kt0 + wn * 32 + (lane & 31);
     if (k < n) {
         const int64_t p0 = (int64_t)pids[(int64_t)j * K + k] + m0 + wm * 32;
@@ -824,7 +905,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
 
 // Per-layer launch: logical tile order is m-tile fastest, then child tile, then batch split,
 // then node block.
-template <typename T, int TM, int TN, typename PF>
+template <typename T, int TM, int TN, typename PF, bool VEC>
 __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
                                                     const float* __restrict__ params, const int* __restrict__ nids,
@@ -839,14 +920,14 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
     const int j = t / (mtiles * ktiles * splits);
     const int64_t bstart = (int64_t)z * bchunk;
     const int64_t bend = (bstart + bchunk < B) ? bstart + bchunk : B;
-    par_tile<T, TM, TN, PF>(pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, K, B, j, m0, kt0,
-                            bstart, bend, modified, atomic_out, sign);
+    par_tile<T, TM, TN, PF, VEC>(pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, K, B, j, m0,
+                                 kt0, bstart, bend, modified, atomic_out, sign);
 }
 
 // Batched launch over the partitions of many sum layers (one descriptor row per partition, see
 // pjamd_sum_backward_par_batched).  The tile order inside a partition is m-tile fastest, then
 // child tile, then node block; each tile reduces over the whole batch.
-template <typename T, int TM, int TN, typename PF>
+template <typename T, int TM, int TN, typename PF, bool VEC>
 __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                                        const float* __restrict__ nmars,
                                                        const float* __restrict__ emars,
@@ -865,8 +946,8 @@ __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, 
     const int kt0 = ((local / mtiles) % ktiles) * TN;
     const int j = local / (mtiles * ktiles);
     const int mode = D[7] ? kPfAtomic : (overwrite ? kPfOverwrite : kPfAdd);
-    par_tile<T, TM, TN, PF>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io, pids + io,
-                            pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign);
+    par_tile<T, TM, TN, PF, VEC>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io,
+                                 pids + io, pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign);
 }
 
 // One 32 x 32 x KI product with the operand / accumulator lane layouts the kernels above assume:
@@ -1038,14 +1119,20 @@ int launch_par(PF* pflows, const float* nflows, const float* nmars, const float*
     const int atomic_out = (pf_mode == kPfAtomic || splits > 1) ? kPfAtomic : pf_mode;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles * splits));
     const unsigned grid = (unsigned)(tiles * splits);
-    if (bs >= 64)
-        sum_par_mfma<T, 64, 64, PF><<<grid, 256, 0, stream>>>(pflows, nflows, nmars, emars, params, nids, cids, pids,
-                                                             pfids, nch, K, B, bchunk, mtiles, ktiles, (int)splits, mod,
-                                                             atomic_out, sign);
-    else
-        sum_par_mfma<T, 32, 128, PF><<<grid, 256, 0, stream>>>(pflows, nflows, nmars, emars, params, nids, cids, pids,
-                                                              pfids, nch, K, B, bchunk, mtiles, ktiles, (int)splits,
-                                                              mod, atomic_out, sign);
+#define PJAMD_PAR(TM_, TN_, V_)                                                                                   \
+    sum_par_mfma<T, TM_, TN_, PF, V_><<<grid, 256, 0, stream>>>(pflows, nflows, nmars, emars, params, nids, cids, pids, \
+                                                               pfids, nch, K, B, bchunk, mtiles, ktiles, (int)splits, \
+                                                               mod, atomic_out, sign)
+    // 16-byte loads along the batch need B % 4 == 0 (split bounds are multiples of TK)
+    const bool vec = sizeof(T) == 2 && B % 4 == 0;
+    if (bs >= 64) {
+        if (vec) PJAMD_PAR(64, 64, sizeof(T) == 2);
+        else PJAMD_PAR(64, 64, false);
+    } else {
+        if (vec) PJAMD_PAR(32, 128, sizeof(T) == 2);
+        else PJAMD_PAR(32, 128, false);
+    }
+#undef PJAMD_PAR
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -1056,12 +1143,18 @@ int launch_par_batched(PF* pflows, const float* nflows, const float* nmars, cons
                        const int* desc, const int* tile_desc, int n_tiles, int tile_rows, int64_t B, bool mod,
                        bool overwrite, float sign, hipStream_t stream) {
     PJAMD_REQUIRE(grid_fits(n_tiles));
-    if (tile_rows == 64)
-        sum_par_batched<T, 64, 64, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
-            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign);
-    else
-        sum_par_batched<T, 32, 128, PF><<<(unsigned)n_tiles, 256, 0, stream>>>(
-            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign);
+#define PJAMD_PARB(TM_, TN_, V_)                                                                                  \
+    sum_par_batched<T, TM_, TN_, PF, V_><<<(unsigned)n_tiles, 256, 0, stream>>>(                                  \
+        pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign)
+    const bool vec = sizeof(T) == 2 && B % 4 == 0;
+    if (tile_rows == 64) {
+        if (vec) PJAMD_PARB(64, 64, sizeof(T) == 2);
+        else PJAMD_PARB(64, 64, false);
+    } else {
+        if (vec) PJAMD_PARB(32, 128, sizeof(T) == 2);
+        else PJAMD_PARB(32, 128, false);
+    }
+#undef PJAMD_PARB
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
