@@ -274,6 +274,110 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
     }
 }
 
+// The same sums when the whole batch fits one chunk (B <= kSortB), for kGroupTiles consecutive
+// tiles per workgroup: a variable's counting sort is done once for all the group's tiles of that
+// variable (the tiles of one variable are consecutive), each tile's flow rows are staged with
+// 16-byte loads, and each (node, category) sum is written once.
+constexpr int kGroupTiles = 8;
+
+template <typename PF>
+__global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pflows,
+                                                            const float* __restrict__ node_flows,
+                                                            const int* __restrict__ data, const int* __restrict__ vids,
+                                                            const int64_t* __restrict__ s_pfids,
+                                                            const int* __restrict__ ncats, int64_t num_nodes,
+                                                            int64_t node_offset, int64_t B, int pf_mode) {
+    __shared__ int cnt[kSortK + 1];
+    __shared__ int cur[kSortK];
+    __shared__ short perm[kSortB];
+    __shared__ __attribute__((aligned(16))) float fl[kSortTile][kSortB];
+    constexpr int KPT = kSortK / 256;
+    const int tid = threadIdx.x;
+    const int nb = (int)B;                                  // B <= kSortB
+    const int64_t g0 = (int64_t)blockIdx.x * kSortTile * kGroupTiles;
+    int cur_v = -1, K = 0;
+    for (int tt = 0; tt < kGroupTiles; ++tt) {
+        const int64_t i0 = g0 + (int64_t)tt * kSortTile;
+        if (i0 >= num_nodes) break;
+        const int n_here = (int)((num_nodes - i0) < kSortTile ? (num_nodes - i0) : kSortTile);
+        const int v = vids[i0];
+        if (v != cur_v) {
+            // counting sort of the variable's categories: cnt[k] = first position of category k
+            cur_v = v;
+            K = ncats[i0];
+            const int* xrow = data + (int64_t)v * B;
+            for (int k = tid; k <= K; k += 256) cnt[k] = 0;
+            __syncthreads();
+            for (int t = tid; t < nb; t += 256) {
+                const int x = xrow[t];
+                if (x >= 0 && x < K) atomicAdd(&cnt[x + 1], 1);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int per = (K + 1 + 63) / 64;
+                const int base = tid * per;
+                int local = 0;
+                for (int q = 0; q < per; ++q)
+                    if (base + q <= K) local += cnt[base + q];
+                int incl = local;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(incl, o, 64);
+                    if (tid >= o) incl += y;
+                }
+                int run = incl - local;
+                for (int q = 0; q < per; ++q)
+                    if (base + q <= K) { run += cnt[base + q]; cnt[base + q] = run; }
+            }
+            __syncthreads();
+            for (int k = tid; k < K; k += 256) cur[k] = cnt[k];
+            __syncthreads();
+            for (int t = tid; t < nb; t += 256) {
+                const int x = xrow[t];
+                if (x >= 0 && x < K) perm[atomicAdd(&cur[x], 1)] = (short)t;
+            }
+        }
+        // the tile's flow rows (B % 4 == 0: 16-byte loads)
+        const int nq = nb / 4;
+        for (int t = tid; t < n_here * nq; t += 256) {
+            const int r = t / nq, c = t - r * nq;
+            *reinterpret_cast<float4*>(&fl[r][4 * c]) =
+                *reinterpret_cast<const float4*>(node_flows + (node_offset + i0 + r) * B + 4 * c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+            const int k = tid + q * 256;
+            if (k >= K) continue;
+            float sums[kSortTile];
+#pragma unroll
+            for (int r = 0; r < kSortTile; ++r) sums[r] = 0.0f;
+            const int p0 = cnt[k], p1 = cnt[k + 1];
+            for (int p = p0; p < p1; ++p) {
+                const int c = perm[p];
+#pragma unroll
+                for (int r = 0; r < kSortTile; ++r) sums[r] += fl[r][c];
+            }
+            if (pf_mode == kPfOverwrite) {
+#pragma unroll
+                for (int r = 0; r < kSortTile; ++r)
+                    if (r < n_here) pflows[s_pfids[i0 + r] + k] = (PF)sums[r];
+            } else if (pf_mode == kPfAtomic) {
+#pragma unroll
+                for (int r = 0; r < kSortTile; ++r)
+                    if (r < n_here && sums[r] != 0.0f) atomicAdd(&pflows[s_pfids[i0 + r] + k], (PF)sums[r]);
+            } else {
+                PF old[kSortTile];
+#pragma unroll
+                for (int r = 0; r < kSortTile; ++r) old[r] = (r < n_here) ? pflows[s_pfids[i0 + r] + k] : (PF)0;
+#pragma unroll
+                for (int r = 0; r < kSortTile; ++r)
+                    if (r < n_here) pflows[s_pfids[i0 + r] + k] = old[r] + (PF)sums[r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Clears the parameter-flow slots of every node (used before the atomic path when the caller
 // asked for the flows to be overwritten).
 template <typename PF>
@@ -438,6 +542,10 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
     if (kind == 1) {
         bern_bwd_kernel<PF><<<(unsigned)num_nodes, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids,
                                                                     node_offset, B, pf_mode);
+    } else if (sorted_tiles && kmax <= kSortK && B <= kSortB && B % 4 == 0 &&
+               ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0) && (node_offset * B) % 4 == 0) {
+        cat_bwd_group_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile * kGroupTiles), 256, 0, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);
     } else if (sorted_tiles && kmax <= kSortK) {
         cat_bwd_sorted_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile), 256, 0, stream>>>(
             pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);

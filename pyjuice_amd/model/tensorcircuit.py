@@ -295,6 +295,17 @@ class TensorCircuit(nn.Module):
                 if covered[a:b].all():
                     premod.add(id(layer))
                     premod_row[a:b] = True
+        # Node-flow rows the backward must clear first: everything except the rows the fused
+        # kernels assign (every row of every child block they cover) and the root rows, as
+        # contiguous ranges (the whole buffer when that would take many launches).
+        need = np.ones(self.num_nodes, dtype=bool)
+        need[covered] = False
+        r0, r1 = self._root_node_range
+        need[r0:r1] = False
+        edges = np.flatnonzero(np.diff(np.concatenate([[0], need.astype(np.int8), [0]])))
+        zero_ranges = [(int(a), int(b)) for a, b in zip(edges[0::2], edges[1::2])]
+        if len(zero_ranges) > 64:
+            zero_ranges = [(0, self.num_nodes)]
         tables = dict()
         for lid, tabs in raw_tables.items():
             out = []
@@ -305,7 +316,7 @@ class TensorCircuit(nn.Module):
                 out.append((torch.from_numpy(code).to(torch.int32).to(device).contiguous(),
                             int((code != 0).sum()), int((code & 1).sum())))
             tables[lid] = out
-        return {"tables": tables, "prod_groups": fused_prod, "premod": premod}
+        return {"tables": tables, "prod_groups": fused_prod, "premod": premod, "zero_ranges": zero_ranges}
 
     def _build_par_batches(self, device):
         """Descriptor tables of the batched parameter-flow launch: every sum layer with block size
@@ -588,7 +599,11 @@ class TensorCircuit(nn.Module):
         with torch.no_grad():
             B = self.node_mars.size(1)
             self._ensure_buffers(B, need_flows=True)
-            self._fill(self.node_flows, 0.0)
+            if prec == "fp32_valu" or "zero_ranges" not in self._fused:
+                self._fill(self.node_flows, 0.0)
+            else:
+                for a, b in self._fused["zero_ranges"]:
+                    self._fill(self.node_flows[a:b], 0.0)
             r0, r1 = self._root_node_range
             if ll_weights is None:
                 self._fill(self.node_flows[r0:r1], 1.0)
