@@ -6,7 +6,10 @@ Workload (the headline configuration): HCLT over 784 categorical variables (28 x
 uint8 images (Gaussian-copula, AR(1) rho = 0.6).  One step is the training loop of the
 reference's examples: ``zero_grad`` (clear parameter flows), ``lls = pc(x)``,
 ``lls.mean().backward()`` (flow backward incl. parameter flows) and ``step`` (mini-batch EM,
-step 0.1, pseudocount 0.1).  Inputs and parameters are resident on the device.
+step 0.1, pseudocount 0.1).  Inputs and parameters are resident on the device.  Each step is
+recorded once as a HIP graph and replayed (``--no-graph``: launched kernel by kernel from the
+host); the per-kernel HIP-event timings behind ``roofline`` and ``kernels_ms_per_step`` come from
+the same kernels launched eagerly.
 
 Prints one JSON line.  ``value`` is samples/s of the precision given by ``--precision``
 (default BF16 on the matrix cores).  The plain FP32 HIP form (vector ALUs, ``fp32_valu``) and
@@ -55,7 +58,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
-    ap.add_argument("--graph", action="store_true", help="replay the inner layers from HIP graphs")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="launch each step's kernels from the host instead of replaying the step's HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=32)
     ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
@@ -92,11 +96,16 @@ def build_circuit(args, dev):
 def make_step(pc, xb, prec, graph):
     import pyjuice_amd as juice
 
+    if graph:
+        # the whole step (clear flows, forward, flow backward, EM) as one HIP graph
+        g = juice.optim.TrainStepGraph(pc, xb, lr=0.1, pseudocount=0.1, precision=prec)
+        return lambda: g()
+
     opt = juice.optim.CircuitOptimizer(pc, lr=0.1, pseudocount=0.1)
 
     def step():
         opt.zero_grad()
-        lls = pc(xb, precision=prec, record_cudagraph=graph, apply_cudagraph=graph)
+        lls = pc(xb, precision=prec)
         lls.mean().backward()
         opt.step()
         return lls
@@ -248,7 +257,9 @@ def main():
             continue
         results[prec] = {"ms_per_step": ms, "wall_ms_per_step": wall, "samples_per_s": args.batch / (ms * 1e-3)}
         log(f"[bench] {prec}: {ms:.3f} ms/step ({args.batch / (ms * 1e-3):.0f} samples/s)")
-        results[prec]["kernels"] = kernel_breakdown(step, min(args.steps, 5))
+        # per-launch HIP-event timing needs eager launches (a graph replay has no host calls)
+        results[prec]["kernels"] = kernel_breakdown(make_step(pc, xb, prec, False) if args.graph else step,
+                                                    min(args.steps, 5))
     if main_prec not in results:
         main_prec = "fp32"
     r = results[main_prec]

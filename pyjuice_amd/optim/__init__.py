@@ -4,7 +4,7 @@ from __future__ import annotations
 
 from typing import Dict, Optional
 
-__all__ = ["CircuitOptimizer"]
+__all__ = ["CircuitOptimizer", "TrainStepGraph"]
 
 
 class CircuitOptimizer:
@@ -40,3 +40,6 @@ class CircuitOptimizer:
         self.method, self.lr, self.pseudocount = pcs["method"], pcs["lr"], pcs["pseudocount"]
         if self.base_optimizer is not None:
             self.base_optimizer.load_state_dict(st)
+
+
+from .step_graph import TrainStepGraph  # noqa: E402

@@ -249,3 +249,43 @@ def test_chain_runs_match_layerwise(name, precision, B, device):
     assert U.rel_err(p1, p0) <= 10 * tol
     for a, b in zip(i1, i0):
         assert U.rel_err(a, b) <= 10 * tol
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs64", "hmm_small", "pd_small"])
+def test_train_step_graph_replays(name, device):
+    """A whole training step (clear flows, forward, flow backward, EM) recorded as one HIP graph:
+    each replay, on a new mini-batch and with eager work and synchronisations between replays,
+    gives the same log-likelihoods and updated parameters as the eager step on a twin circuit."""
+    import pyjuice_amd as juice
+    U.say(f"[step graph] {name}")
+    ns, cards, kw = _circuit(name)
+    pc_g = TensorCircuit(ns, precision="bf16", **kw)
+    pc_e = TensorCircuit(ns, precision="bf16", **kw)
+    pc_g.to(device)
+    pc_e.to(device)
+    pc_e.params.data.copy_(pc_g.params)
+    for a, b in zip(pc_e.input_layer_group, pc_g.input_layer_group):
+        a.params.data.copy_(b.params)
+    batches = [torch.from_numpy(U.random_data(cards, 64, seed=20 + i)).to(device) for i in range(4)]
+    opt = juice.optim.CircuitOptimizer(pc_e, lr=0.1, pseudocount=0.1)
+
+    def eager(x):
+        opt.zero_grad()
+        with torch.no_grad():
+            lls = pc_e(x)
+            pc_e.backward(x, flows_memory=0.0)
+        opt.step()
+        return lls
+
+    lls0 = eager(batches[0])                      # the graph's own eager warm-up step
+    step = juice.optim.TrainStepGraph(pc_g, batches[0], lr=0.1, pseudocount=0.1)
+    assert U.rel_err(step.warmup_lls.cpu().numpy(), lls0.cpu().numpy()) <= 1e-5
+    for x in batches[1:]:
+        want = eager(x).detach().cpu().numpy()
+        got = step(x).detach().cpu().numpy()
+        torch.cuda.synchronize()
+        _ = (pc_g.params.sum() + 0).item()        # eager work between replays
+        assert np.abs(got - want).max() <= 1e-4 * len(cards)
+        assert U.rel_err(pc_g.params.detach().cpu().numpy(), pc_e.params.detach().cpu().numpy()) <= 1e-5
+        for a, b in zip(pc_g.input_layer_group, pc_e.input_layer_group):
+            assert U.rel_err(a.params.detach().cpu().numpy(), b.params.detach().cpu().numpy()) <= 1e-5
