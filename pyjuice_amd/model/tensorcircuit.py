@@ -103,6 +103,12 @@ class TensorCircuit(nn.Module):
         self.use_chains = False
         self.chain_max_param_bytes = 1 << 20
         self._chain_cache = dict()
+        # Layer-wise recomputation (the reference's own scheme, tensorcircuit.py:446-447): one
+        # element buffer of the widest product group instead of one slab per group; the flow
+        # backward recomputes each product group before the sum group above it, and parameter
+        # flows are taken per layer inside the loop.  element_mars then takes max_g rows instead
+        # of the sum over groups, at the cost of a second product pass.
+        self.recompute_elements = False
 
     # ------------------------------------------------------------------ parameter-flow buffer
 
@@ -168,7 +174,7 @@ class TensorCircuit(nn.Module):
 
     def _chains(self, direction: str, precision: str, modify: bool = True, compute_param_flows: bool = True):
         """Chain runs (``pyjuice_amd.model.chain.plan``) of one pass, cached per configuration."""
-        if not self.use_chains or precision not in ("bf16", "fp16", "fp32"):
+        if not self.use_chains or self.recompute_elements or precision not in ("bf16", "fp16", "fp32"):
             return dict()
         key = (direction, "fp32" if precision == "fp32" else "16", modify, compute_param_flows,
                self.chain_max_param_bytes)
@@ -412,11 +418,12 @@ class TensorCircuit(nn.Module):
     def _ensure_buffers(self, B: int, need_flows: bool = False):
         dev = self.device
         assert dev.type == "cuda", "TensorCircuit passes run on the HIP device; call `pc.to('cuda')` first"
-        if self.node_mars is None or self.node_mars.size(1) != B:
+        rows = self.num_elements if self.recompute_elements else self.num_element_rows
+        if self.node_mars is None or self.node_mars.size(1) != B or self.element_mars.size(0) != rows:
             self.node_mars = torch.empty([self.num_nodes, B], device=dev, dtype=torch.float32)
-            self.element_mars = torch.empty([self.num_element_rows, B], device=dev, dtype=torch.float32)
+            self.element_mars = torch.empty([rows, B], device=dev, dtype=torch.float32)
             self._fill(self.node_mars[:self.num_dummy_nodes], 0.0)
-            for off in self._elem_offsets:
+            for off in ([0] if self.recompute_elements else self._elem_offsets):
                 self._fill(self.element_mars[off:off + self.num_dummy_eles], float("-inf"))
             self.node_flows = self.element_flows = None
         if need_flows and (self.node_flows is None or self.node_flows.size(1) != B):
@@ -450,7 +457,7 @@ class TensorCircuit(nn.Module):
 
     def _slab(self, group_id: int) -> torch.Tensor:
         """``element_mars`` rows of the product group ``group_id`` (or of the sum group right above it)."""
-        off = self._elem_offsets[group_id // 2]
+        off = 0 if self.recompute_elements else self._elem_offsets[group_id // 2]
         return self.element_mars.narrow(0, off, self._group_elems[group_id // 2])
 
     def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
@@ -537,7 +544,8 @@ class TensorCircuit(nn.Module):
 
     def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision, overwrite=False):
         # Element marginals of every layer are still resident (one slab per product group), so no
-        # product layer is recomputed here (the reference recomputes, tensorcircuit.py:446-447).
+        # product layer is recomputed here (the reference recomputes, tensorcircuit.py:446-447),
+        # unless ``recompute_elements`` keeps a single shared slab.
         # ``allow_modify_flows`` permits the in-place log(flow) - mar transform of the sum nodes'
         # flows (one pass per layer, re-read by the element- and parameter-flow kernels); with
         # ``modify_flows_in_place = False`` the kernels compute it on the fly instead.
@@ -548,7 +556,9 @@ class TensorCircuit(nn.Module):
         # sum children, already as log(flow) - mar), so that product backward and the children's
         # modify pass are not launched.
         modify = allow_modify_flows and self.modify_flows_in_place
-        batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu") else set()
+        recompute = self.recompute_elements
+        batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu" and
+                                                  not recompute) else set()
         fz = self._fused if precision != "fp32_valu" else {"tables": dict(), "prod_groups": set(), "premod": set()}
         runs = self._chains("bwd", precision, modify, compute_param_flows)
         lid = len(self.inner_layer_groups) - 1
@@ -567,6 +577,9 @@ class TensorCircuit(nn.Module):
                 if gid not in fz["prod_groups"]:
                     g.backward(self.node_flows, self.element_flows)
                 continue
+            if recompute:
+                # the element rows of the product group below, overwritten by later groups' forward
+                self.inner_layer_groups[gid - 1](self.node_mars, self._slab(gid - 1))
             for layer in g:
                 pf = self._pf if compute_param_flows and id(layer) not in batched else None
                 layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(gid), self.params,

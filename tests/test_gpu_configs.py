@@ -9,7 +9,8 @@ circuit and parameters, or against properties that hold at any size.
 3. PD 28x28, 256 latents, block size 32, BF16 and FP16: |dLL| <= 1e-3 nats/var; the FP64
    parameter-flow accumulator over several mini-batches equals the sum of its parts.
 4. HMM, 1024 hidden states, 4096 tokens, length 64, tied: |dLL| <= 1e-4 nats/var (FP32); the
-   autograd adjoint against oracle parameter flows and against exact finite differences.
+   autograd adjoint, with layer-wise recomputation of the element marginals, against oracle
+   parameter flows and against exact finite differences.
 5. HCLT-784, 512 latents, block size 32, 50 % of the sum-edge blocks pruned, batch 2048, BF16:
    |dLL| <= 1e-3 nats/var and parameter flows against the oracle.
 Oracle comparisons use a few samples of the batch (the FP64 CPU pass is the slow part).
@@ -169,7 +170,9 @@ def test_config4_hmm_adjoint(device):
     ref = cpu.forward(seqs[:n])[0]
     assert np.abs(lls[:n] - ref).max() / 64 <= 1e-4
 
-    # adjoint behind torch.autograd.Function against the oracle's parameter flows (4 sequences)
+    # adjoint behind torch.autograd.Function against the oracle's parameter flows (4 sequences),
+    # with layer-wise recomputation of the element marginals (one shared element buffer)
+    pc.recompute_elements = True
     params = pc.params.detach().clone().requires_grad_(True)
     in_params = [l.params.detach().clone().requires_grad_(True) for l in pc.input_layer_group]
     out = juice.log_likelihood(pc, xt[:n], params=params, input_params=in_params, precision="fp32")
@@ -184,6 +187,7 @@ def test_config4_hmm_adjoint(device):
     ref_grad = np.zeros_like(theta)
     ref_grad[ok] = fused[idx[ok]] / theta[ok]
     g = params.grad.detach().cpu().numpy().astype(np.float64)
+    assert pc.element_mars.size(0) == pc.num_elements < pc.num_element_rows
     assert U.rel_err(g[ok] * theta[ok], ref_grad[ok] * theta[ok]) <= 1e-4
 
     # exact finite differences: the circuit is multilinear, so scaling one parameter theta_k by

@@ -289,3 +289,30 @@ def test_train_step_graph_replays(name, device):
         assert U.rel_err(pc_g.params.detach().cpu().numpy(), pc_e.params.detach().cpu().numpy()) <= 1e-5
         for a, b in zip(pc_g.input_layer_group, pc_e.input_layer_group):
             assert U.rel_err(a.params.detach().cpu().numpy(), b.params.detach().cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs32", "hmm_small", "pd_sparse", "tying_block1"])
+@pytest.mark.parametrize("precision", ["bf16", "fp32_valu"])
+def test_recompute_elements_matches_resident(name, precision, device):
+    """Layer-wise recomputation (one shared element buffer, product groups recomputed in the flow
+    backward, per-layer parameter flows) gives the same results as resident element slabs, with
+    an element buffer of the widest group only."""
+    U.say(f"[recompute] {name} {precision}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = torch.from_numpy(U.random_data(cards, 40, seed=9)).to(device)
+    out = dict()
+    for rec in (False, True):
+        pc.recompute_elements = rec
+        lls = pc(X).detach().double().cpu().numpy()
+        pc.backward(X, flows_memory=0.0)
+        out[rec] = (lls, pc.param_flows.detach().double().cpu().numpy(),
+                    [l.param_flows.detach().double().cpu().numpy() for l in pc.input_layer_group],
+                    pc.element_mars.size(0))
+    assert out[True][3] == pc.num_elements and out[False][3] == pc.num_element_rows
+    assert out[True][3] <= out[False][3]
+    assert np.abs(out[True][0] - out[False][0]).max() <= 1e-6 * len(cards)
+    assert U.rel_err(out[True][1], out[False][1]) <= 1e-5
+    for a, b in zip(out[True][2], out[False][2]):
+        assert U.rel_err(a, b) <= 1e-5
