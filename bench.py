@@ -36,8 +36,8 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0
 # profiles/hclt784_L256_B512_bf16_hbm_traffic.txt.  Reported as roofline.traffic for the dominant
 # kernel (counters cannot be read from inside the timed process).
 TRAFFIC_MB_PER_LAUNCH = {
-    "pjamd_sum_backward_ele_to_children": 34.13, "pjamd_sum_forward": 15.15,
-    "pjamd_sum_backward_par_batched": 1141.55, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 981.71,
+    "pjamd_sum_backward_ele_to_children": 33.89, "pjamd_sum_forward": 15.15,
+    "pjamd_sum_backward_par_batched": 1206.89, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 833.87,
     "pjamd_input_forward": 652.21, "pjamd_sum_em": 570.03, "pjamd_input_em": 847.78,
 }
 HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--latents", type=int, default=256)
-    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=-1,
+                    help="HCLT block size: -1 = measured choice among num_latents / 2^k >= 64 (pyjuice_amd.autotune), "
+                         "0 = the builder's default (num_latents)")
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
@@ -78,18 +80,32 @@ def build_circuit(args, dev):
     t0 = time.time()
     data = synthetic.image_categorical(args.fit + args.batch, H=28, W=28, K=256, rho=0.6, seed=0, field_seed=1)
     log(f"[bench] data {data.shape} in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    torch.manual_seed(0)
+    xb = torch.from_numpy(data[args.fit:args.fit + args.batch]).to(dev)
     x_fit = torch.from_numpy(data[:args.fit]).float().to(dev)
-    ns = juice.structures.HCLT(x_fit, num_latents=args.latents, input_dist=juice.distributions.Categorical(256),
-                               chunk_size=64, block_size=args.block_size or None)
+
+    def make_nodes(bs):
+        torch.manual_seed(0)
+        return juice.structures.HCLT(x_fit, num_latents=args.latents, input_dist=juice.distributions.Categorical(256),
+                                     chunk_size=64, block_size=bs)
+
+    args.block_size_ms = None
+    if args.block_size < 0:
+        from pyjuice_amd import autotune
+        t0 = time.time()
+        cands = autotune.block_size_candidates(args.latents)
+        prec = "bf16" if args.precision == "auto" else args.precision
+        best, ms = autotune.tune_block_size(make_nodes, xb, cands, steps=3, warmup=1,
+                                            precision=None if prec == "auto" else prec)
+        args.block_size, args.block_size_ms = best, {str(k): round(v, 4) for k, v in ms.items()}
+        log(f"[bench] block size {best} chosen from {args.block_size_ms} ms/step in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ns = make_nodes(args.block_size or None)
     log(f"[bench] HCLT structure in {time.time() - t0:.1f}s")
     t0 = time.time()
     pc = juice.compile(ns, verbose=False)
     pc.to(dev)
     log(f"[bench] compiled: {pc.num_nodes} nodes, {pc.num_edges} edges, {pc.num_sum_params} sum params, "
         f"{len(pc.inner_layer_groups)} layer groups in {time.time() - t0:.1f}s")
-    xb = torch.from_numpy(data[args.fit:args.fit + args.batch]).to(dev)
     return ns, pc, xb, data[args.fit:args.fit + args.batch]
 
 
@@ -151,7 +167,7 @@ def roofline_of(name, d, prec, args):
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
-    headline = args.latents == 256 and args.batch == 512 and not args.block_size and prec == "bf16"
+    headline = args.latents == 256 and args.batch == 512 and args.block_size == 256 and prec == "bf16"
     traffic = TRAFFIC_MB_PER_LAUNCH.get(name) if headline else None
     traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
@@ -247,10 +263,12 @@ def main():
         pc.params.data.copy_(state["params"])
         for l, p in zip(pc.input_layer_group, in_state):
             l.params.data.copy_(p)
-        step = make_step(pc, xb, prec, args.graph)
         try:
+            # log-likelihoods at the starting parameters, before any step (recording the step
+            # graph runs one eager step)
             with torch.no_grad():
                 lls_by_prec[prec] = pc(xb, precision=prec).detach().cpu().numpy()[:, 0]
+            step = make_step(pc, xb, prec, args.graph)
             ms, wall = time_steps(step, args.steps, args.warmup)
         except Exception as e:  # an unsupported precision is reported, not fatal
             log(f"[bench] {prec}: {e}")
@@ -291,7 +309,8 @@ def main():
                                f"block_size={args.block_size or 'default'}, batch={args.batch}; "
                                f"forward + flow backward + mini-batch EM",
                    "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
-                   "precision": main_prec, "hip_graph": bool(args.graph)},
+                   "precision": main_prec, "hip_graph": bool(args.graph),
+                   "block_size_ms_per_step": args.block_size_ms},
         "roofline": roof,
         "cpu_baseline": cpu,
         "precisions": {p: {"samples_per_s": round(v["samples_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 4)}
@@ -299,7 +318,9 @@ def main():
         "dll_per_var_vs_fp64": acc,
         "kernels_ms_per_step": {k: {"ms": round(v["ms"], 4), "launches": v["launches"],
                                     "GB/s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
-                                    "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2)}
+                                    "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2),
+                                    "largest_launch_GB/s": round(v["big"][0] / max(v["big"][1], 1e-9) / 1e6, 1),
+                                    "largest_launch_MB": round(v["big"][0] / 1e6, 2)}
                                 for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])},
     }
     for base in ("fp32_valu", "fp32"):

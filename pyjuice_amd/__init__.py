@@ -9,8 +9,8 @@ from .nodes import (inputs, multiply, summate, set_block_size, structural_proper
                     CircuitNodes, InputNodes, ProdNodes, SumNodes)
 from .layer import InputLayer, ProdLayer, SumLayer, LayerGroup
 from .model import TensorCircuit, compile, log_likelihood
-from . import structures, optim, queries, transformations
+from . import structures, optim, queries, transformations, autotune
 
 __all__ = ["compile", "TensorCircuit", "log_likelihood", "inputs", "multiply", "summate", "set_block_size",
            "structural_properties", "distributions", "structures", "optim", "queries", "transformations",
-           "InputLayer", "ProdLayer", "SumLayer", "LayerGroup"]
+           "autotune", "InputLayer", "ProdLayer", "SumLayer", "LayerGroup"]

@@ -136,14 +136,19 @@ class LaunchTimer:
         _timer = None
 
     def summary(self):
+        """Per entry point: launches, total ms, algorithmic bytes and flops, and the largest
+        launch (most algorithmic bytes) as ``big = (bytes, ms)``."""
         torch.cuda.synchronize()
         out = dict()
         for name, a, b, nbytes, flops in self.records:
-            d = out.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            d = out.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0, "big": (0.0, 0.0)})
+            ms = a.elapsed_time(b)
             d["launches"] += 1
-            d["ms"] += a.elapsed_time(b)
+            d["ms"] += ms
             d["bytes"] += nbytes
             d["flops"] += flops
+            if nbytes > d["big"][0]:
+                d["big"] = (nbytes, ms)
         return out
 
 

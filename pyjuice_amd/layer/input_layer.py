@@ -323,7 +323,7 @@ class InputLayer(Layer, nn.Module):
         N.call("pjamd_input_backward", self._kind(), N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data),
                N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0],
                B, flags, N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B
-                                            + 2 * self.param_flows.element_size() * self.num_param_flows, 0))
+                                            + (1 if overwrite else 2) * pf.element_size() * self.num_param_flows, 0))
 
     def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False):
         """Tied-flow fusing then the closed-form EM update (``input_layer.py:663-724``)."""
