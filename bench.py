@@ -58,6 +58,11 @@ def parse():
                     help="HCLT block size: -1 = measured choice among num_latents / 2^k >= 64 (pyjuice_amd.autotune), "
                          "0 = the builder's default (num_latents)")
     ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--prune", type=float, default=0.0,
+                    help="fraction of sum-edge blocks removed at seeded random (seed 4) after the structure is "
+                         "built, e.g. 0.5 with --latents 512 --block-size 32 --batch 2048")
+    ap.add_argument("--chains", action="store_true",
+                    help="run eligible stretches of layers as chain kernels (pjamd_chain_forward/_backward)")
     ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
     ap.add_argument("--no-graph", dest="graph", action="store_false",
@@ -85,8 +90,11 @@ def build_circuit(args, dev):
 
     def make_nodes(bs):
         torch.manual_seed(0)
-        return juice.structures.HCLT(x_fit, num_latents=args.latents, input_dist=juice.distributions.Categorical(256),
-                                     chunk_size=64, block_size=bs)
+        ns = juice.structures.HCLT(x_fit, num_latents=args.latents, input_dist=juice.distributions.Categorical(256),
+                                   chunk_size=64, block_size=bs)
+        if args.prune > 0:
+            juice.transformations.prune_sum_blocks(ns, args.prune, seed=4)
+        return ns
 
     args.block_size_ms = None
     if args.block_size < 0:
@@ -103,6 +111,7 @@ def build_circuit(args, dev):
     log(f"[bench] HCLT structure in {time.time() - t0:.1f}s")
     t0 = time.time()
     pc = juice.compile(ns, verbose=False)
+    pc.use_chains = bool(args.chains)
     pc.to(dev)
     log(f"[bench] compiled: {pc.num_nodes} nodes, {pc.num_edges} edges, {pc.num_sum_params} sum params, "
         f"{len(pc.inner_layer_groups)} layer groups in {time.time() - t0:.1f}s")
@@ -167,7 +176,8 @@ def roofline_of(name, d, prec, args):
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
-    headline = args.latents == 256 and args.batch == 512 and args.block_size == 256 and prec == "bf16"
+    headline = (args.latents == 256 and args.batch == 512 and args.block_size == 256 and prec == "bf16"
+                and args.prune == 0 and not args.chains)
     traffic = TRAFFIC_MB_PER_LAUNCH.get(name) if headline else None
     traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
@@ -306,10 +316,11 @@ def main():
         "dtype": {"fp32": "f32", "fp32_valu": "f32", "bf16": "bf16", "fp16": "f16"}[main_prec],
         "data": "synthetic",
         "config": {"workload": f"HCLT-784 (28x28, 256 categories), num_latents={args.latents}, "
-                               f"block_size={args.block_size or 'default'}, batch={args.batch}; "
-                               f"forward + flow backward + mini-batch EM",
+                               f"block_size={args.block_size or 'default'}, "
+                               + (f"{args.prune:.0%} of the sum-edge blocks pruned, " if args.prune > 0 else "")
+                               + f"batch={args.batch}; forward + flow backward + mini-batch EM",
                    "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
-                   "precision": main_prec, "hip_graph": bool(args.graph),
+                   "precision": main_prec, "hip_graph": bool(args.graph), "chain_kernels": bool(args.chains),
                    "block_size_ms_per_step": args.block_size_ms},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -127,47 +127,63 @@ __global__ __launch_bounds__(kThreads) void sum_fwd_tiled(float* __restrict__ no
     }
 }
 
-// Forward, any bs (used for bs < 16): one workgroup per (node, 64 batch columns); its four
-// wavefronts split the children (k = w, w + 4, ...), lanes run along the batch.  Pass 1 takes
-// the column max, pass 2 the sum of theta * exp(e - max); the loop bodies are unrolled so that
-// many child rows are in flight per wavefront (a node with hundreds of children, such as a
-// circuit's root, would otherwise be a chain of dependent loads).
-constexpr int kSmallWaves = kThreads / 64;
+// Forward, any bs (used for bs < 16): one workgroup per (node, 64 batch columns); its W
+// wavefronts split the children in chunks of kRowChunk (wavefront w takes chunks w, w + W, ...),
+// lanes run along the batch.  A chunk's index, element and parameter loads are all issued before
+// the first use (indices clamped to the last real child, extra entries masked afterwards), and the
+// chunk is folded into a running (max, sum of theta * exp(e - max)) pair, so a node with hundreds
+// of children (a circuit's root) costs a few memory round trips instead of one per child.  The
+// wavefronts' pairs are combined through LDS.
+constexpr int kRowChunk = 16;
+constexpr int kSmallWaves = kThreads / 64;   // wavefronts of the other row-wise kernels
 
-__global__ __launch_bounds__(kThreads) void sum_fwd_rowwise(float* __restrict__ node_mars, const float* __restrict__ emars,
-                                                            const float* __restrict__ params, const int* __restrict__ nids,
-                                                            const int* __restrict__ cids, const int* __restrict__ pids,
-                                                            const int* __restrict__ nch, int K, int bs, int64_t B) {
-    __shared__ float red[kSmallWaves][64];
+template <int W>
+__global__ __launch_bounds__(64 * W) void sum_fwd_rowwise(float* __restrict__ node_mars, const float* __restrict__ emars,
+                                                          const float* __restrict__ params, const int* __restrict__ nids,
+                                                          const int* __restrict__ cids, const int* __restrict__ pids,
+                                                          const int* __restrict__ nch, int K, int bs, int64_t B) {
+    __shared__ float rm[W][64], rs[W][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t b = (int64_t)blockIdx.y * 64 + lane;
     const bool ok = b < B;
+    const int64_t bc = ok ? b : 0;      // in-bounds column for lanes past the batch (result unused)
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
     const int n = nch[j];
     const int* crow = cids + (int64_t)j * K;
     const int* prow = pids + (int64_t)j * K;
-    float m = kNegInf;
-#pragma unroll 8
-    for (int k = w; k < n; k += kSmallWaves) m = fmaxf(m, ok ? emars[(int64_t)crow[k] * B + b] : kNegInf);
-    red[w][lane] = m;
-    __syncthreads();
-    m = red[0][lane];
+    float m = kNegInf, s = 0.0f;
+    for (int k0 = w * kRowChunk; k0 < n; k0 += W * kRowChunk) {
+        float e[kRowChunk], p[kRowChunk];
 #pragma unroll
-    for (int q = 1; q < kSmallWaves; ++q) m = fmaxf(m, red[q][lane]);
-    __syncthreads();
-    float s = 0.0f;
-    if (m != kNegInf) {
-#pragma unroll 8
-        for (int k = w; k < n; k += kSmallWaves)
-            s = fmaf(params[(int64_t)prow[k] + i], __expf((ok ? emars[(int64_t)crow[k] * B + b] : kNegInf) - m), s);
+        for (int q = 0; q < kRowChunk; ++q) {
+            const int k = min(k0 + q, n - 1);
+            e[q] = emars[(int64_t)crow[k] * B + bc];
+            p[q] = params[(int64_t)prow[k] + i];
+        }
+        float mc = kNegInf;
+#pragma unroll
+        for (int q = 0; q < kRowChunk; ++q) mc = (k0 + q < n) ? fmaxf(mc, e[q]) : mc;
+        const float mn = fmaxf(m, mc);
+        if (mn != kNegInf) {
+            s *= __expf(m - mn);
+#pragma unroll
+            for (int q = 0; q < kRowChunk; ++q) s = (k0 + q < n) ? fmaf(p[q], __expf(e[q] - mn), s) : s;
+            m = mn;
+        }
     }
-    red[w][lane] = s;
+    rm[w][lane] = m;
+    rs[w][lane] = s;
     __syncthreads();
     if (w == 0 && ok) {
-        s = red[0][lane];
+        float M = kNegInf;
 #pragma unroll
-        for (int q = 1; q < kSmallWaves; ++q) s += red[q][lane];
-        node_mars[((int64_t)nids[j] + i) * B + b] = (s > 0.0f && m != kNegInf) ? __logf(s) + m : kNegInf;
+        for (int q = 0; q < W; ++q) M = fmaxf(M, rm[q][lane]);
+        float S = 0.0f;
+        if (M != kNegInf) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) S += (rm[q][lane] == kNegInf) ? 0.0f : rs[q][lane] * __expf(rm[q][lane] - M);
+        }
+        node_mars[((int64_t)nids[j] + i) * B + b] = (S > 0.0f && M != kNegInf) ? __logf(S) + M : kNegInf;
     }
 }
 
@@ -514,8 +530,14 @@ PJAMD_EXPORT int pjamd_sum_forward(float* node_mars, const float* element_mars, 
         sum_fwd_tiled<16><<<dim3((unsigned)n_blocks, gy), kThreads, 0, stream>>>(node_mars, element_mars, params, nids,
                                                                                 cids, pids, nch, K, bs, B);
     } else {
-        sum_fwd_rowwise<<<dim3((unsigned)((int64_t)n_blocks * bs), cdiv(B, 64)), kThreads, 0, stream>>>(
-            node_mars, element_mars, params, nids, cids, pids, nch, K, bs, B);
+        // wide rows (a root over hundreds of children): 16 wavefronts share a node's children
+        const dim3 grid((unsigned)((int64_t)n_blocks * bs), cdiv(B, 64));
+        if (K >= 8 * kRowChunk)
+            sum_fwd_rowwise<16><<<grid, 1024, 0, stream>>>(node_mars, element_mars, params, nids, cids, pids, nch, K,
+                                                           bs, B);
+        else
+            sum_fwd_rowwise<4><<<grid, 256, 0, stream>>>(node_mars, element_mars, params, nids, cids, pids, nch, K, bs,
+                                                         B);
     }
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;

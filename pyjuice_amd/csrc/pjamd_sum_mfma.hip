@@ -734,6 +734,185 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
     }
 }
 
+// Element flows with 16-byte loads for the shapes sum_ele_vec does not take: child blocks of 32
+// rows (32 x 128 tiles) and parent blocks smaller than a stage (a stage of TK flattened parent
+// rows then spans several parent blocks).  The parent-block indices of the child block (at most
+// kEleIdx) are staged in LDS once; every thread finds the parent block of its own rows there, so
+// each load is unconditional (index clamped to a real entry, value masked afterwards) and a stage
+// is one round trip of 16-byte loads.  Node flows must already hold log(flow) - mar.
+// Requirements (checked by the launcher): 16-bit operands, B % 4 == 0, parent blocks of >= 16 rows,
+// 16-byte-aligned parameter rows, n_pars entries <= kEleIdx.  FUSE as sum_ele_mfma (0 or 1).
+//
+// Thread layout, TN = 4096 / TM columns: flow rows: column quad q = tid % (TN / 4), row group
+// g = tid / (TN / 4) of RPG = TK * TN / 1024 consecutive rows (one parent block: RPG <= 16 <= bs);
+// parameters: k quad kq = tid % 32 (4 consecutive flattened parent rows, one parent block since
+// bs % 4 == 0), child rows ig + 8 r.
+constexpr int kEleIdx = 256;
+
+template <typename T, int TM, int FUSE>
+__global__ __launch_bounds__(256) void sum_ele_vec_blocks(float* __restrict__ eflows, const float* nflows,
+                                                          const float* __restrict__ nmars, const float* __restrict__ emars,
+                                                          const float* __restrict__ params, const int* __restrict__ chids,
+                                                          const int* __restrict__ parids, const int* __restrict__ parpids,
+                                                          const int* __restrict__ npar, int KP, int lbs, int64_t B,
+                                                          int mtiles, int btiles, bool accumulate, float* child_flows,
+                                                          const int* __restrict__ ch_nodes, int Kc, bool mod_children) {
+    constexpr int TN = 4096 / TM, WM = TM / 32;
+    constexpr int QN = TN / 4, GR = 256 / QN, RPG = TK / GR, AR = TM / 8;
+    constexpr int LD = TK + MF<T>::PAD;
+    static_assert(RPG == 8 || RPG == 16, "row groups of 8 or 16 flow rows");
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[GR][TN], mrun[TN], msc[TN];
+    __shared__ int spar[kEleIdx], spp[kEleIdx];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = 1 << lbs;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % WM, wn = w / WM;
+    const int np = npar[j];
+    const int q = tid % QN, g = tid / QN;
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;                          // B % 4 == 0: a quad is all in or all out
+    const int64_t bqc = bok ? bq : B - 4;
+    const int kq = tid % 32, ig = tid / 32;
+    const int mycol = wn * 32 + (lane & 31);
+    for (int u = tid; u < KP; u += 256) {
+        spar[u] = parids[(int64_t)j * KP + u];
+        spp[u] = parpids[(int64_t)j * KP + u];
+    }
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+    const int kf = np << lbs;
+
+    // element marginals of the epilogue, independent of the reduction
+    const int64_t b = b0 + mycol;
+    const int64_t obase = (int64_t)chids[j] + m0 + wm * 32;
+    float epre[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
+    __syncthreads();
+
+    float4 lv[RPG], av[AR];
+    auto load = [&](int k0) {
+        const int kr = k0 + RPG * g;                  // first flattened row of this thread's group
+        const bool rok = kr < kf;
+        const int krc = rok ? kr : 0;
+        const int64_t rbase = (int64_t)spar[krc >> lbs] + (krc & (bs - 1));
+#pragma unroll
+        for (int r = 0; r < RPG; ++r) lv[r] = *reinterpret_cast<const float4*>(nflows + (rbase + r) * B + bqc);
+#pragma unroll
+        for (int r = 0; r < RPG; ++r)
+            if (!(rok && bok)) lv[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        const int ka = k0 + 4 * kq;
+        const bool aok = ka < kf;
+        const int kac = aok ? ka : 0;
+        const float* pp = params + (int64_t)spp[kac >> lbs] + (int64_t)(m0 + ig) * bs + (kac & (bs - 1));
+#pragma unroll
+        for (int r = 0; r < AR; ++r) av[r] = *reinterpret_cast<const float4*>(pp + (int64_t)(8 * r) * bs);
+#pragma unroll
+        for (int r = 0; r < AR; ++r)
+            if (!aok) av[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    if (kf > 0) load(0);
+    for (int k0 = 0; k0 < kf; k0 += TK) {
+        // column max of the stage, folded into the running max
+        float mx[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+        for (int r = 0; r < RPG; ++r) {
+            mx[0] = fmaxf(mx[0], lv[r].x);
+            mx[1] = fmaxf(mx[1], lv[r].y);
+            mx[2] = fmaxf(mx[2], lv[r].z);
+            mx[3] = fmaxf(mx[3], lv[r].w);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[g][4 * q + c] = mx[c];
+        __syncthreads();
+        if (tid < TN) {
+            float x = red[0][tid];
+#pragma unroll
+            for (int i = 1; i < GR; ++i) x = fmaxf(x, red[i][tid]);
+            const float mo = mrun[tid];
+            const float mnew = fmaxf(mo, x);
+            msc[tid] = (mnew == kNegInf) ? 1.0f : __expf(mo - mnew);
+            mrun[tid] = mnew;
+        }
+        __syncthreads();
+        // B tile: Bs[4 q + c][RPG g + r] = exp(lv[r][c] - max[c])
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float mn = mrun[4 * q + c];
+#pragma unroll
+            for (int h = 0; h < RPG / 8; ++h) {
+                float e[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const float4 v = lv[8 * h + r];
+                    const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+                    e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                }
+                Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
+            }
+        }
+        // A tile: As[ig + 8 r][4 kq .. 4 kq + 3]
+#pragma unroll
+        for (int r = 0; r < AR; ++r) {
+            typedef T T4 __attribute__((ext_vector_type(4)));
+            T4 x;
+            x[0] = MF<T>::cvt(av[r].x);
+            x[1] = MF<T>::cvt(av[r].y);
+            x[2] = MF<T>::cvt(av[r].z);
+            x[3] = MF<T>::cvt(av[r].w);
+            *reinterpret_cast<T4*>(&As[ig + 8 * r][4 * kq]) = x;
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < kf) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    if (b < B) {
+        const float m = mrun[mycol];
+        float v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float e = epre[r];
+            v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
+        }
+        if (FUSE) {
+            const int* crow = ch_nodes + (int64_t)j * Kc;
+            for (int k = 0; k < Kc; ++k) {
+                const int code = crow[k];
+                if (code == 0) continue;
+                const int64_t cbase = (int64_t)(code >> 1) + m0 + wm * 32;
+                if ((code & 1) && mod_children) {
+                    float mv[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mv[r] = nmars[(cbase + acc_row(r, lane)) * B + b];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                            (mv[r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t o = (obase + acc_row(r, lane)) * B + b;
+                eflows[o] = accumulate ? eflows[o] + v[r] : v[r];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Parameter flows of one tile = (node block j, TM parent rows from m0) x (TN children from kt0),
 // reduced over the batch range [bstart, bend).  Both operand tiles are loaded with lanes along
@@ -1060,6 +1239,23 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
     const int bt = (int)btiles;
     const bool ob = bs >= TK;
     if constexpr (sizeof(T) == 2) {
+        // child blocks of 32 rows, or parent blocks shorter than a stage: block-spanning form
+        if (aligned && mod && B % 4 == 0 && bs >= 16 && KP <= kEleIdx && (cbs == 32 || (cbs >= 64 && !ob))) {
+#define PJAMD_ELE_BLK(TM_, F_)                                                                                    \
+    sum_ele_vec_blocks<T, TM_, F_><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids,     \
+                                                          parpids, npar, KP, lbs, B, mtiles, bt, accum, child_flows, \
+                                                          ch_nodes, Kc, mod_children)
+            if (cbs == 32) {
+                if (ch_nodes != nullptr) PJAMD_ELE_BLK(32, 1);
+                else PJAMD_ELE_BLK(32, 0);
+            } else {
+                if (ch_nodes != nullptr) PJAMD_ELE_BLK(64, 1);
+                else PJAMD_ELE_BLK(64, 0);
+            }
+#undef PJAMD_ELE_BLK
+            PJAMD_LAUNCH_CHECK();
+            return PJAMD_OK;
+        }
         if (aligned && ob && cbs >= 64 && B % 4 == 0) {
 #define PJAMD_ELE_VEC(F_, P_)                                                                                     \
     sum_ele_vec<T, F_, P_><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, \

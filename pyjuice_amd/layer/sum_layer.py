@@ -37,6 +37,24 @@ def _params_aligned(idx: torch.Tensor, counts: torch.Tensor, bs: int) -> bool:
     return bs % 4 == 0 and bool(((idx[real] % 4) == 0).all())
 
 
+def _merge_rows(tables: Sequence[tuple], count_col: int) -> tuple:
+    """Concatenate partition tables ``(ids [n], idx_1 [n, K_1], ..., counts [n])`` row-wise,
+    zero-padding the 2-D tables to the widest K (index 0 is the padding entry of every table),
+    and order the rows by decreasing count (stable)."""
+    K = max(int(t[1].size(1)) for t in tables)
+    out = []
+    for col in range(len(tables[0])):
+        parts = []
+        for t in tables:
+            v = t[col]
+            if v.dim() == 2 and v.size(1) < K:
+                v = torch.nn.functional.pad(v, (0, K - v.size(1)))
+            parts.append(v)
+        out.append(torch.cat(parts, dim=0))
+    order = torch.sort(out[count_col], descending=True, stable=True).indices
+    return tuple(v[order].contiguous() for v in out)
+
+
 def _i32(t: torch.Tensor) -> torch.Tensor:
     if t.numel() > 0:
         assert int(t.max()) < 2 ** 31, "index exceeds the int32 range of the HIP kernels"
@@ -150,33 +168,54 @@ class SumLayer(Layer, nn.Module):
             self.build_device_tables()
         return self
 
+    # One launch per layer and pass: the partitions of a layer (and, in the backward, those of
+    # one child block size) are concatenated into one table, padded to the widest partition.
+    # Every kernel bounds its reduction by the block's own child / parent count, so the padding
+    # costs index memory only, and blocks of uneven fan-in (pruned circuits) share one grid.
+    merge_partitions = True
+
     def build_device_tables(self):
         """int32 launch tables (``_dev``) of the forward and backward partitions on the layer's
-        device, with each forward partition's child-run promise."""
+        device, with each forward table's child-run promise.  With ``merge_partitions`` a layer
+        has one forward table and one backward table per child block size, rows ordered by
+        decreasing fan-in so the longest tiles are dispatched first."""
         device = self.device
-        fw = []
-        self._fw_child_run = []
-        self._fw_aligned, self._bk_aligned = [], []
+        fw, fw_run, fw_al = [], [], []
         for p in range(self.num_fw_partitions):
             n = self.partitioned_nids[p]
             if n.numel() == 0:
                 continue
             # Real children of block j occupy columns [0, n_chs_j); the rest is padding.
-            fw.append((_i32(n), _i32(self.partitioned_cids[p]), _i32(self.partitioned_pids[p]),
-                       _i32(self.partitioned_pfids[p]), _i32(self._fw_n_chs[p].to(device)),
-                       int(self._fw_n_chs[p].sum())))
-            self._fw_child_run.append(self._child_run(p))
-            self._fw_aligned.append(_params_aligned(self.partitioned_pids[p], self._fw_n_chs[p], self.block_size))
-        bk = []
+            fw.append((n, self.partitioned_cids[p], self.partitioned_pids[p], self.partitioned_pfids[p],
+                       self._fw_n_chs[p].to(device)))
+            fw_run.append(self._child_run(p))
+            fw_al.append(_params_aligned(self.partitioned_pids[p], self._fw_n_chs[p], self.block_size))
+        bk, bk_al = [], []
         for q in range(self.num_bk_partitions):
             c = self.partitioned_chids[q]
             if c.numel() == 0:
                 continue
-            bk.append((_i32(c), _i32(self.partitioned_parids[q]), _i32(self.partitioned_parpids[q]),
-                       _i32(self._bk_n_pars[q].to(device)), int(self.cs_block_sizes[q]),
-                       int(self._bk_n_pars[q].sum())))
-            self._bk_aligned.append(_params_aligned(self.partitioned_parpids[q], self._bk_n_pars[q], self.block_size))
-        self._dev = {"fw": fw, "bk": bk}
+            bk.append((int(self.cs_block_sizes[q]), c, self.partitioned_parids[q], self.partitioned_parpids[q],
+                       self._bk_n_pars[q].to(device)))
+            bk_al.append(_params_aligned(self.partitioned_parpids[q], self._bk_n_pars[q], self.block_size))
+        if self.merge_partitions and len(fw) > 1:
+            fw = [_merge_rows(fw, count_col=4)]
+            fw_run = [min(fw_run) if all(r > 0 for r in fw_run) else 0]
+            fw_al = [all(fw_al)]
+        if self.merge_partitions and len(bk) > 1:
+            by_cbs = dict()
+            for entry, al in zip(bk, bk_al):
+                by_cbs.setdefault(entry[0], []).append((entry, al))
+            bk, bk_al = [], []
+            for cbs, items in by_cbs.items():
+                rows = [e[1:] for e, _ in items]
+                bk.append((cbs,) + (_merge_rows(rows, count_col=3) if len(rows) > 1 else rows[0]))
+                bk_al.append(all(al for _, al in items))
+        self._fw_child_run, self._fw_aligned, self._bk_aligned = fw_run, fw_al, bk_al
+        self._dev = {
+            "fw": [(_i32(n), _i32(c), _i32(p), _i32(f), _i32(k), int(k.sum())) for n, c, p, f, k in fw],
+            "bk": [(_i32(c), _i32(pa), _i32(pp), _i32(k), cbs, int(k.sum())) for cbs, c, pa, pp, k in bk],
+        }
 
     # ------------------------------------------------------------------ passes
 

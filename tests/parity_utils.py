@@ -33,6 +33,14 @@ def hclt_categorical(num_vars=12, K=8, L=32, block_size=None, n=256, seed=0):
     return ns, x, [K] * num_vars
 
 
+def hclt_pruned(num_vars=10, K=8, L=128, block_size=32, frac=0.5, seed=0):
+    """HCLT with a seeded fraction of its sum-edge blocks removed: uneven fan-in, so several sum
+    layers split into more than one forward or backward partition."""
+    ns, x, cards = hclt_categorical(num_vars=num_vars, K=K, L=L, block_size=block_size, seed=seed)
+    juice.transformations.prune_sum_blocks(ns, frac, seed=4)
+    return ns, x, cards
+
+
 def hclt_bernoulli(num_vars=16, L=8, n=512, seed=0):
     """The small Bernoulli HCLT of the CPU-scale workload: chain of flips with p ~ U(0.1, 0.4)."""
     g = np.random.Generator(np.random.PCG64(seed))

@@ -66,3 +66,67 @@ def test_unreached_products_are_masked():
             for _, p, _ in layer._dev["bk"]:
                 vals = set(p.flatten().tolist()) - {0}
                 assert vals <= covered
+
+
+@pytest.mark.parametrize("num_vars,L", [(10, 128), (12, 64)])
+def test_merged_partition_tables(num_vars, L):
+    """One launch table per layer (and child block size): the rows are exactly the partitions'
+    rows with zero padding on the right, merged tables ordered by decreasing child / parent
+    count, and the merged child-run promise holds row by row."""
+    import torch
+    from pyjuice_amd.layer.sum_layer import SumLayer
+    assert SumLayer.merge_partitions
+    pc = TensorCircuit(U.hclt_pruned(num_vars=num_vars, L=L)[0])
+    pc._build_launch_plans(torch.device("cpu"))
+    merged = 0
+    for g in pc.inner_layer_groups:
+        if g.is_prod():
+            continue
+        for layer in g:
+            fw = layer._dev["fw"]
+            parts = [p for p in range(layer.num_fw_partitions) if layer.partitioned_nids[p].numel() > 0]
+            assert len(fw) == 1 and len(layer._fw_child_run) == 1 and len(layer._fw_aligned) == 1
+            nids, cids, pids, pfids, nch, tot = (t.numpy() if hasattr(t, "numpy") else t for t in fw[0])
+            merged += len(parts) > 1
+            want = dict()
+            for p in parts:
+                n = layer._fw_n_chs[p].numpy()
+                for j, nid in enumerate(layer.partitioned_nids[p].numpy()):
+                    k = int(n[j])
+                    want[int(nid)] = (k, layer.partitioned_cids[p].numpy()[j, :k].tolist(),
+                                      layer.partitioned_pids[p].numpy()[j, :k].tolist(),
+                                      layer.partitioned_pfids[p].numpy()[j, :k].tolist())
+            assert sorted(nids.tolist()) == sorted(want)
+            assert tot == int(nch.sum()) and (len(parts) == 1 or np.all(np.diff(nch) <= 0))
+            for j, nid in enumerate(nids.tolist()):
+                k = int(nch[j])
+                assert (k, cids[j, :k].tolist(), pids[j, :k].tolist(), pfids[j, :k].tolist()) == want[nid]
+                assert not cids[j, k:].any() and not pids[j, k:].any() and not pfids[j, k:].any()
+            r = layer._fw_child_run[0]
+            for j in range(cids.shape[0]):
+                for k in range(int(nch[j])):
+                    k0 = (k // r) * r if r else k
+                    assert cids[j, k] == cids[j, k0] + (k - k0)
+                    assert pids[j, k] == pids[j, k0] + (k - k0) * layer.block_size
+            by_cbs = dict()
+            for q in range(layer.num_bk_partitions):
+                if layer.partitioned_chids[q].numel() == 0:
+                    continue
+                npar = layer._bk_n_pars[q].numpy()
+                d = by_cbs.setdefault(int(layer.cs_block_sizes[q]), dict())
+                for i, c in enumerate(layer.partitioned_chids[q].numpy()):
+                    k = int(npar[i])
+                    d[int(c)] = (k, layer.partitioned_parids[q].numpy()[i, :k].tolist(),
+                                 layer.partitioned_parpids[q].numpy()[i, :k].tolist())
+            assert sorted(e[4] for e in layer._dev["bk"]) == sorted(by_cbs)
+            for chids, parids, parpids, npar, cbs, totp in layer._dev["bk"]:
+                chids, parids, parpids, npar = chids.numpy(), parids.numpy(), parpids.numpy(), npar.numpy()
+                n_parts = sum(1 for q in range(layer.num_bk_partitions) if layer.partitioned_chids[q].numel() > 0
+                              and int(layer.cs_block_sizes[q]) == cbs)
+                assert totp == int(npar.sum()) and (n_parts == 1 or np.all(np.diff(npar) <= 0))
+                assert sorted(chids.tolist()) == sorted(by_cbs[cbs])
+                for i, c in enumerate(chids.tolist()):
+                    k = int(npar[i])
+                    assert (k, parids[i, :k].tolist(), parpids[i, :k].tolist()) == by_cbs[cbs][c]
+                    assert not parids[i, k:].any() and not parpids[i, k:].any()
+    assert merged > 0
