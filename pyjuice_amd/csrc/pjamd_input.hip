@@ -274,13 +274,15 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
     }
 }
 
-// The same sums when the whole batch fits one chunk (B <= kSortB), for kGroupTiles consecutive
-// tiles per workgroup: a variable's counting sort is done once for all the group's tiles of that
-// variable (the tiles of one variable are consecutive), each tile's flow rows are staged with
-// 16-byte loads, and each (node, category) sum is written once.
-constexpr int kGroupTiles = 8;
+// The same sums when the whole batch fits one chunk (B <= SB), for the kGroupNodes consecutive
+// nodes of a workgroup taken as tiles of TILE nodes (TILE divides kSortTile, so a tile lies inside
+// one uniform tile of the caller's promise): a variable's counting sort is done once for all the
+// group's tiles of that variable (the tiles of one variable are consecutive), each tile's flow
+// rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
+// staging takes TILE * SB * 4 bytes of LDS (32 KiB for both instantiations).
+constexpr int kGroupNodes = 64;
 
-template <typename PF>
+template <typename PF, int TILE, int SB>
 __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pflows,
                                                             const float* __restrict__ node_flows,
                                                             const int* __restrict__ data, const int* __restrict__ vids,
@@ -289,17 +291,18 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
                                                             int64_t node_offset, int64_t B, int pf_mode) {
     __shared__ int cnt[kSortK + 1];
     __shared__ int cur[kSortK];
-    __shared__ short perm[kSortB];
-    __shared__ __attribute__((aligned(16))) float fl[kSortTile][kSortB];
+    __shared__ short perm[SB];
+    __shared__ __attribute__((aligned(16))) float fl[TILE][SB];
+    static_assert(kSortTile % TILE == 0 && kGroupNodes % TILE == 0 && SB <= 32767, "tile and batch limits");
     constexpr int KPT = kSortK / 256;
     const int tid = threadIdx.x;
-    const int nb = (int)B;                                  // B <= kSortB
-    const int64_t g0 = (int64_t)blockIdx.x * kSortTile * kGroupTiles;
+    const int nb = (int)B;                                  // B <= SB
+    const int64_t g0 = (int64_t)blockIdx.x * kGroupNodes;
     int cur_v = -1, K = 0;
-    for (int tt = 0; tt < kGroupTiles; ++tt) {
-        const int64_t i0 = g0 + (int64_t)tt * kSortTile;
+    for (int tt = 0; tt < kGroupNodes / TILE; ++tt) {
+        const int64_t i0 = g0 + (int64_t)tt * TILE;
         if (i0 >= num_nodes) break;
-        const int n_here = (int)((num_nodes - i0) < kSortTile ? (num_nodes - i0) : kSortTile);
+        const int n_here = (int)((num_nodes - i0) < TILE ? (num_nodes - i0) : TILE);
         const int v = vids[i0];
         if (v != cur_v) {
             // counting sort of the variable's categories: cnt[k] = first position of category k
@@ -348,29 +351,29 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
         for (int q = 0; q < KPT; ++q) {
             const int k = tid + q * 256;
             if (k >= K) continue;
-            float sums[kSortTile];
+            float sums[TILE];
 #pragma unroll
-            for (int r = 0; r < kSortTile; ++r) sums[r] = 0.0f;
+            for (int r = 0; r < TILE; ++r) sums[r] = 0.0f;
             const int p0 = cnt[k], p1 = cnt[k + 1];
             for (int p = p0; p < p1; ++p) {
                 const int c = perm[p];
 #pragma unroll
-                for (int r = 0; r < kSortTile; ++r) sums[r] += fl[r][c];
+                for (int r = 0; r < TILE; ++r) sums[r] += fl[r][c];
             }
             if (pf_mode == kPfOverwrite) {
 #pragma unroll
-                for (int r = 0; r < kSortTile; ++r)
+                for (int r = 0; r < TILE; ++r)
                     if (r < n_here) pflows[s_pfids[i0 + r] + k] = (PF)sums[r];
             } else if (pf_mode == kPfAtomic) {
 #pragma unroll
-                for (int r = 0; r < kSortTile; ++r)
+                for (int r = 0; r < TILE; ++r)
                     if (r < n_here && sums[r] != 0.0f) atomicAdd(&pflows[s_pfids[i0 + r] + k], (PF)sums[r]);
             } else {
-                PF old[kSortTile];
+                PF old[TILE];
 #pragma unroll
-                for (int r = 0; r < kSortTile; ++r) old[r] = (r < n_here) ? pflows[s_pfids[i0 + r] + k] : (PF)0;
+                for (int r = 0; r < TILE; ++r) old[r] = (r < n_here) ? pflows[s_pfids[i0 + r] + k] : (PF)0;
 #pragma unroll
-                for (int r = 0; r < kSortTile; ++r)
+                for (int r = 0; r < TILE; ++r)
                     if (r < n_here) pflows[s_pfids[i0 + r] + k] = old[r] + (PF)sums[r];
             }
         }
@@ -542,10 +545,15 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
     if (kind == 1) {
         bern_bwd_kernel<PF><<<(unsigned)num_nodes, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids,
                                                                     node_offset, B, pf_mode);
-    } else if (sorted_tiles && kmax <= kSortK && B <= kSortB && B % 4 == 0 &&
+    } else if (sorted_tiles && kmax <= kSortK && B <= 2048 && B % 4 == 0 &&
                ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0) && (node_offset * B) % 4 == 0) {
-        cat_bwd_group_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile * kGroupTiles), 256, 0, stream>>>(
-            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);
+        const unsigned grid = (unsigned)cdiv(num_nodes, kGroupNodes);
+        if (B <= 1024)
+            cat_bwd_group_kernel<PF, 8, 1024><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
+                                                                        num_nodes, node_offset, B, pf_mode);
+        else
+            cat_bwd_group_kernel<PF, 4, 2048><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
+                                                                        num_nodes, node_offset, B, pf_mode);
     } else if (sorted_tiles && kmax <= kSortK) {
         cat_bwd_sorted_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile), 256, 0, stream>>>(
             pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);

@@ -31,9 +31,10 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
 // block_size, n_children]; the parameter of child c of node i sits at par_start + c * block_size
 // + i (same for flows).  A workgroup of four wavefronts handles (record, up to 64 nodes): lanes
 // run along the nodes (ti = min(bs, 64) nodes, G = 64 / ti lanes per node) and the wavefronts
-// split the children.  When every lane owns at most kRegChildren children (block size >= 64 and
-// at most 256 children) the flows and parameters stay in registers between the normaliser and
-// the update, so each is read from memory once; otherwise the update re-reads them.
+// split the children.  When every lane owns at most kRegChildren children (a block size dividing
+// or a multiple of 64, and at most kRegChildren children per lane: 256 children at block size
+// >= 64, 512 at 32) the flows and parameters stay in registers between the normaliser and the
+// update, so each is read from memory once; otherwise the update re-reads them.
 constexpr int kEmWaves = 4;
 constexpr int kRegChildren = 64;
 
@@ -55,7 +56,7 @@ __global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params,
     const int stride = G * kEmWaves;
     const bool lane_ok = (lane / ti) < G;   // lanes past G * ti (when 64 % ti != 0) idle
     const double npc = pc / (double)nch;
-    const bool in_regs = (ti == 64) && (nch <= kEmWaves * kRegChildren);
+    const bool in_regs = (64 % ti == 0) && (nch <= stride * kRegChildren);
 
     double fr[kRegChildren];
     float tr[kRegChildren];

@@ -734,6 +734,156 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
     }
 }
 
+// Forward with 16-byte loads for node blocks of 32 rows (32 x 128 tiles) and for child runs
+// shorter than a stage (runs of >= 16 consecutive element rows with parameters bs apart, e.g. the
+// child blocks of 32 of a block-size-32 layer).  The block's index row (at most kFwdIdx entries)
+// is staged in LDS once; each thread reads the run start of its own rows there, so every load is
+// unconditional (row clamped to a real child, value masked afterwards) and a stage is one round
+// trip of 16-byte loads.  Requirements (checked by the launcher): 16-bit operands, B % 4 == 0,
+// 16-byte-aligned parameter rows, child runs >= 16, K <= kFwdIdx.
+//
+// Thread layout, TN = 4096 / TM columns: element rows: column quad q = tid % (TN / 4), row group
+// g of RPG = TK * TN / 1024 consecutive children; parameters: node-row quad iq = tid % (TM / 4),
+// child group kg of KPG = TM / 8 consecutive children.
+constexpr int kFwdIdx = 512;
+
+template <typename T, int TM>
+__global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ node_mars, const float* __restrict__ emars,
+                                                          const float* __restrict__ params, const int* __restrict__ nids,
+                                                          const int* __restrict__ cids, const int* __restrict__ pids,
+                                                          const int* __restrict__ nch, int K, int64_t B, int mtiles,
+                                                          int btiles) {
+    constexpr int TN = 4096 / TM, WM = TM / 32;
+    constexpr int QN = TN / 4, GR = 256 / QN, RPG = TK / GR;
+    constexpr int QM = TM / 4, KPG = TM / 8;
+    constexpr int LD = TK + MF<T>::PAD;
+    static_assert((RPG == 8 || RPG == 16) && (KPG == 4 || KPG == 8), "row groups");
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[GR][TN], mrun[TN], msc[TN];
+    __shared__ int sc[kFwdIdx], sp[kFwdIdx];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = mtiles * TM;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % WM, wn = w / WM;
+    const int n = nch[j];
+    const int q = tid % QN, g = tid / QN;
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;                          // B % 4 == 0: a quad is all in or all out
+    const int64_t bqc = bok ? bq : B - 4;
+    const int iq = tid % QM, kg = tid / QM;
+    const int mycol = wn * 32 + (lane & 31);
+    for (int u = tid; u < K; u += 256) {
+        sc[u] = cids[(int64_t)j * K + u];
+        sp[u] = pids[(int64_t)j * K + u];
+    }
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+    __syncthreads();
+
+    float4 ev[RPG], av[KPG];
+    bool eok[RPG], aok[KPG];
+    auto load = [&](int k0) {
+        // element rows k0 + RPG g + r: one run (RPG divides the run length), clamped to the last
+        // real child so that every address is a real row
+        const int ke = k0 + RPG * g;
+        const int kec = ke < n ? ke : 0;
+        const int64_t c0 = sc[kec];
+        const int last = n - 1 - kec;                 // rows of the run past this are not real
+#pragma unroll
+        for (int r = 0; r < RPG; ++r) {
+            const int rr = r < last ? r : (last > 0 ? last : 0);
+            ev[r] = *reinterpret_cast<const float4*>(emars + (c0 + rr) * B + bqc);
+            eok[r] = (ke + r < n) && bok;
+        }
+        const int ka = k0 + KPG * kg;
+        const int kac = ka < n ? ka : 0;
+        const int64_t p0 = sp[kac];
+        const int lasta = n - 1 - kac;
+#pragma unroll
+        for (int r = 0; r < KPG; ++r) {
+            const int rr = r < lasta ? r : (lasta > 0 ? lasta : 0);
+            av[r] = *reinterpret_cast<const float4*>(params + p0 + (int64_t)rr * bs + m0 + 4 * iq);
+            aok[r] = ka + r < n;
+        }
+    };
+    if (n > 0) load(0);
+    for (int k0 = 0; k0 < n; k0 += TK) {
+        float mx[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+        for (int r = 0; r < RPG; ++r) {
+            if (!eok[r]) ev[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+            mx[0] = fmaxf(mx[0], ev[r].x);
+            mx[1] = fmaxf(mx[1], ev[r].y);
+            mx[2] = fmaxf(mx[2], ev[r].z);
+            mx[3] = fmaxf(mx[3], ev[r].w);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) red[g][4 * q + c] = mx[c];
+        __syncthreads();
+        if (tid < TN) {
+            float x = red[0][tid];
+#pragma unroll
+            for (int i = 1; i < GR; ++i) x = fmaxf(x, red[i][tid]);
+            const float mo = mrun[tid];
+            const float mnew = fmaxf(mo, x);
+            msc[tid] = (mnew == kNegInf) ? 1.0f : __expf(mo - mnew);
+            mrun[tid] = mnew;
+        }
+        __syncthreads();
+        // B tile: Bs[4 q + c][RPG g + r] = exp(ev[r][c] - max[c])
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float mn = mrun[4 * q + c];
+#pragma unroll
+            for (int h = 0; h < RPG / 8; ++h) {
+                float e[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const float4 v = ev[8 * h + r];
+                    const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+                    e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                }
+                Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
+            }
+        }
+        // A tile: As[4 iq + c][KPG kg + r] = params of node row m0 + 4 iq + c, child KPG kg + r
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            typedef T TV __attribute__((ext_vector_type(KPG)));
+            TV x;
+#pragma unroll
+            for (int r = 0; r < KPG; ++r) {
+                const float4 v = av[r];
+                const float y = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+                x[r] = MF<T>::cvt(aok[r] ? y : 0.0f);
+            }
+            *reinterpret_cast<TV*>(&As[4 * iq + c][KPG * kg]) = x;
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < n) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    const int64_t b = b0 + mycol;
+    if (b < B) {
+        const float m = mrun[mycol];
+        const int64_t base = (int64_t)nids[j] + m0 + wm * 32;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float v = acc[r];
+            node_mars[(base + acc_row(r, lane)) * B + b] = (v > 0.0f && m != kNegInf) ? __logf(v) + m : kNegInf;
+        }
+    }
+}
+
 // Element flows with 16-byte loads for the shapes sum_ele_vec does not take: child blocks of 32
 // rows (32 x 128 tiles) and parent blocks smaller than a stage (a stage of TK flattened parent
 // rows then spans several parent blocks).  The parent-block indices of the child block (at most
@@ -1186,6 +1336,17 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     const bool run = child_run >= TK && child_run % TK == 0;
     bool vec = false;
     if constexpr (sizeof(T) == 2) {
+        // node blocks of 32 rows, or child runs shorter than a stage: block-spanning form
+        if (aligned && child_run >= 16 && B % 4 == 0 && K <= kFwdIdx && (bs == 32 || (bs >= 64 && !run))) {
+            if (bs == 32)
+                sum_fwd_vec_blocks<T, 32><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids,
+                                                                               pids, nch, K, B, mtiles, (int)btiles);
+            else
+                sum_fwd_vec_blocks<T, 64><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids,
+                                                                               pids, nch, K, B, mtiles, (int)btiles);
+            PJAMD_LAUNCH_CHECK();
+            return PJAMD_OK;
+        }
         if (run && aligned && bs >= 64 && B % 4 == 0) {
             vec = true;
             if (tiles < 256)
