@@ -9,45 +9,77 @@ namespace {
 constexpr int kLdsFloats = 4096; // largest category count with LDS flow histograms
 
 // node_mars[(off + i) * B + b] = log p_i(x[v_i, b]); out-of-range categories have p = 0.
-// kRowThreads threads per node row, 4 consecutive batch columns each (16-byte loads of the
-// categories, 16-byte stores); the parameter gathers hit the node's row (kmax floats) in L1 / L2.
+// kRowThreads threads per group of NT consecutive node rows, 4 consecutive batch columns each
+// (16-byte loads of the categories, 16-byte stores).  A thread issues the category loads of all NT
+// nodes, then all 4 NT parameter gathers (they hit each node's row of kmax floats in L1 / L2),
+// so a workgroup waits for two memory round trips per NT rows instead of per row.
 constexpr int kRowThreads = 128;
+constexpr int kFwdNodes = 4;
 
-template <int KIND>
+template <int KIND, int NT>
 __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict__ params, float* __restrict__ node_mars,
                                                         const int* __restrict__ data, const int* __restrict__ vids,
                                                         const int64_t* __restrict__ s_pids,
                                                         const int* __restrict__ ncats, int64_t num_nodes,
                                                         int64_t node_offset, int64_t B) {
-    const int64_t i = (int64_t)blockIdx.x * (256 / kRowThreads) + threadIdx.x / kRowThreads;
-    if (i >= num_nodes) return;
+    const int64_t i0 = ((int64_t)blockIdx.x * (256 / kRowThreads) + threadIdx.x / kRowThreads) * NT;
+    if (i0 >= num_nodes) return;
+    const int nn = (num_nodes - i0) < NT ? (int)(num_nodes - i0) : NT;
     const int sub = threadIdx.x % kRowThreads;
-    const int* xrow = data + (int64_t)vids[i] * B;
-    float* out = node_mars + (node_offset + i) * B;
-    const float* th = params + s_pids[i];
-    const int K = (KIND == 0) ? ncats[i] : 2;
-    const float p1 = (KIND == 1) ? th[0] : 0.0f;
+    const int* xrow[NT];
+    const float* th[NT];
+    int K[NT];
+    float p1[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int64_t i = i0 + (n < nn ? n : 0);    // rows past the layer repeat row i0, never stored
+        xrow[n] = data + (int64_t)vids[i] * B;
+        th[n] = params + s_pids[i];
+        K[n] = (KIND == 0) ? ncats[i] : 2;
+        p1[n] = (KIND == 1) ? th[n][0] : 0.0f;
+    }
     for (int64_t b = (int64_t)sub * 4; b < B; b += kRowThreads * 4) {
         if (b + 4 <= B) {
-            const int4 x = *reinterpret_cast<const int4*>(xrow + b);
-            float4 o;
-            if (KIND == 0) {
-                o.x = (x.x >= 0 && x.x < K) ? __logf(th[x.x]) : kNegInf;
-                o.y = (x.y >= 0 && x.y < K) ? __logf(th[x.y]) : kNegInf;
-                o.z = (x.z >= 0 && x.z < K) ? __logf(th[x.z]) : kNegInf;
-                o.w = (x.w >= 0 && x.w < K) ? __logf(th[x.w]) : kNegInf;
-            } else {
-                o.x = __logf(x.x > 0 ? p1 : 1.0f - p1);
-                o.y = __logf(x.y > 0 ? p1 : 1.0f - p1);
-                o.z = __logf(x.z > 0 ? p1 : 1.0f - p1);
-                o.w = __logf(x.w > 0 ? p1 : 1.0f - p1);
+            int4 x[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) x[n] = *reinterpret_cast<const int4*>(xrow[n] + b);
+            float4 o[NT];
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                if (KIND == 0) {
+                    const int k = K[n];
+                    const int c0 = (x[n].x >= 0 && x[n].x < k) ? x[n].x : 0;
+                    const int c1 = (x[n].y >= 0 && x[n].y < k) ? x[n].y : 0;
+                    const int c2 = (x[n].z >= 0 && x[n].z < k) ? x[n].z : 0;
+                    const int c3 = (x[n].w >= 0 && x[n].w < k) ? x[n].w : 0;
+                    o[n] = make_float4(th[n][c0], th[n][c1], th[n][c2], th[n][c3]);
+                }
             }
-            *reinterpret_cast<float4*>(out + b) = o;
+#pragma unroll
+            for (int n = 0; n < NT; ++n) {
+                float4 r;
+                if (KIND == 0) {
+                    const int k = K[n];
+                    r.x = (x[n].x >= 0 && x[n].x < k) ? __logf(o[n].x) : kNegInf;
+                    r.y = (x[n].y >= 0 && x[n].y < k) ? __logf(o[n].y) : kNegInf;
+                    r.z = (x[n].z >= 0 && x[n].z < k) ? __logf(o[n].z) : kNegInf;
+                    r.w = (x[n].w >= 0 && x[n].w < k) ? __logf(o[n].w) : kNegInf;
+                } else {
+                    r.x = __logf(x[n].x > 0 ? p1[n] : 1.0f - p1[n]);
+                    r.y = __logf(x[n].y > 0 ? p1[n] : 1.0f - p1[n]);
+                    r.z = __logf(x[n].z > 0 ? p1[n] : 1.0f - p1[n]);
+                    r.w = __logf(x[n].w > 0 ? p1[n] : 1.0f - p1[n]);
+                }
+                if (n < nn) *reinterpret_cast<float4*>(node_mars + (node_offset + i0 + n) * B + b) = r;
+            }
         } else {
-            for (int64_t c = b; c < B; ++c) {
-                const int xx = xrow[c];
-                if (KIND == 0) out[c] = (xx >= 0 && xx < K) ? __logf(th[xx]) : kNegInf;
-                else out[c] = __logf(xx > 0 ? p1 : 1.0f - p1);
+            for (int n = 0; n < nn; ++n) {
+                float* out = node_mars + (node_offset + i0 + n) * B;
+                for (int64_t c = b; c < B; ++c) {
+                    const int xx = xrow[n][c];
+                    if (KIND == 0) out[c] = (xx >= 0 && xx < K[n]) ? __logf(th[n][xx]) : kNegInf;
+                    else out[c] = __logf(xx > 0 ? p1[n] : 1.0f - p1[n]);
+                }
             }
         }
     }
@@ -526,14 +558,14 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
         PJAMD_LAUNCH_CHECK();
         return PJAMD_OK;
     }
-    const int64_t blocks = cdiv(num_nodes, 256 / kRowThreads);
+    const int64_t blocks = cdiv(num_nodes, (int64_t)(256 / kRowThreads) * kFwdNodes);
     PJAMD_REQUIRE(blocks < ((int64_t)1 << 31));
     if (kind == 0)
-        input_fwd_kernel<0><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats,
-                                                                  num_nodes, node_offset, B);
+        input_fwd_kernel<0, kFwdNodes><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids,
+                                                                             ncats, num_nodes, node_offset, B);
     else
-        input_fwd_kernel<1><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats,
-                                                                  num_nodes, node_offset, B);
+        input_fwd_kernel<1, kFwdNodes><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids,
+                                                                             ncats, num_nodes, node_offset, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
