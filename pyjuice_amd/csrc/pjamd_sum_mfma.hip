@@ -44,7 +44,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 }
 
 constexpr int KCAP = 1024;  // child / parent index entries staged in LDS per chunk
-constexpr int kPreChildren = 4;  // child blocks per product block the FUSE = 2 kernels prefetch
+constexpr int kPreChildren = 6;  // child blocks per product block the FUSE = 2 kernels prefetch
 
 // ---------------------------------------------------------------------------------------------
 // Forward.  Tile = (node block j, TM parent rows) x TN batch columns; logical tile order is
@@ -599,7 +599,8 @@ __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars
     }
 }
 
-// Element flows with 16-byte loads (parent blocks of >= TK rows).  FUSE as sum_ele_mfma (0 or 1);
+// Element flows with 16-byte loads (parent blocks of >= TK rows).  FUSE as sum_ele_mfma (0, 1, or
+// 2: child codes and the marked children's marginals fetched at kernel start, used with PRE);
 // PRE as sum_fwd_vec.
 template <typename T, int FUSE, bool PRE>
 __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, const float* nflows,
@@ -664,6 +665,21 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
     float epre[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
+    int cpre[FUSE == 2 ? kPreChildren : 1];
+    float mpre[FUSE == 2 ? kPreChildren : 1][16];
+    if (FUSE == 2) {
+        const int* crow = ch_nodes + (int64_t)j * Kc;
+        const int64_t bc = b < B ? b : B - 1;   // clamped column: unconditional loads
+#pragma unroll
+        for (int u = 0; u < kPreChildren; ++u) {
+            cpre[u] = u < Kc ? crow[u] : 0;
+            if ((cpre[u] & 1) && mod_children) {
+                const int64_t cb = (int64_t)(cpre[u] >> 1) + m0 + wm * 32;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) mpre[u][r] = nmars[(cb + acc_row(r, lane)) * B + bc];
+            }
+        }
+    }
     for (int k0 = 0; k0 < kf; k0 += TK) {
         float mn[4];
         vec_stage_max(lv, red, mrun, msc, g, q, mn);
@@ -705,7 +721,22 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
             const float e = epre[r];
             v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
         }
-        if (FUSE) {
+        if (FUSE == 2) {
+#pragma unroll
+            for (int u = 0; u < kPreChildren; ++u) {
+                if (cpre[u] == 0) continue;
+                const int64_t cbase = (int64_t)(cpre[u] >> 1) + m0 + wm * 32;
+                if ((cpre[u] & 1) && mod_children) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                            (mpre[u][r] == kNegInf) ? kNegInf : __logf(v[r]) - mpre[u][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+                }
+            }
+        } else if (FUSE) {
             const int* crow = ch_nodes + (int64_t)j * Kc;
             for (int k = 0; k < Kc; ++k) {
                 const int code = crow[k];
@@ -1424,7 +1455,8 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
                                                   mod_children)
             const bool pre = g < 256;
             if (ch_nodes != nullptr) {
-                if (pre) PJAMD_ELE_VEC(1, true);
+                if (pre && Kc <= kPreChildren) PJAMD_ELE_VEC(2, true);
+                else if (pre) PJAMD_ELE_VEC(1, true);
                 else PJAMD_ELE_VEC(1, false);
             } else {
                 if (pre) PJAMD_ELE_VEC(0, true);
