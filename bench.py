@@ -53,7 +53,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--structure", default="hclt", choices=["hclt", "pd", "hmm"],
+                    help="hclt: HCLT-784 (the default workload); pd: PD over the same 28x28 images "
+                         "(split intervals 4 x 4, sum-dominated); hmm: homogeneous HMM over seeded token "
+                         "sequences (--latents hidden states, --vocab tokens, --seq-length)")
     ap.add_argument("--latents", type=int, default=256)
+    ap.add_argument("--vocab", type=int, default=4096, help="HMM vocabulary")
+    ap.add_argument("--seq-length", type=int, default=64, help="HMM sequence length")
     ap.add_argument("--block-size", type=int, default=-1,
                     help="HCLT block size: -1 = measured choice among num_latents / 2^k >= 64 (pyjuice_amd.autotune), "
                          "0 = the builder's default (num_latents)")
@@ -83,10 +89,31 @@ def build_circuit(args, dev):
     from pyjuice_amd import synthetic
 
     t0 = time.time()
-    data = synthetic.image_categorical(args.fit + args.batch, H=28, W=28, K=256, rho=0.6, seed=0, field_seed=1)
+    if args.structure != "hclt":
+        args.fit = 0      # only the HCLT learns its tree from data
+    if args.structure == "hmm":
+        data = synthetic.hmm_sequences(args.fit + args.batch, args.seq_length, args.latents, args.vocab, alpha=0.1,
+                                       seed=2)
+    else:
+        data = synthetic.image_categorical(args.fit + args.batch, H=28, W=28, K=256, rho=0.6, seed=0, field_seed=1)
     log(f"[bench] data {data.shape} in {time.time() - t0:.1f}s")
     xb = torch.from_numpy(data[args.fit:args.fit + args.batch]).to(dev)
     x_fit = torch.from_numpy(data[:args.fit]).float().to(dev)
+    if args.structure != "hclt":
+        torch.manual_seed(0)
+        if args.structure == "pd":
+            ns = juice.structures.PD(data_shape=(28, 28), num_latents=args.latents, split_intervals=(4, 4),
+                                     structure_type="sum_dominated", input_dist=juice.distributions.Categorical(256))
+        else:
+            ns = juice.structures.HMM(seq_length=args.seq_length, num_latents=args.latents, num_emits=args.vocab,
+                                      homogeneous=True)
+        args.block_size, args.block_size_ms = 0, None
+        t0 = time.time()
+        pc = juice.compile(ns, verbose=False)
+        pc.use_chains = bool(args.chains)
+        pc.to(dev)
+        log(f"[bench] compiled {args.structure}: {pc.num_nodes} nodes, {pc.num_edges} edges in {time.time() - t0:.1f}s")
+        return ns, pc, xb, data[args.fit:args.fit + args.batch]
 
     def make_nodes(bs):
         torch.manual_seed(0)
@@ -176,8 +203,8 @@ def roofline_of(name, d, prec, args):
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
-    headline = (args.latents == 256 and args.batch == 512 and args.block_size == 256 and prec == "bf16"
-                and args.prune == 0 and not args.chains)
+    headline = (args.structure == "hclt" and args.latents == 256 and args.batch == 512 and args.block_size == 256
+                and prec == "bf16" and args.prune == 0 and not args.chains)
     traffic = TRAFFIC_MB_PER_LAUNCH.get(name) if headline else None
     traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
@@ -233,6 +260,19 @@ def cpu_baseline(pc, x_np, lls_by_prec, args):
     return {"value": round(n * steps / t_tot, 2), "unit": "samples/s", "cores": _host_threads(), "kind": "port",
             "sample": f"{steps} step(s) of {n} samples of the bench batch: FP32 C/OpenMP forward + flow backward "
                       f"+ EM on the same compiled circuit ({t_tot:.1f} s)"}, acc
+
+
+def workload_name(args):
+    step = "forward + flow backward + mini-batch EM"
+    if args.structure == "pd":
+        return (f"PD over 28x28 images (256 categories, split intervals 4x4, sum-dominated), "
+                f"num_latents={args.latents}, batch={args.batch}; {step}")
+    if args.structure == "hmm":
+        return (f"homogeneous HMM, {args.latents} hidden states, {args.vocab} tokens, length {args.seq_length}, "
+                f"batch={args.batch}; {step}")
+    return (f"HCLT-784 (28x28, 256 categories), num_latents={args.latents}, block_size={args.block_size or 'default'}, "
+            + (f"{args.prune:.0%} of the sum-edge blocks pruned, " if args.prune > 0 else "")
+            + f"batch={args.batch}; {step}")
 
 
 def main():
@@ -315,10 +355,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp32": "f32", "fp32_valu": "f32", "bf16": "bf16", "fp16": "f16"}[main_prec],
         "data": "synthetic",
-        "config": {"workload": f"HCLT-784 (28x28, 256 categories), num_latents={args.latents}, "
-                               f"block_size={args.block_size or 'default'}, "
-                               + (f"{args.prune:.0%} of the sum-edge blocks pruned, " if args.prune > 0 else "")
-                               + f"batch={args.batch}; forward + flow backward + mini-batch EM",
+        "config": {"workload": workload_name(args),
                    "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
                    "precision": main_prec, "hip_graph": bool(args.graph), "chain_kernels": bool(args.chains),
                    "block_size_ms_per_step": args.block_size_ms},

@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
 // update, so each is read from memory once; otherwise the update re-reads them.
 constexpr int kEmWaves = 4;
 constexpr int kRegChildren = 64;
+constexpr int kEmChunk = 16;      // children per lane per round trip when they do not fit registers
 
 template <typename PF>
 __global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
@@ -75,10 +76,20 @@ __global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params,
             if (c < nch) cum += keep_zero ? ((tr[q] < 1e-12f) ? 0.0 : fr[q] + npc) : fr[q];
         }
     } else if (lane_ok) {
-        for (int c = g; c < nch; c += stride) {
-            const int64_t off = (int64_t)c * bs + i;
-            const double f = (double)pflows[pf0 + off];
-            cum += keep_zero ? ((params[par0 + off] < 1e-12f) ? 0.0 : f + npc) : f;
+        // kEmChunk children per lane per round trip (clamped to a real child, masked on use)
+        for (int c0 = g; c0 < nch; c0 += kEmChunk * stride) {
+            double f[kEmChunk];
+            float t[kEmChunk];
+#pragma unroll
+            for (int q = 0; q < kEmChunk; ++q) {
+                const int c = c0 + q * stride;
+                const int64_t off = (int64_t)(c < nch ? c : c0) * bs + i;
+                f[q] = (double)pflows[pf0 + off];
+                t[q] = params[par0 + off];
+            }
+#pragma unroll
+            for (int q = 0; q < kEmChunk; ++q)
+                if (c0 + q * stride < nch) cum += keep_zero ? ((t[q] < 1e-12f) ? 0.0 : f[q] + npc) : f[q];
         }
     }
     // reduce over the G lane groups of a wavefront, then over the wavefronts
@@ -101,12 +112,26 @@ __global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params,
             }
         }
     } else if (lane_ok) {
-        for (int c = g; c < nch; c += stride) {
-            const int64_t off = (int64_t)c * bs + i;
-            const double th = (double)params[par0 + off];
-            double upd = (1.0 - step) * th + step * (((double)pflows[pf0 + off] + npc) / denom);
-            if (keep_zero && th < 1e-12) upd = 0.0;
-            params[par0 + off] = (float)upd;
+        for (int c0 = g; c0 < nch; c0 += kEmChunk * stride) {
+            double f[kEmChunk];
+            float t[kEmChunk];
+#pragma unroll
+            for (int q = 0; q < kEmChunk; ++q) {
+                const int c = c0 + q * stride;
+                const int64_t off = (int64_t)(c < nch ? c : c0) * bs + i;
+                f[q] = (double)pflows[pf0 + off];
+                t[q] = params[par0 + off];
+            }
+#pragma unroll
+            for (int q = 0; q < kEmChunk; ++q) {
+                const int c = c0 + q * stride;
+                if (c < nch) {
+                    const double th = (double)t[q];
+                    double upd = (1.0 - step) * th + step * ((f[q] + npc) / denom);
+                    if (keep_zero && th < 1e-12) upd = 0.0;
+                    params[par0 + (int64_t)c * bs + i] = (float)upd;
+                }
+            }
         }
     }
 }

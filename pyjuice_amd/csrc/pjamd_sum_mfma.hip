@@ -767,11 +767,11 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
 
 // Forward with 16-byte loads for node blocks of 32 rows (32 x 128 tiles) and for child runs
 // shorter than a stage (runs of >= 16 consecutive element rows with parameters bs apart, e.g. the
-// child blocks of 32 of a block-size-32 layer).  The block's index row (at most kFwdIdx entries)
-// is staged in LDS once; each thread reads the run start of its own rows there, so every load is
+// child blocks of 32 of a block-size-32 layer).  The block's index row is staged in LDS (in chunks
+// of kFwdIdx entries); each thread reads the run start of its own rows there, so every load is
 // unconditional (row clamped to a real child, value masked afterwards) and a stage is one round
 // trip of 16-byte loads.  Requirements (checked by the launcher): 16-bit operands, B % 4 == 0,
-// 16-byte-aligned parameter rows, child runs >= 16, K <= kFwdIdx.
+// 16-byte-aligned parameter rows, child runs >= 16.
 //
 // Thread layout, TN = 4096 / TM columns: element rows: column quad q = tid % (TN / 4), row group
 // g of RPG = TK * TN / 1024 consecutive children; parameters: node-row quad iq = tid % (TM / 4),
@@ -808,100 +808,109 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ no
     const int64_t bqc = bok ? bq : B - 4;
     const int iq = tid % QM, kg = tid / QM;
     const int mycol = wn * 32 + (lane & 31);
-    for (int u = tid; u < K; u += 256) {
-        sc[u] = cids[(int64_t)j * K + u];
-        sp[u] = pids[(int64_t)j * K + u];
-    }
     if (tid < TN) mrun[tid] = kNegInf;
     floatx16 acc = {0};
-    __syncthreads();
 
     float4 ev[RPG], av[KPG];
     bool eok[RPG], aok[KPG];
+    // the children of this chunk are [cb, cb + cn); k below counts from cb
+    int cn = 0;
     auto load = [&](int k0) {
         // element rows k0 + RPG g + r: one run (RPG divides the run length), clamped to the last
         // real child so that every address is a real row
         const int ke = k0 + RPG * g;
-        const int kec = ke < n ? ke : 0;
+        const int kec = ke < cn ? ke : 0;
         const int64_t c0 = sc[kec];
-        const int last = n - 1 - kec;                 // rows of the run past this are not real
+        const int last = cn - 1 - kec;                // rows of the run past this are not real
 #pragma unroll
         for (int r = 0; r < RPG; ++r) {
             const int rr = r < last ? r : (last > 0 ? last : 0);
             ev[r] = *reinterpret_cast<const float4*>(emars + (c0 + rr) * B + bqc);
-            eok[r] = (ke + r < n) && bok;
+            eok[r] = (ke + r < cn) && bok;
         }
         const int ka = k0 + KPG * kg;
-        const int kac = ka < n ? ka : 0;
+        const int kac = ka < cn ? ka : 0;
         const int64_t p0 = sp[kac];
-        const int lasta = n - 1 - kac;
+        const int lasta = cn - 1 - kac;
 #pragma unroll
         for (int r = 0; r < KPG; ++r) {
             const int rr = r < lasta ? r : (lasta > 0 ? lasta : 0);
             av[r] = *reinterpret_cast<const float4*>(params + p0 + (int64_t)rr * bs + m0 + 4 * iq);
-            aok[r] = ka + r < n;
+            aok[r] = ka + r < cn;
         }
     };
-    if (n > 0) load(0);
-    for (int k0 = 0; k0 < n; k0 += TK) {
-        float mx[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
-#pragma unroll
-        for (int r = 0; r < RPG; ++r) {
-            if (!eok[r]) ev[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
-            mx[0] = fmaxf(mx[0], ev[r].x);
-            mx[1] = fmaxf(mx[1], ev[r].y);
-            mx[2] = fmaxf(mx[2], ev[r].z);
-            mx[3] = fmaxf(mx[3], ev[r].w);
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) red[g][4 * q + c] = mx[c];
+    // index rows longer than kFwdIdx are staged in chunks of kFwdIdx (a multiple of TK, so runs
+    // never straddle chunks); the running max and the accumulator carry over
+    for (int cb = 0; cb < n; cb += kFwdIdx) {
+        cn = (n - cb < kFwdIdx) ? n - cb : kFwdIdx;
+        const int ck = (K - cb < kFwdIdx) ? K - cb : kFwdIdx;
         __syncthreads();
-        if (tid < TN) {
-            float x = red[0][tid];
-#pragma unroll
-            for (int i = 1; i < GR; ++i) x = fmaxf(x, red[i][tid]);
-            const float mo = mrun[tid];
-            const float mnew = fmaxf(mo, x);
-            msc[tid] = (mnew == kNegInf) ? 1.0f : __expf(mo - mnew);
-            mrun[tid] = mnew;
+        for (int u = tid; u < ck; u += 256) {
+            sc[u] = cids[(int64_t)j * K + cb + u];
+            sp[u] = pids[(int64_t)j * K + cb + u];
         }
         __syncthreads();
-        // B tile: Bs[4 q + c][RPG g + r] = exp(ev[r][c] - max[c])
+        load(0);
+        for (int k0 = 0; k0 < cn; k0 += TK) {
+            float mx[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float mn = mrun[4 * q + c];
+            for (int r = 0; r < RPG; ++r) {
+                if (!eok[r]) ev[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+                mx[0] = fmaxf(mx[0], ev[r].x);
+                mx[1] = fmaxf(mx[1], ev[r].y);
+                mx[2] = fmaxf(mx[2], ev[r].z);
+                mx[3] = fmaxf(mx[3], ev[r].w);
+            }
 #pragma unroll
-            for (int h = 0; h < RPG / 8; ++h) {
-                float e[8];
+            for (int c = 0; c < 4; ++c) red[g][4 * q + c] = mx[c];
+            __syncthreads();
+            if (tid < TN) {
+                float x = red[0][tid];
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const float4 v = ev[8 * h + r];
-                    const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
-                    e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                for (int i = 1; i < GR; ++i) x = fmaxf(x, red[i][tid]);
+                const float mo = mrun[tid];
+                const float mnew = fmaxf(mo, x);
+                msc[tid] = (mnew == kNegInf) ? 1.0f : __expf(mo - mnew);
+                mrun[tid] = mnew;
+            }
+            __syncthreads();
+            // B tile: Bs[4 q + c][RPG g + r] = exp(ev[r][c] - max[c])
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float mn = mrun[4 * q + c];
+#pragma unroll
+                for (int h = 0; h < RPG / 8; ++h) {
+                    float e[8];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        const float4 v = ev[8 * h + r];
+                        const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+                        e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                    }
+                    Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
                 }
-                Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
             }
-        }
-        // A tile: As[4 iq + c][KPG kg + r] = params of node row m0 + 4 iq + c, child KPG kg + r
+            // A tile: As[4 iq + c][KPG kg + r] = params of node row m0 + 4 iq + c, child KPG kg + r
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            typedef T TV __attribute__((ext_vector_type(KPG)));
-            TV x;
+            for (int c = 0; c < 4; ++c) {
+                typedef T TV __attribute__((ext_vector_type(KPG)));
+                TV x;
 #pragma unroll
-            for (int r = 0; r < KPG; ++r) {
-                const float4 v = av[r];
-                const float y = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
-                x[r] = MF<T>::cvt(aok[r] ? y : 0.0f);
+                for (int r = 0; r < KPG; ++r) {
+                    const float4 v = av[r];
+                    const float y = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
+                    x[r] = MF<T>::cvt(aok[r] ? y : 0.0f);
+                }
+                *reinterpret_cast<TV*>(&As[4 * iq + c][KPG * kg]) = x;
             }
-            *reinterpret_cast<TV*>(&As[4 * iq + c][KPG * kg]) = x;
-        }
-        const float s = msc[mycol];
+            const float s = msc[mycol];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] *= s;
-        __syncthreads();
-        if (k0 + TK < n) load(k0 + TK);
-        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
-        __syncthreads();
+            for (int r = 0; r < 16; ++r) acc[r] *= s;
+            __syncthreads();
+            if (k0 + TK < cn) load(k0 + TK);
+            acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+            __syncthreads();
+        }
     }
     const int64_t b = b0 + mycol;
     if (b < B) {
@@ -1368,7 +1377,7 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     bool vec = false;
     if constexpr (sizeof(T) == 2) {
         // node blocks of 32 rows, or child runs shorter than a stage: block-spanning form
-        if (aligned && child_run >= 16 && B % 4 == 0 && K <= kFwdIdx && (bs == 32 || (bs >= 64 && !run))) {
+        if (aligned && child_run >= 16 && B % 4 == 0 && (bs == 32 || (bs >= 64 && !run))) {
             if (bs == 32)
                 sum_fwd_vec_blocks<T, 32><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids,
                                                                                pids, nch, K, B, mtiles, (int)btiles);

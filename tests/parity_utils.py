@@ -41,6 +41,21 @@ def hclt_pruned(num_vars=10, K=8, L=128, block_size=32, frac=0.5, seed=0):
     return ns, x, cards
 
 
+def wide_sums(block_size=64, seed=0, K=5):
+    """A sum node block with more children than the EM kernel keeps in registers (320 children at
+    block size 64, 640 at block size 32) under a block-size-1 root."""
+    from pyjuice_amd.nodes import inputs, multiply, summate
+    torch.manual_seed(seed)
+    nb = 5 if block_size == 64 else 20
+    with juice.set_block_size(block_size):
+        a = [inputs(v, num_node_blocks=nb, dist=dists.Categorical(num_cats=K)) for v in range(2)]
+        s = summate(multiply(a[0], a[1]), num_node_blocks=1)
+        top = multiply(s)
+    root = summate(top, num_node_blocks=1, block_size=1)
+    root.init_parameters()
+    return root, [K] * 2
+
+
 def hclt_bernoulli(num_vars=16, L=8, n=512, seed=0):
     """The small Bernoulli HCLT of the CPU-scale workload: chain of flips with p ~ U(0.1, 0.4)."""
     g = np.random.Generator(np.random.PCG64(seed))

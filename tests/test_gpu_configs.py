@@ -7,7 +7,8 @@ circuit and parameters, or against properties that hold at any size.
 2. HCLT-784 (256 categories), 128 latents, block size 32, FP32: |dLL| <= 1e-4 nats/var, parameter
    flows and EM against the oracle.
 3. PD 28x28, 256 latents, block size 32, BF16 and FP16: |dLL| <= 1e-3 nats/var; the FP64
-   parameter-flow accumulator over several mini-batches equals the sum of its parts.
+   parameter-flow accumulator over 60 mini-batches of 1024 equals the FP64 sum of its parts,
+   and a mini-batch's accumulated parameter flows match the oracle's.
 4. HMM, 1024 hidden states, 4096 tokens, length 64, tied: |dLL| <= 1e-4 nats/var (FP32); the
    autograd adjoint, with layer-wise recomputation of the element marginals, against oracle
    parameter flows and against exact finite differences.
@@ -133,27 +134,35 @@ def test_config3_pd_28x28_L256(precision, device):
                              structure_type="sum_dominated", input_dist=juice.distributions.Categorical(256))
     pc = juice.compile(ns)
     pc.to(device)
-    x = _images(1024, seed=7).astype(np.int64)
+    n_batches, bsz = 60, 1024
+    x = _images(n_batches * bsz, seed=7).astype(np.int64)
     xt = torch.from_numpy(x).to(device)
-    lls = pc(xt, precision=precision).detach().cpu().numpy()[:, 0]
+    lls = pc(xt[:bsz], precision=precision).detach().cpu().numpy()[:, 0]
     cpu = CompiledCPU(pc, np.float64)
     n = 4
     ref = cpu.forward(x[:n])[0]
     assert np.abs(lls[:n] - ref).max() / 784 <= 1e-3
-    # FP64 accumulator over several mini-batches = sum of the single-batch flows
-    parts = []
-    for i in range(3):
-        xb = xt[i * 256:(i + 1) * 256]
+    # FP64 accumulator over 60 mini-batches of 1024 (61,440 samples) = the FP64 sum of the
+    # single-batch flows (each batch's flows stored, then added on the device in the same order)
+    want = None
+    for i in range(n_batches):
+        xb = xt[i * bsz:(i + 1) * bsz]
         pc(xb, precision=precision)
         pc.backward(xb, flows_memory=0.0, precision=precision)
-        parts.append(pc.param_flows.detach().clone())
+        want = pc.param_flows.clone() if want is None else want + pc.param_flows
     pc.init_param_flows(0.0)
-    for i in range(3):
-        xb = xt[i * 256:(i + 1) * 256]
+    for i in range(n_batches):
+        xb = xt[i * bsz:(i + 1) * bsz]
         pc(xb, precision=precision)
         pc.backward(xb, flows_memory=1.0, precision=precision)
     assert pc.param_flows.dtype == torch.float64
-    assert U.rel_err(pc.param_flows.cpu().numpy(), sum(p.cpu().numpy() for p in parts)) <= 1e-9
+    assert U.rel_err(pc.param_flows.cpu().numpy(), want.cpu().numpy()) <= 1e-12
+    # and the accumulated flows of the first mini-batch's samples match the FP64 oracle's
+    pc.init_param_flows(0.0)
+    pc(xt[:n], precision=precision)
+    pc.backward(xt[:n], flows_memory=1.0, precision=precision)
+    cpu.backward()
+    assert _sum_pflows_close(pc, cpu, 2e-2) <= 2e-2
 
 
 def test_config4_hmm_adjoint(device):

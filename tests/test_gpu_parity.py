@@ -46,6 +46,9 @@ def _circuit(name):
     if name == "pd_sparse":
         ns, cards = U.pd_sparse()
         return ns, cards, dict()
+    if name.startswith("wide_sums"):   # sum EM beyond the register-resident child count
+        ns, cards = U.wide_sums(block_size=int(name[len("wide_sums"):]))
+        return ns, cards, dict()
     if name == "hclt_pruned":      # several backward partitions in two sum layers
         ns, x, cards = U.hclt_pruned(num_vars=10, L=128)
         return ns, cards, dict()
@@ -56,7 +59,8 @@ def _circuit(name):
 
 
 CIRCUITS = ["simple_model", "tying_block1", "hclt_bern16", "hclt_cat_bs32", "hclt_cat_bs64", "hclt_cat_bs128",
-            "pd_small", "pd_sparse", "hmm_small", "hclt_pruned", "hclt_pruned_fw"]
+            "pd_small", "pd_sparse", "hmm_small", "hclt_pruned", "hclt_pruned_fw",
+            "wide_sums64", "wide_sums32"]
 
 # (LL nats/var, node-mar tol per scope variable, node-mar offset, relative flow / parameter-flow
 # tol, EM abs tol).  A single BF16 sum node can be off by ~2^-8 (operand rounding), hence the
