@@ -154,6 +154,23 @@ int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* 
                       const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
                       int child_run, int precision, pjamd_stream_t stream);
 
+/* pjamd_sum_forward with the product layer below it fused in (BF16 / FP16 operands: precision 1
+ * or 2).  The partition's element rows are not read from element_mars but summed from the node
+ * rows of the products' children, and stored into element_mars on the way (the flow backward
+ * reads them).  prod_bases is int32 [n_blocks][ceil(K / 128)][prod_k], prod_k in {2, 4, 8}: for
+ * node block j and its stage s of 128 children (element rows cids[j][128 s] + 0..127, a run
+ * inside one product block), the first node row of each child block of that product block plus
+ * the offset of the stage's first row in the product block; unused entries point into the
+ * dummy node rows [0, max element block size), which hold 0.  Requires child_run with
+ * PJAMD_PARAMS_ALIGNED and a run of >= 128, bs a multiple of 64 and B % 4 == 0, else returns
+ * PJAMD_EUNSUPPORTED (then call pjamd_prod_forward and pjamd_sum_forward).  Replaces
+ * ProdLayer.forward (src/pyjuice/layer/prod_layer.py:160-204) followed by SumLayer.forward
+ * (sum_layer.py:220-262) for two consecutive layer groups of TensorCircuit.forward
+ * (tensorcircuit.py:269-283). */
+int pjamd_sum_forward_prod(float* node_mars, float* element_mars, const float* params, const int* nids,
+                           const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
+                           int child_run, const int* prod_bases, int prod_k, int precision, pjamd_stream_t stream);
+
 /* node_flows[r][b] <- log(node_flows[r][b]) - node_mars[r][b] for the rows of the partition's
  * node blocks (-inf where the marginal is -inf).  Replaces SumLayer._bk_triton_modify_flow
  * (sum_layer.py:1355-1515). */

@@ -46,6 +46,8 @@ _SIGNATURES = {
     "pjamd_prod_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_prod_backward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_sum_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
+    "pjamd_sum_forward_prod": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,
+                               _i32, _c_p],
     "pjamd_sum_modify_flows": [_c_p, _c_p, _c_p, _i32, _i32, _i64, _c_p],
     "pjamd_sum_backward_ele": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
                                _i32, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],

@@ -506,12 +506,20 @@ __device__ __forceinline__ void vec_stage_b(const float4 (&v)[kVecRows], const f
 
 // Forward with 16-byte loads (RUN layout).  PRE: the first two stages' loads are issued together
 // (grids smaller than the chip).
-template <typename T, bool PRE>
+//
+// KP > 0 (product layer fused in): the element rows are not read from emars but computed, as the
+// product layer below would, from the node rows of the products' children: stage s of node block j
+// covers element rows inside one product block, and pch[(j * nstages + s) * KP + u] is the first
+// node row of the u-th child block of that product block, offset to the stage's first element
+// row (padding points into the dummy node rows, which hold 0).  The workgroups of the first
+// row tile (m0 = 0) also store the element rows into ewrite (the slab the backward reads).
+template <typename T, bool PRE, int KP>
 __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                    const float* __restrict__ params, const int* __restrict__ nids,
                                                    const int* __restrict__ cids, const int* __restrict__ pids,
                                                    const int* __restrict__ nch, int K, int64_t B, int mtiles,
-                                                   int btiles) {
+                                                   int btiles, const float* __restrict__ nsrc,
+                                                   const int* __restrict__ pch, int nstages, float* ewrite) {
     constexpr int TM = 64, TN = 64, LD = TK + MF<T>::PAD;
     __shared__ __attribute__((aligned(16))) T As[TM][LD];
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
@@ -537,14 +545,35 @@ __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars
     floatx16 acc = {0};
 
     float4 ev[kVecRows], av[kVecRows], ev2[PRE ? kVecRows : 1], av2[PRE ? kVecRows : 1];
-    auto load = [&](int k0, float4* e, float4* a) {
+    int64_t ce = 0, ce2 = 0;     // first element row of the stage held in ev / ev2
+    auto load = [&](int k0, float4* e, float4* a, int64_t& ec) {
         const int64_t c0 = crow[k0], p0 = prow[k0];
+        ec = c0;
+        if constexpr (KP > 0) {
+            const int* pb = pch + ((int64_t)j * nstages + k0 / TK) * KP;
+            int64_t cb[KP];
 #pragma unroll
-        for (int r = 0; r < kVecRows; ++r) {
-            const int k = 8 * g + r;
-            const int kc = (k0 + k < n) ? k : 0;
-            e[r] = *reinterpret_cast<const float4*>(emars + (c0 + kc) * B + bqc);
-            if (!(k0 + k < n && bok)) e[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+            for (int u = 0; u < KP; ++u) cb[u] = pb[u];
+#pragma unroll
+            for (int r = 0; r < kVecRows; ++r) {
+                const int k = 8 * g + r;
+                const int kc = (k0 + k < n) ? k : 0;
+                float4 v = *reinterpret_cast<const float4*>(nsrc + (cb[0] + kc) * B + bqc);
+#pragma unroll
+                for (int u = 1; u < KP; ++u) {
+                    const float4 x = *reinterpret_cast<const float4*>(nsrc + (cb[u] + kc) * B + bqc);
+                    v.x += x.x; v.y += x.y; v.z += x.z; v.w += x.w;
+                }
+                e[r] = v;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < kVecRows; ++r) {
+                const int k = 8 * g + r;
+                const int kc = (k0 + k < n) ? k : 0;
+                e[r] = *reinterpret_cast<const float4*>(emars + (c0 + kc) * B + bqc);
+                if (!(k0 + k < n && bok)) e[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+            }
         }
 #pragma unroll
         for (int r = 0; r < kVecRows; ++r) {
@@ -554,9 +583,19 @@ __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars
             if (!(k0 + k < n)) a[r] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
-    if (n > 0) load(0, ev, av);
-    if (PRE && TK < n) load(TK, ev2, av2);
+    if (n > 0) load(0, ev, av, ce);
+    if (PRE && TK < n) load(TK, ev2, av2, ce2);
     for (int k0 = 0; k0 < n; k0 += TK) {
+        if constexpr (KP > 0) {
+            // the product rows of this stage: stored once (first row tile), masked for the reduction
+#pragma unroll
+            for (int r = 0; r < kVecRows; ++r) {
+                const int k = 8 * g + r;
+                const bool ok = k0 + k < n && bok;
+                if (ok && m0 == 0) *reinterpret_cast<float4*>(ewrite + (ce + k) * B + bq) = ev[r];
+                if (!ok) ev[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+            }
+        }
         float mn[4];
         vec_stage_max(ev, red, mrun, msc, g, q, mn);
         vec_stage_b<T, LD>(ev, mn, Bs, g, q);
@@ -580,8 +619,9 @@ __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars
                     ev[r] = ev2[r];
                     av[r] = av2[r];
                 }
+                ce = ce2;
             } else {
-                load(k0 + TK, ev, av);
+                load(k0 + TK, ev, av, ce);
             }
         }
         acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
@@ -1390,11 +1430,13 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
         if (run && aligned && bs >= 64 && B % 4 == 0) {
             vec = true;
             if (tiles < 256)
-                sum_fwd_vec<T, true><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids,
-                                                                         nch, K, B, mtiles, (int)btiles);
+                sum_fwd_vec<T, true, 0><<<(unsigned)tiles, 256, 0, stream>>>(
+                    node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles, (int)btiles, nullptr, nullptr, 0,
+                    nullptr);
             else
-                sum_fwd_vec<T, false><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids,
-                                                                          nch, K, B, mtiles, (int)btiles);
+                sum_fwd_vec<T, false, 0><<<(unsigned)tiles, 256, 0, stream>>>(
+                    node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles, (int)btiles, nullptr, nullptr, 0,
+                    nullptr);
         }
     }
     if (vec) {
@@ -1404,6 +1446,48 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     else
         launch_fwd_tile<T, 32, 128>(run, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles,
                                     (int)btiles, stream);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+// Sum forward with the product layer below fused in (sum_fwd_vec with KP > 0).  Only the 16-byte
+// RUN form exists: the caller checks the layout (runs of >= TK children inside one product block)
+// and calls the plain product and sum forwards otherwise.
+template <typename T, int KP>
+void launch_fwd_prod_kp(bool pre, unsigned tiles, float* node_mars, float* emars, const float* params,
+                        const int* nids, const int* cids, const int* pids, const int* nch, int K, int64_t B,
+                        int mtiles, int btiles, const int* pch, int nstages, hipStream_t stream) {
+    if (pre)
+        sum_fwd_vec<T, true, KP><<<tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                            mtiles, btiles, node_mars, pch, nstages, emars);
+    else
+        sum_fwd_vec<T, false, KP><<<tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                                             mtiles, btiles, node_mars, pch, nstages, emars);
+}
+
+template <typename T>
+int launch_fwd_prod(float* node_mars, float* emars, const float* params, const int* nids, const int* cids,
+                    const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B, int child_run,
+                    const int* pch, int kp, hipStream_t stream) {
+    const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
+    child_run &= ~PJAMD_PARAMS_ALIGNED;
+    if (!(aligned && child_run >= TK && child_run % TK == 0 && bs >= 64 && bs % 64 == 0 && B % 4 == 0))
+        return PJAMD_EUNSUPPORTED;
+    const int mtiles = bs / 64;
+    const int64_t btiles = cdiv(B, 64);
+    const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
+    PJAMD_REQUIRE(grid_fits(tiles));
+    const int nstages = (int)cdiv(K, TK);
+    const bool pre = tiles < 256;
+    switch (kp) {
+        case 2: launch_fwd_prod_kp<T, 2>(pre, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                         mtiles, (int)btiles, pch, nstages, stream); break;
+        case 4: launch_fwd_prod_kp<T, 4>(pre, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                         mtiles, (int)btiles, pch, nstages, stream); break;
+        case 8: launch_fwd_prod_kp<T, 8>(pre, (unsigned)tiles, node_mars, emars, params, nids, cids, pids, nch, K, B,
+                                         mtiles, (int)btiles, pch, nstages, stream); break;
+        default: return PJAMD_EUNSUPPORTED;
+    }
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -1572,6 +1656,21 @@ int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const fl
                                     child_run, stream);
     return launch_fwd<float>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, child_run,
                              stream);
+}
+
+PJAMD_EXPORT int pjamd_sum_forward_prod(float* node_mars, float* element_mars, const float* params, const int* nids,
+                                        const int* cids, const int* pids, const int* n_chs, int n_blocks, int K,
+                                        int bs, int64_t B, int child_run, const int* prod_bases, int prod_k,
+                                        int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && prod_k >= 1);
+    if (n_blocks == 0) return PJAMD_OK;
+    if (precision == 1)
+        return launch_fwd_prod<__bf16>(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs, B,
+                                       child_run, prod_bases, prod_k, stream);
+    if (precision == 2)
+        return launch_fwd_prod<_Float16>(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs,
+                                         B, child_run, prod_bases, prod_k, stream);
+    return PJAMD_EUNSUPPORTED;
 }
 
 int pjamd_sum_backward_ele_mfma(float* eflows, const float* nflows, const float* nmars, const float* emars,

@@ -221,18 +221,29 @@ class SumLayer(Layer, nn.Module):
 
     def forward(self, node_mars: torch.Tensor, element_mars: torch.Tensor, params: torch.Tensor,
                 force_use_bf16: bool = False, force_use_fp32: bool = False, propagation_alg: str = "LL",
-                precision: Optional[str] = None, **kwargs):
+                precision: Optional[str] = None, prod_tables: Optional[dict] = None, **kwargs):
         """``node_mars[n] = log sum_c theta[n, c] exp(element_mars[c])`` for every node of the
-        layer (``sum_layer.py:220-262``), with per-tile max subtraction."""
+        layer (``sum_layer.py:220-262``), with per-tile max subtraction.  ``prod_tables`` (from
+        ``TensorCircuit._build_prod_fusion``, keyed by layer): compute the element rows from the
+        product layer's children in the same launch and store them (``pjamd_sum_forward_prod``)."""
         assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
         prec = PRECISIONS[precision] if precision is not None else (1 if force_use_bf16 else 0)
         if force_use_fp32:
             prec = 0
         B = node_mars.size(1)
         bs = self.block_size
-        for (nids, cids, pids, pfids, nch, tot), run, al in zip(self._dev["fw"], self._fw_child_run,
-                                                                self._fw_aligned):
+        fused = prod_tables.get(id(self)) if prod_tables is not None else None
+        for e, ((nids, cids, pids, pfids, nch, tot), run, al) in enumerate(zip(self._dev["fw"], self._fw_child_run,
+                                                                               self._fw_aligned)):
             run = run | (PARAMS_ALIGNED if al else 0)
+            if fused is not None:
+                tab, kp, reads, writes = fused[e]
+                # product rows gathered and stored, node rows written, parameters read once
+                algo = (4 * (reads + writes + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
+                N.call("pjamd_sum_forward_prod", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
+                       N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, N.ptr(tab), kp,
+                       prec, N.stream(), algo=algo)
+                continue
             # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
             # per sample, parameters read once
             algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)

@@ -109,6 +109,14 @@ class TensorCircuit(nn.Module):
         # flows are taken per layer inside the loop.  element_mars then takes max_g rows instead
         # of the sum over groups, at the cost of a second product pass.
         self.recompute_elements = False
+        # Product layer groups computed inside the forward of the sum group above them
+        # (pjamd_sum_forward_prod) where the layout allows it; BF16 / FP16 operands, B % 4 == 0,
+        # sum launches below fuse_products_max_tiles workgroups.  Off by default: on HCLT-784
+        # (256 latents, batch 512) the node block's row tiles gathering the product children
+        # again cost more than the product launches they save (3.05 vs 3.02 ms per step).
+        self.fuse_products = False
+        self.fuse_products_max_tiles = 256
+        self._prod_fusion = {"groups": set(), "tables": dict(), "tiles": dict()}
 
     # ------------------------------------------------------------------ parameter-flow buffer
 
@@ -169,6 +177,7 @@ class TensorCircuit(nn.Module):
         self._mask_unreached_products()
         self._par_batches = self._build_par_batches(device)
         self._fused = self._build_fused_backward(device)
+        self._prod_fusion = self._build_prod_fusion(device)
         self._pf_overwrite_ok = self._sum_flows_overwritable()
         self._chain_cache = dict()
 
@@ -231,6 +240,90 @@ class TensorCircuit(nn.Module):
                     slots = (f[real][:, None] + np.arange(layer.block_size)[None, :]).reshape(-1)
                     np.add.at(cover, slots, 1)
         return bool(np.all(cover == 1))
+
+    # Product children a fused forward stage reads (pjamd_sum_forward_prod's prod_k choices).
+    _PROD_K = (2, 4, 8)
+
+    def _build_prod_fusion(self, device):
+        """Product layer groups whose forward runs inside the next sum group's forward
+        (``pjamd_sum_forward_prod``), with each sum table's ``prod_bases``.  Checked on the index
+        tables: every forward table of the sum group takes the 16-byte run form (block size a
+        multiple of 64, child runs of >= 128, aligned parameters); every stage of 128 children of
+        a node block lies inside one product block of at most 8 child blocks; the stages cover
+        every product row (so the fused forward stores the whole element slab the backward
+        reads).  Used at run time only for BF16 / FP16 operands, B % 4 == 0 and sum launches of
+        fewer than ``fuse_products_max_tiles`` workgroups: there a launch is a few dependent memory
+        round trips and the product launch is saved whole, while on wide layers each of a node
+        block's row tiles would gather the product children again."""
+        groups = self.inner_layer_groups
+        fused_groups, tables, tiles = set(), dict(), dict()
+        for gid in range(len(groups) - 1):
+            gp, gs = groups[gid], groups[gid + 1]
+            if not gp.is_prod() or gs.is_prod():
+                continue
+            starts, sizes, kids = [], [], []
+            for layer in gp:
+                for nids, cids, _ in layer._dev["fw"]:
+                    c = cids.detach().cpu().numpy().astype(np.int64)
+                    for e, row in zip(nids.detach().cpu().numpy().astype(np.int64).tolist(), c):
+                        starts.append(e)
+                        sizes.append(layer.block_size)
+                        kids.append(row[row != 0])
+            if not starts or max(len(k) for k in kids) > self._PROD_K[-1]:
+                continue
+            order = np.argsort(np.asarray(starts))
+            st = np.asarray(starts, dtype=np.int64)[order]
+            sz = np.asarray(sizes, dtype=np.int64)[order]
+            kd = [kids[i] for i in order]
+            covered = np.zeros(int((st + sz).max()), dtype=bool)
+            ok, per_layer = True, dict()
+            for layer in gs:
+                ents = []
+                for (nids, cids, pids, pfids, nch, tot), run, al in zip(layer._dev["fw"], layer._fw_child_run,
+                                                                        layer._fw_aligned):
+                    if not (layer.block_size % 64 == 0 and run >= 128 and run % 128 == 0 and al):
+                        ok = False
+                        break
+                    c = cids.detach().cpu().numpy().astype(np.int64)
+                    n = nch.detach().cpu().numpy().astype(np.int64)
+                    nst = -(-c.shape[1] // 128)
+                    rows_tab = np.zeros([c.shape[0], nst, self._PROD_K[-1]], dtype=np.int64)
+                    kp_max, reads, writes = 1, 0, 0
+                    for j in range(c.shape[0]):
+                        for sidx, k0 in enumerate(range(0, int(n[j]), 128)):
+                            e0, rows = int(c[j, k0]), min(128, int(n[j]) - k0)
+                            pb = int(np.searchsorted(st, e0, side="right")) - 1
+                            if pb < 0 or e0 + rows > st[pb] + sz[pb]:
+                                ok = False
+                                break
+                            r0 = e0 - int(st[pb])
+                            k = kd[pb]
+                            kp_max = max(kp_max, len(k))
+                            rows_tab[j, sidx, :] = r0                  # dummy rows (value 0) by default
+                            rows_tab[j, sidx, :len(k)] = k + r0
+                            covered[e0:e0 + rows] = True
+                            reads += rows * len(k)
+                            writes += rows
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                    kp = next(v for v in self._PROD_K if v >= kp_max)
+                    tab = torch.from_numpy(np.ascontiguousarray(rows_tab[:, :, :kp]).astype(np.int32)).to(device)
+                    ents.append((tab, kp, reads, writes))
+                if not ok:
+                    break
+                per_layer[id(layer)] = ents
+            if not ok:
+                continue
+            if not all(bool(covered[a:a + b].all()) for a, b in zip(st.tolist(), sz.tolist())):
+                continue
+            fused_groups.add(gid)
+            tables.update(per_layer)
+            # workgroups per 64 batch columns of the group's widest forward launch
+            tiles[gid] = max(int(nids.numel()) * (layer.block_size // 64)
+                             for layer in gs for nids, *_ in layer._dev["fw"])
+        return {"groups": fused_groups, "tables": tables, "tiles": tiles}
 
     def _build_fused_backward(self, device):
         """Sum layers whose element-flow kernel also performs the product backward below them
@@ -460,14 +553,22 @@ class TensorCircuit(nn.Module):
         off = 0 if self.recompute_elements else self._elem_offsets[group_id // 2]
         return self.element_mars.narrow(0, off, self._group_elems[group_id // 2])
 
+    def _fuse_now(self, fz, gid: int, B: int) -> bool:
+        return (fz is not None and gid in fz["groups"]
+                and fz["tiles"][gid] * -(-B // 64) < self.fuse_products_max_tiles)
+
     def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
         runs = self._chains("fwd", precision)
         groups = self.inner_layer_groups
+        B = self.node_mars.size(1)
+        fz = self._prod_fusion
+        if (not self.fuse_products or runs or self.recompute_elements or precision not in ("bf16", "fp16")
+                or B % 4 != 0):
+            fz = None
         gid = 0
         while gid < len(groups):
             run = runs.get(gid)
             if run is not None:
-                B = self.node_mars.size(1)
                 N.call("pjamd_chain_forward", N.ptr(self.node_mars), N.ptr(self.element_mars), N.ptr(self.params),
                        N.ptr(run.ops), run.n_ops, N.ptr(run.idx), B, PRECISIONS[precision], N.stream(),
                        algo=run.algo(B))
@@ -475,10 +576,12 @@ class TensorCircuit(nn.Module):
                 continue
             g = groups[gid]
             if g.is_prod():
-                g(self.node_mars, self._slab(gid))
+                if not self._fuse_now(fz, gid, B):   # else run inside the next sum group
+                    g(self.node_mars, self._slab(gid))
             else:
+                tabs = fz["tables"] if self._fuse_now(fz, gid - 1, B) else None
                 g(self.node_mars, self._slab(gid), self.params, force_use_bf16=force_use_bf16,
-                  force_use_fp32=force_use_fp32, precision=precision)
+                  force_use_fp32=force_use_fp32, precision=precision, prod_tables=tabs)
             gid += 1
 
     def forward(self, inputs: torch.Tensor, input_layer_fn: Optional[Union[str, Callable]] = None,

@@ -326,3 +326,32 @@ def test_recompute_elements_matches_resident(name, precision, device):
     assert U.rel_err(out[True][1], out[False][1]) <= 1e-5
     for a, b in zip(out[True][2], out[False][2]):
         assert U.rel_err(a, b) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(6, 128, 10), (8, 256, 6)])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_product_fusion_matches_layerwise(shape, precision, device):
+    """The product layers computed inside the next sum forward (pjamd_sum_forward_prod) give the
+    same node marginals, element rows and parameter flows, bit for bit, as separate product and
+    sum launches (the same additions in the same order), and the log-likelihoods match the FP64
+    oracle."""
+    num_vars, L, K = shape
+    U.say(f"[prod fusion] HCLT V={num_vars} L={L} {precision}")
+    ns, x, cards = U.hclt_categorical(num_vars=num_vars, L=L, K=K)
+    pc = TensorCircuit(ns, precision=precision)
+    pc.to(device)
+    assert pc._prod_fusion["groups"] and pc._fuse_now(pc._prod_fusion, min(pc._prod_fusion["groups"]), 64)
+    X = U.random_data(cards, 64, seed=21)
+    xt = torch.from_numpy(X).to(device)
+    out = dict()
+    for fuse in (True, False):    # fused first: element rows it failed to store would differ
+        pc.fuse_products = fuse
+        lls = pc(xt).detach().clone()
+        nm, em = pc.node_mars.detach().clone(), pc.element_mars.detach().clone()
+        pc.backward(xt, flows_memory=0.0)
+        out[fuse] = (lls, nm, em, pc.param_flows.detach().clone())
+    for a, b in zip(out[False], out[True]):
+        assert torch.equal(a, b)
+    pc.update_parameters()
+    ref = dag_eval.forward(ns, X)[ns][0]
+    assert np.abs(out[True][0].cpu().numpy()[:, 0] - ref).max() <= 1e-3 * num_vars

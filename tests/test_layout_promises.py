@@ -130,3 +130,44 @@ def test_merged_partition_tables(num_vars, L):
                     assert (k, parids[i, :k].tolist(), parpids[i, :k].tolist()) == by_cbs[cbs][c]
                     assert not parids[i, k:].any() and not parpids[i, k:].any()
     assert merged > 0
+
+
+@pytest.mark.parametrize("num_vars,L,K", [(6, 128, 10), (8, 256, 6)])
+def test_product_fusion_tables(num_vars, L, K):
+    """The fused forward's ``prod_bases`` rebuild the product layer: for every stage of every node
+    block, the sum over the listed node rows equals the element rows the stage reads (FP64 oracle
+    buffers), padding entries point into the zero dummy node rows, and the fused product groups'
+    element rows are all covered."""
+    import torch
+    from oracle.pc_cpu import CompiledCPU
+    ns, x, cards = U.hclt_categorical(num_vars=num_vars, L=L, K=K)
+    pc = TensorCircuit(ns)
+    pc._build_launch_plans(torch.device("cpu"))
+    fz = pc._prod_fusion
+    assert fz["groups"], "no product group fused"
+    cpu = CompiledCPU(pc, np.float64)
+    cpu.forward(U.random_data(cards, 8, seed=5))
+    nm = cpu.node_mars
+    G = pc.inner_layer_groups
+    assert not nm[:pc.num_dummy_nodes].any()
+    for gid in fz["groups"]:
+        assert G[gid].is_prod() and not G[gid + 1].is_prod()
+        em = cpu.element_mars[pc._elem_offsets[gid // 2]:]
+        seen = set()
+        for layer in G[gid + 1]:
+            for (nids, cids, pids, pfids, nch, tot), (tab, kp, reads, writes) in zip(layer._dev["fw"],
+                                                                                      fz["tables"][id(layer)]):
+                c, n, t = cids.numpy(), nch.numpy(), tab.numpy().astype(np.int64)
+                assert t.shape == (c.shape[0], -(-c.shape[1] // 128), kp) and kp in (2, 4, 8)
+                for j in range(c.shape[0]):
+                    for s, k0 in enumerate(range(0, int(n[j]), 128)):
+                        rows = min(128, int(n[j]) - k0)
+                        e0 = int(c[j, k0])
+                        got = sum(nm[t[j, s, u]:t[j, s, u] + rows] for u in range(kp))
+                        np.testing.assert_allclose(got, em[e0:e0 + rows], rtol=1e-12, atol=1e-12)
+                        assert (t[j, s] + rows <= pc.num_nodes).all()
+                        seen.update(range(e0, e0 + rows))
+        for layer in G[gid]:
+            for nids, cids, _ in layer._dev["fw"]:
+                for e in nids.numpy().tolist():
+                    assert set(range(e, e + layer.block_size)) <= seen
