@@ -279,7 +279,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the data-parallel design for "
-              f"several GPUs is described in DESIGN.md and is not run here.", file=sys.stderr)
+              f"several GPUs (data-parallel over the batch, parameter flows reduce-scattered) is not run "
+              f"here.", file=sys.stderr)
         sys.exit(2)
     import torch
 

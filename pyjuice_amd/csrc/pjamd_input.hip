@@ -464,7 +464,42 @@ __global__ __launch_bounds__(256) void bern_bwd_kernel(PF* __restrict__ pflows, 
     }
 }
 
-// Categorical EM, one wavefront per source node; sums and the update run in double.
+// Categorical EM, one wavefront per source node; sums and the update run in double.  Nodes of at
+// most 64 * KR categories (KR = 4: 256, KR = 16: 1024) hold their flows and parameters in
+// registers: every lane issues its KR loads of each at once (index clamped, value masked on use),
+// so the normaliser and the update cost one memory round trip; wider nodes loop.
+template <typename PF, int KR>
+__device__ __forceinline__ void cat_em_regs(float* __restrict__ params, const PF* __restrict__ pflows, int K,
+                                            int64_t p0, int64_t f0, int lane, double step, double pc,
+                                            bool keep_zero) {
+    const double npc = pc / (double)K;
+    double f[KR];
+    float t[KR];
+#pragma unroll
+    for (int q = 0; q < KR; ++q) {
+        const int k = lane + 64 * q;
+        const int kc = k < K ? k : 0;
+        f[q] = (double)pflows[f0 + kc];
+        t[q] = params[p0 + kc];
+    }
+    double cum = 0.0;
+#pragma unroll
+    for (int q = 0; q < KR; ++q) {
+        if (lane + 64 * q < K) cum += keep_zero ? ((t[q] < 1e-12f) ? 0.0 : f[q] + npc) : f[q];
+    }
+    cum = warp_sum(cum) + pc;
+#pragma unroll
+    for (int q = 0; q < KR; ++q) {
+        const int k = lane + 64 * q;
+        if (k >= K) continue;
+        const double th = (double)t[q];
+        double nv;
+        if (keep_zero) nv = (th < 1e-12) ? 0.0 : (1.0 - step) * th + step * (f[q] + npc) / (cum - pc);
+        else nv = (1.0 - step) * th + step * (f[q] + npc) / cum;
+        params[p0 + k] = (float)nv;
+    }
+}
+
 template <typename PF>
 __global__ __launch_bounds__(256) void cat_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
                                                      const int* __restrict__ src, const int64_t* __restrict__ s_pids,
@@ -476,6 +511,14 @@ __global__ __launch_bounds__(256) void cat_em_kernel(float* __restrict__ params,
     const int i = src[w];
     const int K = ncats[i];
     const int64_t p0 = s_pids[i], f0 = s_pfids[i];
+    if (K <= 256) {
+        cat_em_regs<PF, 4>(params, pflows, K, p0, f0, lane, step, pc, keep_zero);
+        return;
+    }
+    if (K <= 1024) {
+        cat_em_regs<PF, 16>(params, pflows, K, p0, f0, lane, step, pc, keep_zero);
+        return;
+    }
     const double npc = pc / (double)K;
     double cum = 0.0;
     for (int k = lane; k < K; k += 64) {

@@ -29,18 +29,18 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
 
 // Mini-batch EM of the sum parameters.  Record = [par_start, pflow_start, node_start,
 // block_size, n_children]; the parameter of child c of node i sits at par_start + c * block_size
-// + i (same for flows).  A workgroup of four wavefronts handles (record, up to 64 nodes): lanes
+// + i (same for flows).  A workgroup of kEmWaves wavefronts handles (record, up to 64 nodes): lanes
 // run along the nodes (ti = min(bs, 64) nodes, G = 64 / ti lanes per node) and the wavefronts
 // split the children.  When every lane owns at most kRegChildren children (a block size dividing
 // or a multiple of 64, and at most kRegChildren children per lane: 256 children at block size
 // >= 64, 512 at 32) the flows and parameters stay in registers between the normaliser and the
 // update, so each is read from memory once; otherwise the update re-reads them.
-constexpr int kEmWaves = 4;
-constexpr int kRegChildren = 64;
+constexpr int kEmWaves = 8;
+constexpr int kRegChildren = 32;
 constexpr int kEmChunk = 16;      // children per lane per round trip when they do not fit registers
 
 template <typename PF>
-__global__ __launch_bounds__(256) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
+__global__ __launch_bounds__(64 * kEmWaves) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
                                                      const int* __restrict__ recs, int n_recs, double step, double pc,
                                                      bool keep_zero) {
     __shared__ double part[kEmWaves][64];
@@ -227,11 +227,11 @@ PJAMD_EXPORT int pjamd_sum_em(float* params, const void* pflows, int pf64, const
     if (n_recs == 0) return PJAMD_OK;
     dim3 grid((unsigned)n_recs, cdiv(max_bs, 64));
     if (pf64)
-        sum_em_kernel<double><<<grid, 256, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step, pseudocount,
-                                                        keep_zero != 0);
+        sum_em_kernel<double><<<grid, 64 * kEmWaves, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step,
+                                                                  pseudocount, keep_zero != 0);
     else
-        sum_em_kernel<float><<<grid, 256, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step, pseudocount,
-                                                       keep_zero != 0);
+        sum_em_kernel<float><<<grid, 64 * kEmWaves, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step,
+                                                                 pseudocount, keep_zero != 0);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
