@@ -77,11 +77,20 @@ int pjamd_fill_f32(float* p, float v, int64_t n, pjamd_stream_t stream);
  * (tensorcircuit.py:226-230). */
 int pjamd_data_to_vb(int* out, int src_code, const void* src, int64_t B, int64_t V, pjamd_stream_t stream);
 
+/* data[v][b] = -1 wherever the missing mask is set: a marginalized entry, which
+ * pjamd_input_forward gives log-probability 0 and pjamd_input_missing_flows takes the flows of.
+ * data is int32 [V][B] as pjamd_data_to_vb writes it; mask is uint8, [V] (mode 0), [B][V]
+ * (mode 1) or [V][B] (mode 2).  Replaces the missing_mask pass of InputLayer.forward
+ * (src/pyjuice/layer/input_layer.py:374-401, _fw_missing_mask_kernel 1058-1095) and the
+ * mask loads of the flows kernel (input_layer.py:1120-1133). */
+int pjamd_data_mask(int* data, const uint8_t* mask, int mode, int64_t B, int64_t V, pjamd_stream_t stream);
+
 /* ----------------------------------------------------------------------------- input layers */
 
 /* node_mars[node_offset + n][b] = log p_n(data[vids[n]][b]) for the num_nodes nodes of one
  * input layer; s_pids[n] is node n's first parameter, ncats[n] its category count and kmax the
- * largest category count of the layer (Categorical; ignored for Bernoulli).
+ * largest category count of the layer (Categorical; ignored for Bernoulli).  A negative data
+ * value is a marginalized entry (log p = 0); a category >= ncats[n] has p = 0.
  * Replaces InputLayer.forward (src/pyjuice/layer/input_layer.py:283-404) with
  * Categorical.fw_mar_fn (categorical.py:83-92) / Bernoulli.fw_mar_fn (bernoulli.py:65-71). */
 int pjamd_input_forward(int kind, const float* params, float* node_mars, const int* data, const int* vids,
@@ -97,10 +106,22 @@ int pjamd_input_forward(int kind, const float* params, float* node_mars, const i
  * share parameter-flow slots (then the adds are atomic).
  * Replaces InputLayer.backward
  * (input_layer.py:406-571) with Categorical.bk_flow_fn (categorical.py:94-101) /
- * Bernoulli.bk_flow_fn (bernoulli.py:73-77). */
+ * Bernoulli.bk_flow_fn (bernoulli.py:73-77).  Negative data values (marginalized entries) are
+ * skipped; see pjamd_input_missing_flows. */
 int pjamd_input_backward(int kind, void* pflows, const float* node_flows, int pf64, const int* data, const int* vids,
                          const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
                          int layout, pjamd_stream_t stream);
+
+/* Flows of the marginalized entries (negative data values) of one input layer, after
+ * pjamd_input_backward: Categorical node n adds params[s_pids[n] + k] * (its flows summed over
+ * the marginalized batch columns) to pflows[s_pfids[n] + k] for every category k (atomically);
+ * Bernoulli nodes take none, as in the reference.  Replaces the masked pass of
+ * InputLayer.backward (input_layer.py:529-566) with Categorical.bk_flow_mask_fn
+ * (categorical.py:103-134). */
+int pjamd_input_missing_flows(int kind, void* pflows, int pf64, const float* node_flows, const float* params,
+                              const int* data, const int* vids, const int64_t* s_pids, const int64_t* s_pfids,
+                              const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
+                              pjamd_stream_t stream);
 
 /* Mini-batch EM of the num_src source nodes src[] of an input layer:
  * theta <- (1 - step) theta + step * (flows + pseudocount / K) / (sum flows + pseudocount),

@@ -45,6 +45,9 @@ _SIGNATURES = {
     "pjamd_input_sample": [_i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i64, _i64, _c_p],
     "pjamd_prod_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_prod_backward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
+    "pjamd_data_mask": [_c_p, _c_p, _i32, _i64, _i64, _c_p],
+    "pjamd_input_missing_flows": [_i32, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i64, _i64, _i64,
+                                  _c_p],
     "pjamd_sum_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_forward_prod": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,
                                _i32, _c_p],

@@ -8,13 +8,24 @@ namespace {
 
 constexpr int kLdsFloats = 4096; // largest category count with LDS flow histograms
 
-// node_mars[(off + i) * B + b] = log p_i(x[v_i, b]); out-of-range categories have p = 0.
+// node_mars[(off + i) * B + b] = log p_i(x[v_i, b]); categories past K have p = 0, negative ones
+// (missing entries) are marginalized: log 1 = 0.
 // kRowThreads threads per group of NT consecutive node rows, 4 consecutive batch columns each
 // (16-byte loads of the categories, 16-byte stores).  A thread issues the category loads of all NT
 // nodes, then all 4 NT parameter gathers (they hit each node's row of kmax floats in L1 / L2),
 // so a workgroup waits for two memory round trips per NT rows instead of per row.
 constexpr int kRowThreads = 128;
 constexpr int kFwdNodes = 4;
+
+// log p of one data entry.  A negative category marks a marginalized (missing) entry, whose
+// probability summed over all values is 1; a category past K has probability 0.  th is the
+// gathered parameter of category x (any finite value when x is out of range).
+__device__ __forceinline__ float cat_logp(int x, int K, float th) {
+    return x < 0 ? 0.0f : (x < K ? __logf(th) : kNegInf);
+}
+__device__ __forceinline__ float bern_logp(int x, float p1) {
+    return x < 0 ? 0.0f : __logf(x > 0 ? p1 : 1.0f - p1);
+}
 
 template <int KIND, int NT>
 __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict__ params, float* __restrict__ node_mars,
@@ -60,15 +71,15 @@ __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict_
                 float4 r;
                 if (KIND == 0) {
                     const int k = K[n];
-                    r.x = (x[n].x >= 0 && x[n].x < k) ? __logf(o[n].x) : kNegInf;
-                    r.y = (x[n].y >= 0 && x[n].y < k) ? __logf(o[n].y) : kNegInf;
-                    r.z = (x[n].z >= 0 && x[n].z < k) ? __logf(o[n].z) : kNegInf;
-                    r.w = (x[n].w >= 0 && x[n].w < k) ? __logf(o[n].w) : kNegInf;
+                    r.x = cat_logp(x[n].x, k, o[n].x);
+                    r.y = cat_logp(x[n].y, k, o[n].y);
+                    r.z = cat_logp(x[n].z, k, o[n].z);
+                    r.w = cat_logp(x[n].w, k, o[n].w);
                 } else {
-                    r.x = __logf(x[n].x > 0 ? p1[n] : 1.0f - p1[n]);
-                    r.y = __logf(x[n].y > 0 ? p1[n] : 1.0f - p1[n]);
-                    r.z = __logf(x[n].z > 0 ? p1[n] : 1.0f - p1[n]);
-                    r.w = __logf(x[n].w > 0 ? p1[n] : 1.0f - p1[n]);
+                    r.x = bern_logp(x[n].x, p1[n]);
+                    r.y = bern_logp(x[n].y, p1[n]);
+                    r.z = bern_logp(x[n].z, p1[n]);
+                    r.w = bern_logp(x[n].w, p1[n]);
                 }
                 if (n < nn) *reinterpret_cast<float4*>(node_mars + (node_offset + i0 + n) * B + b) = r;
             }
@@ -77,8 +88,8 @@ __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict_
                 float* out = node_mars + (node_offset + i0 + n) * B;
                 for (int64_t c = b; c < B; ++c) {
                     const int xx = xrow[n][c];
-                    if (KIND == 0) out[c] = (xx >= 0 && xx < K[n]) ? __logf(th[n][xx]) : kNegInf;
-                    else out[c] = __logf(xx > 0 ? p1[n] : 1.0f - p1[n]);
+                    if (KIND == 0) out[c] = cat_logp(xx, K[n], (xx >= 0 && xx < K[n]) ? th[n][xx] : 1.0f);
+                    else out[c] = bern_logp(xx, p1[n]);
                 }
             }
         }
@@ -98,10 +109,9 @@ __global__ __launch_bounds__(256) void cat_fwd_global_kernel(const float* __rest
     const int x = data[(int64_t)vids[i] * B + b];
     float v;
     if (kind == 0) {
-        v = (x >= 0 && x < ncats[i]) ? __logf(params[s_pids[i] + x]) : kNegInf;
+        v = cat_logp(x, ncats[i], (x >= 0 && x < ncats[i]) ? params[s_pids[i] + x] : 1.0f);
     } else {
-        const float p = params[s_pids[i]];
-        v = __logf(x > 0 ? p : 1.0f - p);
+        v = bern_logp(x, params[s_pids[i]]);
     }
     node_mars[(node_offset + i) * B + b] = v;
 }
@@ -449,7 +459,7 @@ __global__ __launch_bounds__(256) void bern_bwd_kernel(PF* __restrict__ pflows, 
     for (int64_t b = threadIdx.x; b < B; b += blockDim.x) {
         const int x = data[(int64_t)vids[i] * B + b];
         const float f = node_flows[(node_offset + i) * B + b];
-        if (x > 0) s1 += f; else s0 += f;
+        if (x > 0) s1 += f; else if (x == 0) s0 += f;     // x < 0: marginalized, no flow
     }
     s1 = warp_sum(s1);
     s0 = warp_sum(s0);
@@ -462,6 +472,37 @@ __global__ __launch_bounds__(256) void bern_bwd_kernel(PF* __restrict__ pflows, 
         pf_add<PF>(&pflows[s_pfids[i]], (PF)a, atomic_out);
         pf_add<PF>(&pflows[s_pfids[i] + 1], (PF)c, atomic_out);
     }
+}
+
+// Flows of marginalized entries (negative categories), Categorical: a node's flow on a missing
+// entry is spread over its categories in proportion to theta (the posterior of the category when
+// the variable is summed out): param_flows[s_pfid_i + k] += theta_ik * sum of those flows.  One
+// workgroup per node; runs after the per-category pass, which skips these entries.
+template <typename PF>
+__global__ __launch_bounds__(256) void cat_missing_flows_kernel(PF* __restrict__ pflows,
+                                                                const float* __restrict__ node_flows,
+                                                                const float* __restrict__ params,
+                                                                const int* __restrict__ data,
+                                                                const int* __restrict__ vids,
+                                                                const int64_t* __restrict__ s_pids,
+                                                                const int64_t* __restrict__ s_pfids,
+                                                                const int* __restrict__ ncats, int64_t node_offset,
+                                                                int64_t B) {
+    __shared__ float red[4];
+    const int64_t i = blockIdx.x;
+    const int* xr = data + (int64_t)vids[i] * B;
+    const float* fr = node_flows + (node_offset + i) * B;
+    float f = 0.0f;
+    for (int64_t b = threadIdx.x; b < B; b += blockDim.x) f += xr[b] < 0 ? fr[b] : 0.0f;
+    f = warp_sum(f);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = f;
+    __syncthreads();
+    const float tot = red[0] + red[1] + red[2] + red[3];
+    if (tot == 0.0f) return;
+    const float* th = params + s_pids[i];
+    for (int k = threadIdx.x; k < ncats[i]; k += blockDim.x)
+        atomicAdd(&pflows[s_pfids[i] + k], (PF)th[k] * (PF)tot);
 }
 
 // Categorical EM, one wavefront per source node; sums and the update run in double.  Nodes of at
@@ -644,6 +685,22 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
         cat_bwd_global_kernel<PF><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
                                                            node_offset, B);
     }
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_input_missing_flows(int kind, void* pflows, int pf64, const float* node_flows,
+                                           const float* params, const int* data, const int* vids,
+                                           const int64_t* s_pids, const int64_t* s_pfids, const int* ncats,
+                                           int64_t num_nodes, int64_t node_offset, int64_t B, hipStream_t stream) {
+    PJAMD_REQUIRE(num_nodes >= 0 && B > 0 && (kind == 0 || kind == 1) && num_nodes < ((int64_t)1 << 31));
+    if (num_nodes == 0 || kind == 1) return PJAMD_OK;   // Bernoulli: missing entries carry no flow
+    if (pf64)
+        cat_missing_flows_kernel<double><<<(unsigned)num_nodes, 256, 0, stream>>>(
+            (double*)pflows, node_flows, params, data, vids, s_pids, s_pfids, ncats, node_offset, B);
+    else
+        cat_missing_flows_kernel<float><<<(unsigned)num_nodes, 256, 0, stream>>>(
+            (float*)pflows, node_flows, params, data, vids, s_pids, s_pfids, ncats, node_offset, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }

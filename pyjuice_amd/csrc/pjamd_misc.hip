@@ -221,6 +221,24 @@ PJAMD_EXPORT int pjamd_data_to_vb(int* out, int src_code, const void* src, int64
     return PJAMD_OK;
 }
 
+// data[v][b] <- -1 (marginalized) where the missing mask is set.  mode 0: mask[v]; 1: mask[b][v];
+// 2: mask[v][b].
+__global__ __launch_bounds__(256) void data_mask_kernel(int* __restrict__ data, const uint8_t* __restrict__ mask,
+                                                        int mode, int64_t B, int64_t V) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = blockIdx.y;
+    if (b >= B) return;
+    const uint8_t m = mode == 0 ? mask[v] : (mode == 1 ? mask[b * V + v] : mask[v * B + b]);
+    if (m) data[v * B + b] = -1;
+}
+
+PJAMD_EXPORT int pjamd_data_mask(int* data, const uint8_t* mask, int mode, int64_t B, int64_t V, hipStream_t stream) {
+    PJAMD_REQUIRE(B > 0 && V > 0 && V < 65536 && mode >= 0 && mode <= 2);
+    data_mask_kernel<<<dim3(cdiv(B, 256), (unsigned)V), 256, 0, stream>>>(data, mask, mode, B, V);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
 PJAMD_EXPORT int pjamd_sum_em(float* params, const void* pflows, int pf64, const int* recs, int n_recs, float step,
                               float pseudocount, int keep_zero, int max_bs, hipStream_t stream) {
     PJAMD_REQUIRE(n_recs >= 0 && max_bs >= 1);

@@ -18,6 +18,11 @@ from .layer import Layer
 __all__ = ["InputLayer"]
 
 
+def _apply_missing_mask(data_vb, missing_mask, batch_first=True):
+    from pyjuice_amd.model.tensorcircuit import apply_missing_mask
+    apply_missing_mask(data_vb, missing_mask, batch_first)
+
+
 class InputLayer(Layer, nn.Module):
     def __init__(self, nodes: Sequence, cum_nodes: int = 0, pc_num_vars: int = 0,
                  max_tied_ns_per_parflow_block: int = 4):
@@ -298,7 +303,10 @@ class InputLayer(Layer, nn.Module):
         self._used_external_params = params is not None
         p = self.params if params is None else params["params"]
         assert node_mars.is_cuda, "InputLayer.forward runs on the HIP device only"
-        assert missing_mask is None, "missing_mask is not supported by the HIP input layers"
+        if missing_mask is not None:
+            # marginalized entries (log p = 0) are marked -1 in a copy of the data
+            data = data.clone()
+            _apply_missing_mask(data, missing_mask, kwargs.get("_batch_first", True))
         t = self._dev
         B = node_mars.size(1)
         N.call("pjamd_input_forward", self._kind(), N.ptr(p), N.ptr(node_mars), N.ptr(data), N.ptr(t["vids"]),
@@ -306,9 +314,16 @@ class InputLayer(Layer, nn.Module):
                N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B + 4 * self.num_parameters, 0))
 
     def backward(self, data: torch.Tensor, node_flows: torch.Tensor, node_mars: torch.Tensor,
-                 params: Optional[Dict] = None, logspace_flows: bool = False, **kwargs):
-        """``param_flows[s_pfid_i + x] += node_flows[off + i, b]`` (``input_layer.py:406-571``)."""
+                 params: Optional[Dict] = None, logspace_flows: bool = False,
+                 missing_mask: Optional[torch.Tensor] = None, marginalized: bool = False, **kwargs):
+        """``param_flows[s_pfid_i + x] += node_flows[off + i, b]`` (``input_layer.py:406-571``).
+        Marginalized entries (``missing_mask``, or -1 in ``data`` with ``marginalized``) add their
+        flows to every category in proportion to ``theta`` (``categorical.py:103-134``)."""
         assert not logspace_flows, "logspace flows are not supported by the HIP input layers"
+        if missing_mask is not None:
+            data = data.clone()
+            _apply_missing_mask(data, missing_mask, kwargs.get("_batch_first", True))
+            marginalized = True
         if self._pf is None:
             self.init_param_flows(0.0)
         t = self._dev
@@ -324,6 +339,11 @@ class InputLayer(Layer, nn.Module):
                N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0],
                B, flags, N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B
                                             + (1 if overwrite else 2) * pf.element_size() * self.num_param_flows, 0))
+        if marginalized:
+            N.call("pjamd_input_missing_flows", self._kind(), N.ptr(pf), pf64, N.ptr(node_flows), N.ptr(self.params),
+                   N.ptr(data), N.ptr(t["vids"]), N.ptr(t["s_pids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]),
+                   self.num_nodes, self._output_ind_range[0], B, N.stream(),
+                   algo=(8 * self.num_nodes * B + 2 * pf.element_size() * self.num_param_flows, 0))
 
     def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False):
         """Tied-flow fusing then the closed-form EM update (``input_layer.py:663-724``)."""

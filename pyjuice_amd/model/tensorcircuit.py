@@ -62,6 +62,25 @@ def layer_iterator(pc, reverse=False, ret_layer_groups=False, ignore_input_layer
                 yield layer
 
 
+def apply_missing_mask(data_vb: torch.Tensor, missing_mask: torch.Tensor, batch_first: bool = True):
+    """Marks the entries of ``missing_mask`` in the ``[V, B]`` int32 device data as marginalized
+    (-1) in place.  ``missing_mask``: ``[V]``, ``[B, V]``, or ``[V, B]`` when ``batch_first`` is
+    False or the shape says so (the reference's ``_batch_first``, ``input_layer.py:378-383``)."""
+    V, B = data_vb.shape
+    m = missing_mask.to(device=data_vb.device, dtype=torch.uint8).contiguous()
+    if m.dim() == 1:
+        assert m.numel() == V, "a 1-D missing_mask has one entry per variable"
+        mode = 0
+    else:
+        assert m.dim() == 2
+        if tuple(m.shape) == (B, V) and (batch_first or (B, V) != (V, B)):
+            mode = 1
+        else:
+            assert tuple(m.shape) == (V, B), "missing_mask must be [V], [B, V] or [V, B]"
+            mode = 2
+    N.call("pjamd_data_mask", N.ptr(data_vb), N.ptr(m), mode, B, V, N.stream())
+
+
 class TensorCircuit(nn.Module):
     def __init__(self, root_ns: CircuitNodes, layer_sparsity_tol: float = 0.5,
                  max_num_partitions: Optional[int] = None, disable_gpu_compilation: bool = False,
@@ -117,6 +136,7 @@ class TensorCircuit(nn.Module):
         self.fuse_products = False
         self.fuse_products_max_tiles = 256
         self._prod_fusion = {"groups": set(), "tables": dict(), "tiles": dict()}
+        self._data_vb, self._data_marginalized = None, False
 
     # ------------------------------------------------------------------ parameter-flow buffer
 
@@ -529,8 +549,11 @@ class TensorCircuit(nn.Module):
         if t.numel() > 0:
             N.call("pjamd_fill_f32", N.ptr(t), float(value), t.numel(), N.stream())
 
-    def _to_vb(self, inputs: torch.Tensor) -> torch.Tensor:
-        """``[B, V]`` integer / bool data -> contiguous ``[V, B]`` int32 on the device."""
+    def _to_vb(self, inputs: torch.Tensor, missing_mask: Optional[torch.Tensor] = None,
+               batch_first: bool = True) -> torch.Tensor:
+        """``[B, V]`` integer / bool data -> contiguous ``[V, B]`` int32 on the device; entries of
+        ``missing_mask`` (``[V]``, ``[B, V]`` or, with ``batch_first=False``, ``[V, B]``) become -1,
+        which the input layers marginalize (``input_layer.py:374-401``)."""
         assert inputs.dim() == 2 and inputs.size(1) == self.num_vars
         x = inputs if inputs.device == self.device else inputs.to(self.device)
         x = x.contiguous()
@@ -539,6 +562,8 @@ class TensorCircuit(nn.Module):
         B, V = x.shape
         out = torch.empty([V, B], device=self.device, dtype=torch.int32)
         N.call("pjamd_data_to_vb", N.ptr(out), code, N.ptr(x), B, V, N.stream())
+        if missing_mask is not None:
+            apply_missing_mask(out, missing_mask, batch_first)
         return out
 
     # ------------------------------------------------------------------ passes
@@ -597,14 +622,17 @@ class TensorCircuit(nn.Module):
         if force_use_fp32:
             prec = "fp32"
         origin_inputs = inputs
+        missing_mask = kwargs.get("missing_mask")
+        batch_first = kwargs.get("_batch_first", True)
         with torch.no_grad():
             B = inputs.size(0)
             self._ensure_buffers(B)
             if input_layer_fn is None:
-                data = self._to_vb(inputs)
-                self._data_vb = data
+                data = self._to_vb(inputs, missing_mask, batch_first)
+                self._data_vb, self._data_marginalized = data, missing_mask is not None
+                kw = {k: v for k, v in kwargs.items() if k not in ("missing_mask", "_batch_first")}
                 for layer in self.input_layer_group:
-                    layer(data, self.node_mars, **kwargs)
+                    layer(data, self.node_mars, **kw)
             elif isinstance(input_layer_fn, str):
                 for layer in self.input_layer_group:
                     getattr(layer, input_layer_fn)(inputs, self.node_mars, **kwargs)
@@ -701,7 +729,11 @@ class TensorCircuit(nn.Module):
                  propagation_alg: Union[str, Sequence[str]] = "LL", logspace_flows: bool = False,
                  negate_pflows: bool = False, precision: Optional[str] = None, force_use_fp32: bool = False,
                  **kwargs):
-        """Node flows and accumulated parameter flows (``tensorcircuit.py:344-538``)."""
+        """Node flows and accumulated parameter flows (``tensorcircuit.py:344-538``).  With
+        ``missing_mask`` (as in ``forward``; when ``inputs`` is None, the forward's mask holds), a
+        marginalized input entry's flow is spread over its categories by ``theta``."""
+        missing_mask = kwargs.get("missing_mask")
+        batch_first = kwargs.get("_batch_first", True)
         assert propagation_alg == "LL" and not logspace_flows and not negate_pflows, \
             "only LL flows in linear space are implemented"
         assert sum_layer_pre_backward_callback is None and sum_layer_post_backward_callback is None, \
@@ -752,10 +784,14 @@ class TensorCircuit(nn.Module):
                 self._pf_fresh = False
 
             if input_layer_fn is None:
-                data = self._data_vb if inputs is None else self._to_vb(inputs)
+                if inputs is None:
+                    data, marg = self._data_vb, self._data_marginalized
+                else:
+                    data, marg = self._to_vb(inputs, missing_mask, batch_first), missing_mask is not None
+                kw = {k: v for k, v in kwargs.items() if k not in ("missing_mask", "_batch_first")}
                 for layer in self.input_layer_group:
                     if compute_param_flows:
-                        layer.backward(data, self.node_flows, self.node_mars, **kwargs)
+                        layer.backward(data, self.node_flows, self.node_mars, marginalized=marg, **kw)
             elif isinstance(input_layer_fn, str):
                 for layer in self.input_layer_group:
                     getattr(layer, input_layer_fn)(inputs, self.node_flows, self.node_mars, **kwargs)

@@ -1,3 +1,4 @@
+from .marginal import marginal
 from .sample import sample
 
-__all__ = ["sample"]
+__all__ = ["marginal", "sample"]
