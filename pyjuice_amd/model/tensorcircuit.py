@@ -810,11 +810,21 @@ class TensorCircuit(nn.Module):
     def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False,
                       step_size_rescaling: bool = False, use_cudagraph: bool = True):
         """EM update from the accumulated parameter flows (``tensorcircuit.py:549-590``):
-        input layers, then tied-flow fusing, then the sum parameters."""
-        assert not step_size_rescaling, "step-size rescaling (top-down probabilities) is not implemented"
+        input layers, then tied-flow fusing, then the sum parameters.  ``step_size_rescaling``:
+        the flows become step_size / (accumulated root flow) times themselves plus (1 - step_size)
+        times the current model's expected flows (theta * top-down probability), and the update
+        then takes step 1 (``tensorcircuit.py:566-578``)."""
+        assert not step_size_rescaling or self._cum_flow > 0.0, \
+            "Please perform a backward pass before calling `mini_batch_em`."
         assert 0.0 < step_size <= 1.0, "`step_size` should be between 0 and 1."
         if self.param_flows is None:
             self.init_param_flows(0.0)
+        if step_size_rescaling:
+            from .top_down import eval_top_down_probs
+            self.init_param_flows(flows_memory=step_size / self._cum_flow)
+            eval_top_down_probs(self, update_pflow=True, scale=1.0 - step_size, use_cudagraph=use_cudagraph)
+            self._cum_flow = 0.0
+            step_size = 1.0
         with torch.no_grad():
             for layer in self.input_layer_group:
                 layer.mini_batch_em(step_size=step_size, pseudocount=pseudocount, keep_zero_params=keep_zero_params)

@@ -432,3 +432,85 @@ def test_marginal_missing_mask(name, mask_kind, precision, device):
             else:
                 want = pflows[n]
             assert U.rel_err(got, want) <= t_flow, (n, U.rel_err(got, want))
+
+
+def _all_assignments(cards):
+    import itertools
+    return np.asarray(list(itertools.product(*[range(k) for k in cards])), dtype=np.int64)
+
+
+@pytest.mark.parametrize("name", ["simple_model", "hclt_cat_tiny", "hmm_tiny", "hclt_bern16"])
+def test_top_down_probs_and_rescaled_em(name, device):
+    """Top-down probabilities, their parameter-flow update, the partition function and
+    step-size-rescaled EM (``top_down_prob.py:290-402``, ``tensorcircuit.py:566-578``) against
+    enumeration on the FP64 oracle: the top-down probability of a node is its flow averaged over
+    every assignment x weighted by p(x), and theta * probability is the expected parameter flow."""
+    import pyjuice_amd as juice
+    U.say(f"[top-down] {name}")
+    if name == "hclt_cat_tiny":
+        ns, _, cards = U.hclt_categorical(num_vars=4, K=3, L=32)
+        kw = dict()
+    elif name == "hmm_tiny":
+        ns, cards = U.hmm_small(T=3, L=32, V=4)
+        kw = dict()
+    else:
+        ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.to(device)
+    bern = any(layer._kind() == 1 for layer in pc.input_layer_group)
+    Xall = _all_assignments(cards)
+    vals = dag_eval.forward(ns, Xall)
+    px = np.exp(vals[ns][0])
+    assert abs(px.sum() - 1.0) < 1e-6      # parameters normalized in FP32
+    flows, pf_exp = dag_eval.backward(ns, Xall, vals, root_flows=px[None, :])
+
+    # partition function of the normalized circuit
+    assert abs(float(juice.model.eval_partition_fn(pc)[0])) <= 1e-5
+
+    # top-down probabilities and their parameter-flow update, from zero flows
+    pc.init_param_flows(0.0)
+    s = 0.375
+    juice.model.eval_top_down_probs(pc, update_pflow=True, scale=s)
+    tdp = pc._single_node_flows.cpu().numpy().astype(np.float64)
+    pc.update_param_flows(origin_ns_only=False)
+    for n in node_iterator(ns):
+        if n.is_sum() or n.is_input():
+            a, b = n._output_ind_range
+            assert U.rel_err(tdp[a:b], flows[n].sum(axis=1)) <= 1e-5, n
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            want = pf_exp[n] + sum((pf_exp[m] for m in node_iterator(ns) if m is not n and m.is_tied()
+                                    and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range), 0)
+            assert U.rel_err(got, s * want) <= 1e-5, n
+    if not bern:
+        for layer in pc.input_layer_group:
+            pfi = layer.param_flows.detach().cpu().numpy().astype(np.float64)
+            for n in layer.nodes:
+                a, b = n._param_flow_range
+                got = pfi[a:b].reshape(n.num_nodes, n.dist.num_param_flows())
+                want = sum(pf_exp[m] for m in layer.nodes if m._param_flow_range == n._param_flow_range)
+                assert U.rel_err(got, s * want) <= 1e-5, n
+
+    # step-size-rescaled EM on a mini-batch
+    X = U.random_data(cards, 16, seed=31)
+    xt = torch.from_numpy(X).to(device)
+    pc.init_param_flows(0.0)
+    pc._cum_flow = 0.0
+    pc(xt)
+    pc.backward(xt, flows_memory=0.0)
+    vb = dag_eval.forward(ns, X)
+    _, pf_b = dag_eval.backward(ns, X, vb)
+    step, pcount = 0.25, 0.1
+    pc.mini_batch_em(step_size=step, pseudocount=pcount, step_size_rescaling=True)
+    assert pc._cum_flow == 0.0
+    comb = {n: step / 16 * pf_b[n] + (1.0 - step) * pf_exp[n] for n in pf_b}
+    if bern:      # Bernoulli inputs take no top-down flows (no masked flow function, as the reference)
+        for layer in pc.input_layer_group:
+            for n in layer.nodes:
+                comb[n] = step / 16 * pf_b[n]
+    new = dag_eval.em_update(ns, comb, step_size=1.0, pseudocount=pcount)
+    pc.update_parameters()
+    for n, p in new.items():
+        got = U.dense_pf_from_blocks(n, n._params.numpy()) if n.is_sum() else \
+            n._params.numpy().astype(np.float64).reshape(p.shape)
+        assert np.abs(got - p).max() <= 2e-6, (n, np.abs(got - p).max())
