@@ -9,7 +9,7 @@ from .nodes import (inputs, multiply, summate, set_block_size, structural_proper
                     CircuitNodes, InputNodes, ProdNodes, SumNodes)
 from .layer import InputLayer, ProdLayer, SumLayer, LayerGroup
 from .model import TensorCircuit, compile, log_likelihood
-from . import structures, optim, queries, transformations, autotune
+from . import structures, optim, queries, transformations, autotune, io
 
 __all__ = ["compile", "TensorCircuit", "log_likelihood", "inputs", "multiply", "summate", "set_block_size",
            "structural_properties", "distributions", "structures", "optim", "queries", "transformations",
