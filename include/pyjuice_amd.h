@@ -175,6 +175,22 @@ int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* 
                       const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
                       int child_run, int precision, pjamd_stream_t stream);
 
+/* pjamd_sum_forward with each node block's reduction split over n_splits workgroups (BF16 / FP16
+ * operands: precision 1 or 2), for layers of few tiles and long child lists: split s takes an
+ * equal share of the 128-child stages and stores its partial sum and column max into workspace,
+ * then a second kernel combines the partials into node_mars (their log-sum-exp).  workspace:
+ * float [n_splits][n_blocks * bs][B] followed by float [n_splits][n_blocks * bs / 64][B]; calls
+ * sharing a workspace must not overlap.  n_blocks * bs < 65536.  Requires child_run with
+ * PJAMD_PARAMS_ALIGNED and a run of >= 128, bs a multiple of 64 and B % 4 == 0, else returns
+ * PJAMD_EUNSUPPORTED.  Same results as pjamd_sum_forward up to the rounding of the 16-bit
+ * operands (each split scales by its own column max).  Replaces SumLayer.forward
+ * (src/pyjuice/layer/sum_layer.py:220-262), whose block-sparse kernel runs one program per node
+ * tile over the whole child list (sum_layer.py:454-568). */
+int pjamd_sum_forward_split(float* node_mars, const float* element_mars, const float* params, const int* nids,
+                            const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs,
+                            int64_t B, int child_run, int n_splits, float* workspace, int precision,
+                            pjamd_stream_t stream);
+
 /* pjamd_sum_forward with the product layer below it fused in (BF16 / FP16 operands: precision 1
  * or 2).  The partition's element rows are not read from element_mars but summed from the node
  * rows of the products' children, and stored into element_mars on the way (the flow backward

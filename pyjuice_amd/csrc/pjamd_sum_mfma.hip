@@ -805,6 +805,122 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
     }
 }
 
+// Forward with the reduction split over workgroups (16-byte RUN form, 64 x 64 tiles): for layers
+// with few tiles and long child lists (an HMM's single 1024-row node block over 1024 children is
+// 64 tiles of 8 dependent stages), workgroup (tile, split s) reduces stages [s P, s P + P) and
+// stores its partial sum (relative to its own column max) and the max into the workspace;
+// sum_fwd_split_combine then forms node = log(sum_s acc_s exp(m_s - M)) + M, M = max_s m_s.  The
+// combine is a kernel of its own: a last-arriving workgroup would need a device-scope fence per
+// tile, which on this chip writes back the XCD's L2 and costs more than the launch.
+// Workspace: ws_acc [nsplit][n_blocks * bs][B] floats, ws_max [nsplit][n_blocks * mtiles][B] floats.
+template <typename T>
+__global__ __launch_bounds__(256) void sum_fwd_vec_split(float* __restrict__ node_mars, const float* __restrict__ emars,
+                                                         const float* __restrict__ params, const int* __restrict__ nids,
+                                                         const int* __restrict__ cids, const int* __restrict__ pids,
+                                                         const int* __restrict__ nch, int K, int64_t B, int mtiles,
+                                                         int btiles, int nsplit, float* __restrict__ ws_acc,
+                                                         float* __restrict__ ws_max) {
+    constexpr int TM = 64, TN = 64, LD = TK + MF<T>::PAD;
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[kVecGroups][64], mrun[TN], msc[TN];
+
+    const int u = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = u / nsplit, sp = u % nsplit;
+    const int m0 = (tile % mtiles) * TM;
+    const int j = tile / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((tile / mtiles) % btiles) * TN;
+    const int bs = mtiles * TM;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % 2, wn = w / 2;
+    const int n = nch[j];
+    const int nst = (n + TK - 1) / TK, per = (nst + nsplit - 1) / nsplit;
+    const int kbeg = sp * per * TK;
+    const int kend = min(n, (sp + 1) * per * TK);
+    const int* crow = cids + (int64_t)j * K;
+    const int* prow = pids + (int64_t)j * K;
+    const int g = tid / 16, q = tid % 16;
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;
+    const int64_t bqc = bok ? bq : B - 4;
+    const int iq = tid % 16, kg = tid / 16;
+    const int mycol = wn * 32 + (lane & 31);
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+
+    float4 ev[kVecRows], av[kVecRows];
+    auto load = [&](int k0) {
+        const int64_t c0 = crow[k0], p0 = prow[k0];
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int k = 8 * g + r;
+            const int kc = (k0 + k < kend) ? k : 0;
+            ev[r] = *reinterpret_cast<const float4*>(emars + (c0 + kc) * B + bqc);
+            if (!(k0 + k < kend && bok)) ev[r] = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        }
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int k = 8 * kg + r;
+            const int kc = (k0 + k < kend) ? k : 0;
+            av[r] = *reinterpret_cast<const float4*>(params + p0 + (int64_t)kc * bs + m0 + 4 * iq);
+            if (!(k0 + k < kend)) av[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    if (kbeg < kend) load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        float mn[4];
+        vec_stage_max(ev, red, mrun, msc, g, q, mn);
+        vec_stage_b<T, LD>(ev, mn, Bs, g, q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x[8];
+#pragma unroll
+            for (int r = 0; r < kVecRows; ++r)
+                x[r] = c == 0 ? av[r].x : c == 1 ? av[r].y : c == 2 ? av[r].z : av[r].w;
+            Pack8<T>::store(&As[4 * iq + c][8 * kg], x);
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < kend) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    // this split's partial
+    const int64_t rows = (int64_t)gridDim.x / nsplit / btiles * TM;   // n_blocks * bs
+    const int64_t b = b0 + mycol;
+    const int64_t row0 = (int64_t)j * bs + m0 + wm * 32;
+    if (b < B) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) ws_acc[((int64_t)sp * rows + row0 + acc_row(r, lane)) * B + b] = acc[r];
+        if (wm == 0 && lane < 32)
+            ws_max[((int64_t)sp * (rows / TM) + (int64_t)j * mtiles + m0 / TM) * B + b] = mrun[mycol];
+    }
+}
+
+// node_mars[nids[j] + i][b] from the partials of sum_fwd_vec_split; one thread per (row, column).
+__global__ __launch_bounds__(256) void sum_fwd_split_combine(float* __restrict__ node_mars, const int* __restrict__ nids,
+                                                             const float* __restrict__ ws_acc,
+                                                             const float* __restrict__ ws_max, int nsplit, int bs,
+                                                             int64_t rows, int64_t B) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = blockIdx.y;
+    if (b >= B) return;
+    const int64_t mrow = row / 64;
+    float M = kNegInf;
+    for (int s = 0; s < nsplit; ++s) M = fmaxf(M, ws_max[((int64_t)s * (rows / 64) + mrow) * B + b]);
+    float S = 0.0f;
+    if (M != kNegInf) {
+        for (int s = 0; s < nsplit; ++s) {
+            const float ms = ws_max[((int64_t)s * (rows / 64) + mrow) * B + b];
+            S += (ms == kNegInf ? 0.0f : __expf(ms - M)) * ws_acc[((int64_t)s * rows + row) * B + b];
+        }
+    }
+    const int64_t j = row / bs, i = row % bs;
+    node_mars[((int64_t)nids[j] + i) * B + b] = (S > 0.0f && M != kNegInf) ? __logf(S) + M : kNegInf;
+}
+
 // Forward with 16-byte loads for node blocks of 32 rows (32 x 128 tiles) and for child runs
 // shorter than a stage (runs of >= 16 consecutive element rows with parameters bs apart, e.g. the
 // child blocks of 32 of a block-size-32 layer).  The block's index row is staged in LDS (in chunks
@@ -1450,6 +1566,32 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     return PJAMD_OK;
 }
 
+template <typename T>
+int launch_fwd_split(float* node_mars, const float* emars, const float* params, const int* nids, const int* cids,
+                     const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t B, int child_run,
+                     int nsplit, float* ws, hipStream_t stream) {
+    const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
+    child_run &= ~PJAMD_PARAMS_ALIGNED;
+    if (!(aligned && child_run >= TK && child_run % TK == 0 && bs >= 64 && bs % 64 == 0 && B % 4 == 0))
+        return PJAMD_EUNSUPPORTED;
+    const int mtiles = bs / 64;
+    const int64_t btiles = cdiv(B, 64);
+    const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
+    PJAMD_REQUIRE(grid_fits(tiles * nsplit));
+    float* ws_acc = ws;
+    float* ws_max = ws + (int64_t)nsplit * n_blocks * bs * B;
+    sum_fwd_vec_split<T><<<(unsigned)(tiles * nsplit), 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids,
+                                                                          nch, K, B, mtiles, (int)btiles, nsplit,
+                                                                          ws_acc, ws_max);
+    PJAMD_LAUNCH_CHECK();
+    const int64_t rows = (int64_t)n_blocks * bs;
+    PJAMD_REQUIRE(rows < 65536);
+    sum_fwd_split_combine<<<dim3(cdiv(B, 256), (unsigned)rows), 256, 0, stream>>>(node_mars, nids, ws_acc, ws_max,
+                                                                                  nsplit, bs, rows, B);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
 // Sum forward with the product layer below fused in (sum_fwd_vec with KP > 0).  Only the 16-byte
 // RUN form exists: the caller checks the layout (runs of >= TK children inside one product block)
 // and calls the plain product and sum forwards otherwise.
@@ -1656,6 +1798,21 @@ int pjamd_sum_forward_mfma(float* node_mars, const float* element_mars, const fl
                                     child_run, stream);
     return launch_fwd<float>(node_mars, element_mars, params, nids, cids, pids, nch, n_blocks, K, bs, B, child_run,
                              stream);
+}
+
+PJAMD_EXPORT int pjamd_sum_forward_split(float* node_mars, const float* element_mars, const float* params,
+                                         const int* nids, const int* cids, const int* pids, const int* n_chs,
+                                         int n_blocks, int K, int bs, int64_t B, int child_run, int n_splits,
+                                         float* workspace, int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && n_splits >= 2 && n_splits <= 16);
+    if (n_blocks == 0) return PJAMD_OK;
+    if (precision == 1)
+        return launch_fwd_split<__bf16>(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs, B,
+                                        child_run, n_splits, workspace, stream);
+    if (precision == 2)
+        return launch_fwd_split<_Float16>(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs,
+                                          B, child_run, n_splits, workspace, stream);
+    return PJAMD_EUNSUPPORTED;
 }
 
 PJAMD_EXPORT int pjamd_sum_forward_prod(float* node_mars, float* element_mars, const float* params, const int* nids,

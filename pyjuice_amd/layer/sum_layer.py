@@ -173,6 +173,11 @@ class SumLayer(Layer, nn.Module):
     # Every kernel bounds its reduction by the block's own child / parent count, so the padding
     # costs index memory only, and blocks of uneven fan-in (pruned circuits) share one grid.
     merge_partitions = True
+    # Forward launches of fewer than split_max_tiles tiles whose node blocks have at least
+    # split_min_stages stages of 128 children split each reduction over workgroups
+    # (pjamd_sum_forward_split; BF16 / FP16 operands, 16-byte layout).
+    split_max_tiles = 128
+    split_min_stages = 4
 
     def build_device_tables(self):
         """int32 launch tables (``_dev``) of the forward and backward partitions on the layer's
@@ -180,6 +185,7 @@ class SumLayer(Layer, nn.Module):
         has one forward table and one backward table per child block size, rows ordered by
         decreasing fan-in so the longest tiles are dispatched first."""
         device = self.device
+        self.__dict__.pop("_split_cache", None)
         fw, fw_run, fw_al = [], [], []
         for p in range(self.num_fw_partitions):
             n = self.partitioned_nids[p]
@@ -236,6 +242,14 @@ class SumLayer(Layer, nn.Module):
         for e, ((nids, cids, pids, pfids, nch, tot), run, al) in enumerate(zip(self._dev["fw"], self._fw_child_run,
                                                                                self._fw_aligned)):
             run = run | (PARAMS_ALIGNED if al else 0)
+            split = self._split_plan(e, nids.numel(), nch, run & ~PARAMS_ALIGNED, al, prec, B)
+            if split is not None and fused is None:
+                nsplit, ws = split
+                algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
+                N.call("pjamd_sum_forward_split", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
+                       N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, nsplit,
+                       N.ptr(ws), prec, N.stream(), algo=algo)
+                continue
             if fused is not None:
                 tab, kp, reads, writes = fused[e]
                 # product rows gathered and stored, node rows written, parameters read once
@@ -250,6 +264,27 @@ class SumLayer(Layer, nn.Module):
             N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
                    N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, prec,
                    N.stream(), algo=algo)
+
+    def _split_plan(self, e: int, n_blocks: int, nch: torch.Tensor, run: int, aligned: bool, prec: int, B: int):
+        """``(n_splits, workspace)`` for forward table ``e`` at batch ``B`` when the split reduction
+        applies, else None; the workspace is cached per (table, batch)."""
+        bs = self.block_size
+        if not (prec in (1, 2) and aligned and run >= 128 and run % 128 == 0 and bs % 64 == 0 and B % 4 == 0):
+            return None
+        tiles = n_blocks * (bs // 64) * -(-B // 64)
+        key = (e, B)
+        cache = self.__dict__.setdefault("_split_cache", dict())
+        if key not in cache:
+            nst = -(-int(nch.max()) // 128)
+            plan = None
+            if tiles < self.split_max_tiles and nst >= self.split_min_stages and n_blocks * bs < 65536:
+                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16))
+                rows = n_blocks * bs
+                ws = torch.empty([nsplit * rows * B + nsplit * (rows // 64) * B], dtype=torch.float32,
+                                 device=nch.device)
+                plan = (nsplit, ws)
+            cache[key] = plan
+        return cache[key]
 
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
                  element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,

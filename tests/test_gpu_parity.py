@@ -514,3 +514,40 @@ def test_top_down_probs_and_rescaled_em(name, device):
         got = U.dense_pf_from_blocks(n, n._params.numpy()) if n.is_sum() else \
             n._params.numpy().astype(np.float64).reshape(p.shape)
         assert np.abs(got - p).max() <= 2e-6, (n, np.abs(got - p).max())
+
+
+@pytest.mark.parametrize("L", [512, 1024])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_split_reduction_forward(L, precision, device):
+    """pjamd_sum_forward_split (a node block's reduction split over workgroups, the last one
+    combining the partial log-sum-exps) against one workgroup per tile, and the log-likelihood
+    against the FP64 oracle; two calls in a row give the same result."""
+    from pyjuice_amd.layer.sum_layer import SumLayer
+    U.say(f"[split forward] HMM L={L} {precision}")
+    ns, cards = U.hmm_small(T=4, L=L, V=16)
+    pc = TensorCircuit(ns, precision=precision)
+    pc.to(device)
+    X = U.random_data(cards, 64, seed=41)
+    xt = torch.from_numpy(X).to(device)
+    out = []
+    saved = SumLayer.split_max_tiles
+    try:
+        for max_tiles in (saved, saved, 0):
+            SumLayer.split_max_tiles = max_tiles
+            for g in pc.inner_layer_groups:
+                for layer in g:
+                    layer.__dict__.pop("_split_cache", None)
+            out.append(pc(xt).detach().double().cpu().numpy()[:, 0])
+            split_used = any(v is not None for g in pc.inner_layer_groups for layer in g
+                             for v in layer.__dict__.get("_split_cache", {}).values())
+            assert split_used == (max_tiles > 0)
+    finally:
+        SumLayer.split_max_tiles = saved
+    assert np.array_equal(out[0], out[1])
+    # 16-bit operands are rounded relative to each split's own column max, so the two forms
+    # differ at operand precision; both are held to the 16-bit contract against FP64 below
+    assert np.abs(out[0] - out[2]).max() <= 1e-3 * len(cards)
+    pc.update_parameters()
+    ref = dag_eval.forward(ns, X)[ns][0]
+    assert np.abs(out[0] - ref).max() <= 1e-3 * len(cards)
+    assert np.abs(out[2] - ref).max() <= 1e-3 * len(cards)
