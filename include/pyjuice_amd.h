@@ -242,6 +242,21 @@ int pjamd_sum_backward_ele_to_children(float* node_flows, const float* node_mars
                                        const int* n_pars, const int* ch_nodes, int Kc, int n_cblocks, int KP, int cbs,
                                        int bs, int64_t B, int flags, int precision, pjamd_stream_t stream);
 
+/* pjamd_sum_backward_ele_to_children with the reduction over parent rows split over n_splits
+ * workgroups (BF16 / FP16 operands, parent blocks of >= 128 rows, child blocks a multiple of 64
+ * rows, flags bit 2 set, B % 4 == 0; else PJAMD_EUNSUPPORTED), for layers of few tiles and long
+ * parent lists: each split stores its partial sums and column maxima into workspace (laid out as
+ * for pjamd_sum_forward_split, over the n_cblocks * cbs child rows, < 65536), and a second kernel
+ * combines them and writes the children's flows.  Same results as
+ * pjamd_sum_backward_ele_to_children up to the rounding of the 16-bit operands.  Replaces
+ * SumLayer._backward_block_sparse_ele_flows (sum_layer.py:1891-2048) followed by
+ * ProdLayer.backward (prod_layer.py:206-261). */
+int pjamd_sum_backward_ele_split(float* node_flows, const float* node_mars, const float* element_mars,
+                                 const float* params, const int* chids, const int* parids, const int* parpids,
+                                 const int* n_pars, const int* ch_nodes, int Kc, int n_cblocks, int KP, int cbs, int bs,
+                                 int64_t B, int flags, int n_splits, float* workspace, int precision,
+                                 pjamd_stream_t stream);
+
 /* param_flows[pfids[j][k] + i] += sum_b params[pids[j][k] + i] * node_flows[nids[j] + i][b] *
  * exp(element_mars[cids[j][k]][b] - node_mars[nids[j] + i][b]) for one forward partition.
  * flags bit 0: flows hold log(flow) - mar; bit 1: slots are shared (atomic adds); bit 2: negate;

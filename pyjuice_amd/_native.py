@@ -49,6 +49,8 @@ _SIGNATURES = {
     "pjamd_input_missing_flows": [_i32, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i64, _i64, _i64,
                                   _c_p],
     "pjamd_sum_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
+    "pjamd_sum_backward_ele_split": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i32,
+                                     _i32, _i64, _i32, _i32, _c_p, _i32, _c_p],
     "pjamd_sum_forward_split": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32,
                                 _c_p, _i32, _c_p],
     "pjamd_sum_forward_prod": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,

@@ -921,6 +921,141 @@ __global__ __launch_bounds__(256) void sum_fwd_split_combine(float* __restrict__
     node_mars[((int64_t)nids[j] + i) * B + b] = (S > 0.0f && M != kNegInf) ? __logf(S) + M : kNegInf;
 }
 
+// Element flows with the reduction over parent rows split over workgroups (16-byte form, parent
+// blocks of >= TK rows, child blocks of >= 64 rows), the counterpart of sum_fwd_vec_split for the
+// flow backward of few-tile layers; sum_ele_split_combine then forms each child row's flow and
+// stores it into the children of the product row (the fused product backward).  Workspace as
+// sum_fwd_vec_split, over child rows.
+template <typename T>
+__global__ __launch_bounds__(256) void sum_ele_vec_split(const float* nflows, const float* __restrict__ nmars,
+                                                         const float* __restrict__ params, const int* __restrict__ parids,
+                                                         const int* __restrict__ parpids, const int* __restrict__ npar,
+                                                         int KP, int lbs, int64_t B, int mtiles, int btiles,
+                                                         bool modified, int nsplit, float* __restrict__ ws_acc,
+                                                         float* __restrict__ ws_max) {
+    constexpr int TM = 64, TN = 64, LD = TK + MF<T>::PAD;
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
+    __shared__ float red[kVecGroups][64], mrun[TN], msc[TN];
+
+    const int u = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = u / nsplit, sp = u % nsplit;
+    const int m0 = (tile % mtiles) * TM;
+    const int j = tile / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((tile / mtiles) % btiles) * TN;
+    const int bs = 1 << lbs;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w % 2, wn = w / 2;
+    const int kf = npar[j] << lbs;
+    const int nst = kf / TK, per = (nst + nsplit - 1) / nsplit;
+    const int kbeg = sp * per * TK;
+    const int kend = min(kf, (sp + 1) * per * TK);
+    const int* parrow = parids + (int64_t)j * KP;
+    const int* pprow = parpids + (int64_t)j * KP;
+    const int g = tid / 16, q = tid % 16;
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;
+    const int64_t bqc = bok ? bq : B - 4;
+    const int kq = tid % 32, ig = tid / 32;
+    const int mycol = wn * 32 + (lane & 31);
+    if (tid < TN) mrun[tid] = kNegInf;
+    floatx16 acc = {0};
+
+    float4 lv[kVecRows], av[kVecRows];
+    auto load = [&](int k0) {
+        const int kb = k0 >> lbs, k_in = k0 & (bs - 1);
+        const int64_t rbase = (int64_t)parrow[kb] + k_in + 8 * g;
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            const int64_t o = (rbase + r) * B + bqc;
+            float4 f = *reinterpret_cast<const float4*>(nflows + o);
+            if (!modified) {
+                const float4 m = *reinterpret_cast<const float4*>(nmars + o);
+                f.x = m.x == kNegInf ? kNegInf : __logf(f.x) - m.x;
+                f.y = m.y == kNegInf ? kNegInf : __logf(f.y) - m.y;
+                f.z = m.z == kNegInf ? kNegInf : __logf(f.z) - m.z;
+                f.w = m.w == kNegInf ? kNegInf : __logf(f.w) - m.w;
+            }
+            lv[r] = bok ? f : make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+        }
+        const float* pp = params + (int64_t)pprow[kb] + (int64_t)(m0 + ig) * bs + k_in + 4 * kq;
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) av[r] = *reinterpret_cast<const float4*>(pp + (int64_t)(8 * r) * bs);
+    };
+    if (kbeg < kend) load(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        float mn[4];
+        vec_stage_max(lv, red, mrun, msc, g, q, mn);
+        vec_stage_b<T, LD>(lv, mn, Bs, g, q);
+#pragma unroll
+        for (int r = 0; r < kVecRows; ++r) {
+            typedef T T4 __attribute__((ext_vector_type(4)));
+            T4 x;
+            x[0] = MF<T>::cvt(av[r].x);
+            x[1] = MF<T>::cvt(av[r].y);
+            x[2] = MF<T>::cvt(av[r].z);
+            x[3] = MF<T>::cvt(av[r].w);
+            *reinterpret_cast<T4*>(&As[ig + 8 * r][4 * kq]) = x;
+        }
+        const float s = msc[mycol];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] *= s;
+        __syncthreads();
+        if (k0 + TK < kend) load(k0 + TK);
+        acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
+        __syncthreads();
+    }
+    const int64_t rows = (int64_t)gridDim.x / nsplit / btiles * TM;   // n_cblocks * cbs
+    const int64_t b = b0 + mycol;
+    const int64_t row0 = (int64_t)j * mtiles * TM + m0 + wm * 32;
+    if (b < B) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) ws_acc[((int64_t)sp * rows + row0 + acc_row(r, lane)) * B + b] = acc[r];
+        if (wm == 0 && lane < 32)
+            ws_max[((int64_t)sp * (rows / TM) + (int64_t)j * mtiles + m0 / TM) * B + b] = mrun[mycol];
+    }
+}
+
+// Child-row flows from the partials of sum_ele_vec_split, written into the product's children
+// (ch_nodes as pjamd_sum_backward_ele_to_children); one thread per (child row, column).
+__global__ __launch_bounds__(256) void sum_ele_split_combine(float* __restrict__ child_flows,
+                                                             const float* __restrict__ nmars,
+                                                             const float* __restrict__ emars,
+                                                             const int* __restrict__ chids,
+                                                             const int* __restrict__ ch_nodes, int Kc,
+                                                             bool mod_children, const float* __restrict__ ws_acc,
+                                                             const float* __restrict__ ws_max, int nsplit, int cbs,
+                                                             int64_t rows, int64_t B) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = blockIdx.y;
+    if (b >= B) return;
+    const int64_t mrow = row / 64;
+    float M = kNegInf;
+    for (int s = 0; s < nsplit; ++s) M = fmaxf(M, ws_max[((int64_t)s * (rows / 64) + mrow) * B + b]);
+    float S = 0.0f;
+    if (M != kNegInf) {
+        for (int s = 0; s < nsplit; ++s) {
+            const float ms = ws_max[((int64_t)s * (rows / 64) + mrow) * B + b];
+            S += (ms == kNegInf ? 0.0f : __expf(ms - M)) * ws_acc[((int64_t)s * rows + row) * B + b];
+        }
+    }
+    const int64_t j = row / cbs, i = row % cbs;
+    const float e = emars[((int64_t)chids[j] + i) * B + b];
+    const float v = (M == kNegInf || e == kNegInf) ? 0.0f : S * __expf(e + M);
+    const int* crow = ch_nodes + j * Kc;
+    for (int k = 0; k < Kc; ++k) {
+        const int code = crow[k];
+        if (code == 0) continue;
+        const int64_t o = ((int64_t)(code >> 1) + i) * B + b;
+        if ((code & 1) && mod_children) {
+            const float m = nmars[o];
+            child_flows[o] = (m == kNegInf) ? kNegInf : __logf(v) - m;
+        } else {
+            child_flows[o] = v;
+        }
+    }
+}
+
 // Forward with 16-byte loads for node blocks of 32 rows (32 x 128 tiles) and for child runs
 // shorter than a stage (runs of >= 16 consecutive element rows with parameters bs apart, e.g. the
 // child blocks of 32 of a block-size-32 layer).  The block's index row is staged in LDS (in chunks
@@ -1592,6 +1727,30 @@ int launch_fwd_split(float* node_mars, const float* emars, const float* params, 
     return PJAMD_OK;
 }
 
+template <typename T>
+int launch_ele_split(float* node_flows, const float* nmars, const float* emars, const float* params,
+                     const int* chids, const int* parids, const int* parpids, const int* npar, const int* ch_nodes,
+                     int Kc, int n, int KP, int cbs, int bs, int64_t B, bool mod, bool modc, bool aligned, int nsplit,
+                     float* ws, hipStream_t stream) {
+    const int lbs = log2_exact(bs);
+    if (!(aligned && lbs >= 0 && bs >= TK && cbs >= 64 && cbs % 64 == 0 && B % 4 == 0)) return PJAMD_EUNSUPPORTED;
+    const int mtiles = cbs / 64;
+    const int64_t btiles = cdiv(B, 64);
+    const int64_t tiles = (int64_t)n * mtiles * btiles;
+    const int64_t rows = (int64_t)n * cbs;
+    PJAMD_REQUIRE(grid_fits(tiles * nsplit) && rows < 65536);
+    float* ws_acc = ws;
+    float* ws_max = ws + (int64_t)nsplit * rows * B;
+    sum_ele_vec_split<T><<<(unsigned)(tiles * nsplit), 256, 0, stream>>>(node_flows, nmars, params, parids, parpids,
+                                                                          npar, KP, lbs, B, mtiles, (int)btiles, mod,
+                                                                          nsplit, ws_acc, ws_max);
+    PJAMD_LAUNCH_CHECK();
+    sum_ele_split_combine<<<dim3(cdiv(B, 256), (unsigned)rows), 256, 0, stream>>>(
+        node_flows, nmars, emars, chids, ch_nodes, Kc, modc, ws_acc, ws_max, nsplit, cbs, rows, B);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
 // Sum forward with the product layer below fused in (sum_fwd_vec with KP > 0).  Only the 16-byte
 // RUN form exists: the caller checks the layout (runs of >= TK children inside one product block)
 // and calls the plain product and sum forwards otherwise.
@@ -1812,6 +1971,26 @@ PJAMD_EXPORT int pjamd_sum_forward_split(float* node_mars, const float* element_
     if (precision == 2)
         return launch_fwd_split<_Float16>(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs,
                                           B, child_run, n_splits, workspace, stream);
+    return PJAMD_EUNSUPPORTED;
+}
+
+PJAMD_EXPORT int pjamd_sum_backward_ele_split(float* node_flows, const float* node_mars, const float* element_mars,
+                                              const float* params, const int* chids, const int* parids,
+                                              const int* parpids, const int* n_pars, const int* ch_nodes, int Kc,
+                                              int n_cblocks, int KP, int cbs, int bs, int64_t B, int flags,
+                                              int n_splits, float* workspace, int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && Kc >= 1 && bs >= 1 && B > 0 && ch_nodes != nullptr && n_splits >= 2 &&
+                  n_splits <= 16);
+    if (n_cblocks == 0) return PJAMD_OK;
+    const bool mod = flags & 1, modc = (flags & 2) != 0, aligned = (flags & 4) != 0;
+    if (precision == 1)
+        return launch_ele_split<__bf16>(node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
+                                        ch_nodes, Kc, n_cblocks, KP, cbs, bs, B, mod, modc, aligned, n_splits,
+                                        workspace, stream);
+    if (precision == 2)
+        return launch_ele_split<_Float16>(node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
+                                          ch_nodes, Kc, n_cblocks, KP, cbs, bs, B, mod, modc, aligned, n_splits,
+                                          workspace, stream);
     return PJAMD_EUNSUPPORTED;
 }
 

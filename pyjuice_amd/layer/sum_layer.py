@@ -286,6 +286,27 @@ class SumLayer(Layer, nn.Module):
             cache[key] = plan
         return cache[key]
 
+    def _split_plan_bk(self, q: int, n_cblocks: int, npar: torch.Tensor, cbs: int, aligned: bool, prec: int, B: int):
+        """``(n_splits, workspace)`` for the fused element-flow pass of backward table ``q`` at
+        batch ``B`` when the split reduction applies (parent blocks of >= 128 rows), else None."""
+        bs = self.block_size
+        if not (prec in (1, 2) and aligned and bs >= 128 and cbs % 64 == 0 and B % 4 == 0):
+            return None
+        tiles = n_cblocks * (cbs // 64) * -(-B // 64)
+        key = ("bk", q, B)
+        cache = self.__dict__.setdefault("_split_cache", dict())
+        if key not in cache:
+            nst = int(npar.max()) * bs // 128
+            plan = None
+            rows = n_cblocks * cbs
+            if tiles < self.split_max_tiles and nst >= self.split_min_stages and rows < 65536:
+                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16))
+                ws = torch.empty([nsplit * rows * B + nsplit * (rows // 64) * B], dtype=torch.float32,
+                                 device=npar.device)
+                plan = (nsplit, ws)
+            cache[key] = plan
+        return cache[key]
+
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
                  element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,
                  allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
@@ -320,11 +341,19 @@ class SumLayer(Layer, nn.Module):
             al = 4 if self._bk_aligned[q] else 0
             if child_tables is not None:
                 tab, n_blk, n_mod = child_tables[q]
+                split = self._split_plan_bk(q, chids.numel(), npar, cbs, bool(al), prec, B)
                 # parent rows (and marginals unless modified) + element rows read, one write per
                 # child row (+ its marginal when stored as log(flow) - mar), parameters once
                 n_rd = n_mod if mod else 0
                 algo = (4 * (totp * bs * (1 if mod else 2) + chids.numel() * cbs + (n_blk + n_rd) * cbs) * B
                         + 4 * totp * bs * cbs, 2 * totp * bs * cbs * B)
+                if split is not None:
+                    nsplit, ws = split
+                    N.call("pjamd_sum_backward_ele_split", N.ptr(node_flows), N.ptr(node_mars), N.ptr(element_mars),
+                           N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar), N.ptr(tab),
+                           tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
+                           mod | (2 if allow_modify_flows else 0) | al, nsplit, N.ptr(ws), prec, s, algo=algo)
+                    continue
                 N.call("pjamd_sum_backward_ele_to_children", N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids),
                        N.ptr(npar), N.ptr(tab), tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,

@@ -551,3 +551,48 @@ def test_split_reduction_forward(L, precision, device):
     ref = dag_eval.forward(ns, X)[ns][0]
     assert np.abs(out[0] - ref).max() <= 1e-3 * len(cards)
     assert np.abs(out[2] - ref).max() <= 1e-3 * len(cards)
+
+
+@pytest.mark.parametrize("L", [512, 1024])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_split_reduction_backward(L, precision, device):
+    """pjamd_sum_backward_ele_split (the fused element-flow pass with its parent reduction split
+    over workgroups, partials combined by a second kernel that writes the children's flows):
+    parameter flows against one workgroup per tile and against the FP64 oracle."""
+    from pyjuice_amd.layer.sum_layer import SumLayer
+    U.say(f"[split backward] HMM L={L} {precision}")
+    ns, cards = U.hmm_small(T=4, L=L, V=16)
+    pc = TensorCircuit(ns, precision=precision)
+    pc.to(device)
+    X = U.random_data(cards, 64, seed=43)
+    xt = torch.from_numpy(X).to(device)
+    res = dict()
+    saved = SumLayer.split_max_tiles
+    try:
+        for max_tiles in (saved, 0):
+            SumLayer.split_max_tiles = max_tiles
+            for g in pc.inner_layer_groups:
+                for layer in g:
+                    layer.__dict__.pop("_split_cache", None)
+            pc(xt)
+            pc.backward(xt, flows_memory=0.0)
+            used = any(v is not None for g in pc.inner_layer_groups for layer in g
+                       for k, v in layer.__dict__.get("_split_cache", {}).items() if k[0] == "bk")
+            assert used == (max_tiles > 0)
+            res[max_tiles] = (pc.param_flows.detach().double().cpu().numpy(),
+                              [l.param_flows.detach().double().cpu().numpy() for l in pc.input_layer_group])
+    finally:
+        SumLayer.split_max_tiles = saved
+    assert U.rel_err(res[saved][0], res[0][0]) <= 2e-2
+    for a, b in zip(res[saved][1], res[0][1]):
+        assert U.rel_err(a, b) <= 2e-2
+    # input parameter flows of the split run (they take the children flows the split pass wrote)
+    vals = dag_eval.forward(ns, X)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    got_in = res[saved][1]
+    for layer, pfi in zip(pc.input_layer_group, got_in):
+        for n in layer.nodes:
+            a, b = n._param_flow_range
+            got = pfi[a:b].reshape(n.num_nodes, n.dist.num_param_flows())
+            want = sum(pflows[m] for m in layer.nodes if m._param_flow_range == n._param_flow_range)
+            assert U.rel_err(got, want) <= 2e-2, n
