@@ -269,14 +269,17 @@ int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const flo
                            int flags, int precision, pjamd_stream_t stream);
 
 /* Number of int32 fields per row of the `desc` table below. */
-#define PJAMD_PAR_DESC_FIELDS 8
+#define PJAMD_PAR_DESC_FIELDS 9
 
 /* pjamd_sum_backward_par for many partitions (of many sum layers) in one launch, after the
  * whole flow backward.  nids / n_chs are the partitions' node-block tables concatenated,
  * cids / pids / pfids their index rows concatenated (each partition keeps its own width K).
- * desc[p][0..7] = {first tile, K, m-tiles per block (bs / tile_rows), child tiles per block,
- * block offset, index-row offset, first element row of the layer, shared-slot flag};
- * tile_desc[t] = p for each of the n_tiles tiles.  tile_rows is 64 (block sizes >= 64, child
+ * desc[p][0..8] = {first tile, K, m-tiles per block (bs / tile_rows), child tiles per block,
+ * block offset, index-row offset, first element row of the layer, shared-slot flag, copies};
+ * tile_desc[t] = p for each of the n_tiles tiles.  copies = c > 1 makes rows p+1 .. p+c-1 tied
+ * copies of row p (same K, block count, n_chs, pids and pfids; their own node blocks, child
+ * rows and element rows): row p's tiles sum the products of all c copies before writing each
+ * slot once, and the copy rows get no tiles of their own.  tile_rows is 64 (block sizes >= 64, child
  * tiles of 64) or 32 (block size 32, child tiles of 128).  element_mars is the buffer that
  * holds every layer's element rows.  flags as pjamd_sum_backward_par (bit 1 unused; bit 4 applies
  * to the partitions without the shared-slot flag).

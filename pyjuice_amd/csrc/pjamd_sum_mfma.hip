@@ -1405,7 +1405,8 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
                                          const int* __restrict__ cids, const int* __restrict__ pids,
                                          const int* __restrict__ pfids, const int* __restrict__ nch, int K, int64_t B,
                                          int j, int m0, int kt0, int64_t bstart, int64_t bend, bool modified,
-                                         int atomic_out, float sign) {
+                                         int atomic_out, float sign, const int* __restrict__ copies = nullptr,
+                                         int n_copies = 1) {
     constexpr int WM = TM / 32, WN = TN / 32;
     static_assert(WM * WN == 4, "four wavefronts per workgroup");
     constexpr int LD = TK + MF<T>::PAD;
@@ -1422,12 +1423,22 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
     if (kt0 >= n) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w % WM, wn = w / WM;
-    const int64_t nrow0 = (int64_t)nids[j] + m0;
+    floatx16 acc = {0};
+    // Tied copies (rows copies[r * PJAMD_PAR_DESC_FIELDS]: block offset, index-row offset and
+    // element row relative to copy 0, same parameter and flow slots) add into one accumulator.
+    for (int r = 0; r < n_copies; ++r) {
+    int64_t nrow0 = (int64_t)nids[j] + m0;
     const int* crow = cids + (int64_t)j * K;
+    const float* em = emars;
+    if (r > 0) {
+        const int* R = copies + (int64_t)r * PJAMD_PAR_DESC_FIELDS;
+        nrow0 = (int64_t)nids[(int64_t)R[4] - copies[4] + j] + m0;
+        crow = cids + ((int64_t)R[5] - copies[5]) + (int64_t)j * K;
+        em = emars + ((int64_t)R[6] - copies[6]) * B;
+    }
 
     if (tid < TN) sy[tid] = (kt0 + tid < n) ? (int64_t)crow[kt0 + tid] * B : -1;
     __syncthreads();
-    floatx16 acc = {0};
 
     if constexpr (VEC) {
         // 16-byte loads (B % 4 == 0, batch range bounds multiples of 4): thread = (row group
@@ -1456,7 +1467,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
 #pragma unroll
             for (int q = 0; q < PYV; ++q) {
                 const int64_t yb = sy[rg + q * RGV];
-                const float4 y = *reinterpret_cast<const float4*>(emars + (yb >= 0 ? yb : 0) + bc);
+                const float4 y = *reinterpret_cast<const float4*>(em + (yb >= 0 ? yb : 0) + bc);
                 yv[q] = (ok && yb >= 0) ? y : ninf;
             }
         };
@@ -1520,7 +1531,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
 #pragma unroll
         for (int q = 0; q < PY; ++q) {
             const int64_t yb = sy[r0 + q * RG];
-            yv[q] = (ok && yb >= 0) ? emars[yb + b] : kNegInf;
+            yv[q] = (ok && yb >= 0) ? em[yb + b] : kNegInf;
         }
     };
 
@@ -1550,6 +1561,8 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
         acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
     }
     }
+    }
+
     const int k = kt0 + wn * 32 + (lane & 31);
     if (k < n) {
         const int64_t p0 = (int64_t)pids[(int64_t)j * K + k] + m0 + wm * 32;
@@ -1607,7 +1620,8 @@ __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, 
     const int j = local / (mtiles * ktiles);
     const int mode = D[7] ? kPfAtomic : (overwrite ? kPfOverwrite : kPfAdd);
     par_tile<T, TM, TN, PF, VEC>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io,
-                                 pids + io, pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign);
+                                 pids + io, pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign, D,
+                                 D[8]);
 }
 
 // One 32 x 32 x KI product with the operand / accumulator lane layouts the kernels above assume:

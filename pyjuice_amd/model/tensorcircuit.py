@@ -23,6 +23,9 @@ from pyjuice_amd import _native as N
 from pyjuice_amd.layer.sum_layer import PRECISIONS
 from pyjuice_amd.layer import InputLayer, LayerGroup, ProdLayer, SumLayer
 from pyjuice_amd.nodes import CircuitNodes, multiply, summate
+
+# int32 fields per row of the batched parameter-flow descriptor table (PJAMD_PAR_DESC_FIELDS)
+PAR_DESC_FIELDS = 9
 from pyjuice_amd.nodes.nodes import dfs_postorder
 from pyjuice_amd.utils import BitSet
 from . import chain as _chain
@@ -244,7 +247,7 @@ class TensorCircuit(nn.Module):
         """True if one flow backward writes every sum parameter-flow slot exactly once with no
         atomics: no tied sharing inside or across layers, every slot owned by one (node, child)
         edge.  Then the first backward after the flows were cleared may store instead of add."""
-        if any(t["desc"].view(-1, 8)[:, 7].any().item() for t in self._par_batches["tables"].values()):
+        if any(t["desc"].view(-1, PAR_DESC_FIELDS)[:, 7].any().item() for t in self._par_batches["tables"].values()):
             return False
         cover = np.zeros(self.num_param_flows, dtype=np.int64)
         for g in self.inner_layer_groups:
@@ -477,21 +480,28 @@ class TensorCircuit(nn.Module):
         for tm, parts in acc.items():
             if not parts:
                 continue
+            units = self._tied_par_units(parts, tm)
+            parts = [parts[i] for u in units for i in u]
             desc, tile_desc = [], []
             n_tiles = blk_off = idx_off = 0
             tot_rows = tot_edges = tot_edges_bs = 0
-            for d, (nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared) in enumerate(parts):
-                nb, K = nids.numel(), cids.size(1)
-                mt, kt = bs // tm, (K + tn - 1) // tn
-                tiles = nb * mt * kt
-                desc.append([n_tiles, K, mt, kt, blk_off, idx_off, erow, 1 if shared else 0])
-                tile_desc.append(torch.full([tiles], d, dtype=torch.int32))
-                n_tiles += tiles
-                blk_off += nb
-                idx_off += nb * K
-                tot_rows += nb * bs
-                tot_edges += tot
-                tot_edges_bs += tot * bs
+            d = 0
+            for u in units:
+                for c in range(len(u)):
+                    nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared = parts[d]
+                    nb, K = nids.numel(), cids.size(1)
+                    mt, kt = bs // tm, (K + tn - 1) // tn
+                    tiles = nb * mt * kt if c == 0 else 0
+                    desc.append([n_tiles, K, mt, kt, blk_off, idx_off, erow, 1 if shared else 0,
+                                 len(u) if c == 0 else 1])
+                    tile_desc.append(torch.full([tiles], d, dtype=torch.int32))
+                    n_tiles += tiles
+                    blk_off += nb
+                    idx_off += nb * K
+                    tot_rows += nb * bs
+                    tot_edges += tot
+                    tot_edges_bs += tot * bs
+                    d += 1
             assert idx_off < 2 ** 31 and n_tiles < 2 ** 31 // 256
             cat = lambda k: torch.cat([p[k].reshape(-1) for p in parts]).to(torch.int32).to(device).contiguous()
             batches[tm] = {
@@ -505,6 +515,37 @@ class TensorCircuit(nn.Module):
             if not g.is_prod():
                 batched_layers.update(id(layer) for layer in g if layer.block_size >= 32)
         return {"tables": batches, "layers": batched_layers}
+
+    # Tiles a group of tied partitions should keep in one batched launch: fewer, longer tiles
+    # mean fewer atomic adds into the shared slots, more keep the chip busy.
+    par_tied_tiles = 1024
+
+    def _tied_par_units(self, parts, tm):
+        """Order the batched parameter-flow partitions into units: a unit is a run of partitions
+        whose tiles are summed before the slots are written once.  Partitions with the shared-slot
+        flag and equal shapes, ``n_chs``, ``pids`` and ``pfids`` (the time steps of a homogeneous
+        HMM, tied copies of one sum node set) are tied copies; a group of them is split into just
+        enough units to give ``par_tied_tiles`` tiles.  Other partitions are units of one."""
+        groups, order = dict(), []
+        for i, (nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared) in enumerate(parts):
+            if not shared:
+                order.append([i])
+                continue
+            key = (bs, tn, nids.numel(), cids.size(1), nch.cpu().numpy().tobytes(),
+                   pids.cpu().numpy().tobytes(), pfids.cpu().numpy().tobytes())
+            if key not in groups:
+                groups[key] = []
+                order.append(groups[key])
+            groups[key].append(i)
+        units = []
+        for g in order:
+            nids, cids, bs, tn = parts[g[0]][0], parts[g[0]][1], parts[g[0]][6], parts[g[0]][7]
+            tiles = nids.numel() * (bs // tm) * -(-cids.size(1) // tn)
+            n_units = min(len(g), max(1, -(-self.par_tied_tiles // max(tiles, 1))))
+            for u in range(n_units):
+                lo, hi = u * len(g) // n_units, (u + 1) * len(g) // n_units
+                units.append(g[lo:hi])
+        return units
 
     def _run_par_batched(self, modify: bool, precision: str, overwrite: bool):
         pf = self._pf

@@ -596,3 +596,45 @@ def test_split_reduction_backward(L, precision, device):
             got = pfi[a:b].reshape(n.num_nodes, n.dist.num_param_flows())
             want = sum(pflows[m] for m in layer.nodes if m._param_flow_range == n._param_flow_range)
             assert U.rel_err(got, want) <= 2e-2, n
+
+
+@pytest.mark.parametrize("shape", [(6, 32, 10), (5, 128, 12)])
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+def test_tied_copies_share_parameter_flow_tiles(shape, precision, device):
+    """Batched parameter flows of a homogeneous HMM with its tied time steps summed inside each
+    tile (one write per slot, PJAMD_PAR_DESC_FIELDS copies field) against one tile per time step
+    adding atomically, and both against the FP64 oracle's tied parameter flows."""
+    from pyjuice_amd.model.tensorcircuit import PAR_DESC_FIELDS
+    T, L, V = shape
+    U.say(f"[tied par tiles] HMM T={T} L={L} {precision}")
+    ns, cards = U.hmm_small(T=T, L=L, V=V)
+    pc = TensorCircuit(ns, precision=precision)
+    pc.to(device)
+    X = U.random_data(cards, 48, seed=5)
+    xt = torch.from_numpy(X).to(device)
+    out = dict()
+    for tiles in (1 << 30, 1):
+        pc.par_tied_tiles = tiles
+        pc._par_batches = pc._build_par_batches(xt.device)
+        copies = [int(t["desc"].view(-1, PAR_DESC_FIELDS)[:, 8].max()) for t in pc._par_batches["tables"].values()]
+        assert (max(copies) >= T - 2) if tiles == 1 else (max(copies) == 1), copies
+        pc(xt)
+        pc.backward(xt, flows_memory=0.0)
+        out[tiles] = pc.param_flows.detach().double().cpu().numpy()
+    assert U.rel_err(out[1], out[1 << 30]) <= 1e-5
+    # the last backward ran with the copies summed inside each tile
+    vals = dag_eval.forward(ns, X)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    tol = TOL[precision][3]
+    pc.update_param_flows(origin_ns_only=False)
+    checked = 0
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            ref = pflows[n]
+            for m in node_iterator(ns):
+                if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
+                    ref = ref + pflows[m]
+            assert U.rel_err(got, ref) <= tol, (n, U.rel_err(got, ref))
+            checked += 1
+    assert checked > 0

@@ -70,6 +70,13 @@ def test_ctypes_table_matches_header():
         assert N._SIGNATURES[name] == want, (name, N._SIGNATURES[name], want)
 
 
+def test_table_widths_match_header():
+    import re
+    from pyjuice_amd.model.tensorcircuit import PAR_DESC_FIELDS
+    text = open(HEADER).read()
+    assert int(re.search(r"#define PJAMD_PAR_DESC_FIELDS (\d+)", text).group(1)) == PAR_DESC_FIELDS
+
+
 def test_host_only_entry_points(lib):
     assert lib.pjamd_abi_version() == 1
     assert lib.pjamd_error_string(0) is not None
