@@ -112,6 +112,21 @@ int pjamd_input_backward(int kind, void* pflows, const float* node_flows, int pf
                          const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B,
                          int layout, pjamd_stream_t stream);
 
+/* pjamd_input_backward for a Categorical layer whose parameter-flow ranges are shared by tied
+ * vectors: nodes group_nodes[group_offsets[g] .. group_offsets[g + 1]) (indices into the layer,
+ * one category count per group, kmax <= 16384) are all the nodes with flow range
+ * s_pfids[group_nodes[group_offsets[g]]]; each group's flows are summed over its members and
+ * batch, then its slots are written once, without atomics (stored when overwrite != 0, which
+ * needs the groups to cover every slot of the accumulator, else added).  Returns
+ * PJAMD_EUNSUPPORTED for Bernoulli layers or larger category counts.  Replaces the tied-node
+ * part of InputLayer.backward (input_layer.py:406-571): the reference gives up to
+ * max_tied_ns_per_parflow_block tied vectors one flow range (input_layer.py:83-110) and adds
+ * their flows atomically in Categorical.bk_flow_fn (categorical.py:94-101). */
+int pjamd_input_backward_tied(int kind, void* pflows, const float* node_flows, int pf64, const int* data,
+                              const int* vids, const int64_t* s_pfids, const int* ncats, const int* group_offsets,
+                              const int* group_nodes, int64_t n_groups, int kmax, int64_t node_offset, int64_t B,
+                              int overwrite, pjamd_stream_t stream);
+
 /* Flows of the marginalized entries (negative data values) of one input layer, after
  * pjamd_input_backward: Categorical node n adds params[s_pids[n] + k] * (its flows summed over
  * the marginalized batch columns) to pflows[s_pfids[n] + k] for every category k (atomically);

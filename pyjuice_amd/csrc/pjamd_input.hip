@@ -447,6 +447,77 @@ __global__ __launch_bounds__(256) void cat_bwd_global_kernel(PF* __restrict__ pf
     if (x >= 0 && x < ncats[i] && f != 0.0f) atomicAdd(&pflows[s_pfids[i] + x], (PF)f);
 }
 
+// Categorical flow backward for tied vectors that share a flow range: one workgroup per range
+// (group g = nodes gnodes[goff[g] .. goff[g + 1]), all of one category count K), whose four
+// wavefronts take the members in turn and build one flow histogram over the range's K slots in
+// LDS (K floats of dynamic LDS); then each slot is written once, reads before writes, zero bins
+// skipped unless the slots are to be overwritten.  The group is the only writer of its range, so
+// no global atomics are needed.  VEC: 16-byte loads (B % 4 == 0, 16-byte aligned rows).
+constexpr int kTiedMaxK = 16384;
+
+template <typename PF, bool VEC>
+__global__ __launch_bounds__(256) void cat_bwd_tied_kernel(PF* __restrict__ pflows,
+                                                           const float* __restrict__ node_flows,
+                                                           const int* __restrict__ data, const int* __restrict__ vids,
+                                                           const int64_t* __restrict__ s_pfids,
+                                                           const int* __restrict__ ncats, const int* __restrict__ goff,
+                                                           const int* __restrict__ gnodes, int64_t node_offset,
+                                                           int64_t B, int pf_mode) {
+    extern __shared__ float hist[];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int a = goff[blockIdx.x], e = goff[blockIdx.x + 1];
+    const int i0 = gnodes[a];
+    const int K = ncats[i0];
+    for (int k = tid; k < K; k += 256) hist[k] = 0.0f;
+    __syncthreads();
+    for (int m = a + w; m < e; m += 4) {
+        const int i = gnodes[m];
+        const int* xrow = data + (int64_t)vids[i] * B;
+        const float* frow = node_flows + (node_offset + i) * B;
+        if constexpr (VEC) {
+            for (int64_t b = (int64_t)lane * 4; b < B; b += 256) {
+                const int4 x = *reinterpret_cast<const int4*>(xrow + b);
+                const float4 f = *reinterpret_cast<const float4*>(frow + b);
+                if (x.x >= 0 && x.x < K && f.x != 0.0f) atomicAdd(&hist[x.x], f.x);
+                if (x.y >= 0 && x.y < K && f.y != 0.0f) atomicAdd(&hist[x.y], f.y);
+                if (x.z >= 0 && x.z < K && f.z != 0.0f) atomicAdd(&hist[x.z], f.z);
+                if (x.w >= 0 && x.w < K && f.w != 0.0f) atomicAdd(&hist[x.w], f.w);
+            }
+        } else {
+            for (int64_t b = lane; b < B; b += 64) {
+                const int x = xrow[b];
+                const float f = frow[b];
+                if (x >= 0 && x < K && f != 0.0f) atomicAdd(&hist[x], f);
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int kPer = 8;
+    const int64_t f0 = s_pfids[i0];
+    for (int k0 = 0; k0 < K; k0 += 256 * kPer) {
+        float hv[kPer];
+        int kk[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int k = k0 + q * 256 + tid;
+            kk[q] = (k < K && (pf_mode == kPfOverwrite || hist[k] != 0.0f)) ? k : -1;
+            hv[q] = kk[q] >= 0 ? hist[k] : 0.0f;
+        }
+        if (pf_mode == kPfOverwrite) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (kk[q] >= 0) pflows[f0 + kk[q]] = (PF)hv[q];
+        } else {
+            PF old[kPer];
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) old[q] = kk[q] >= 0 ? pflows[f0 + kk[q]] : (PF)0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q)
+                if (kk[q] >= 0) pflows[f0 + kk[q]] = old[q] + (PF)hv[q];
+        }
+    }
+}
+
 // Bernoulli flow backward: slot 0 collects flows of x = 1, slot 1 of x = 0.
 template <typename PF>
 __global__ __launch_bounds__(256) void bern_bwd_kernel(PF* __restrict__ pflows, const float* __restrict__ node_flows,
@@ -703,6 +774,40 @@ PJAMD_EXPORT int pjamd_input_missing_flows(int kind, void* pflows, int pf64, con
             (float*)pflows, node_flows, params, data, vids, s_pids, s_pfids, ncats, node_offset, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
+}
+
+template <typename PF>
+static int input_backward_tied_impl(PF* pflows, const float* node_flows, const int* data, const int* vids,
+                                    const int64_t* s_pfids, const int* ncats, const int* goff, const int* gnodes,
+                                    int64_t n_groups, int kmax, int64_t node_offset, int64_t B, int pf_mode,
+                                    hipStream_t stream) {
+    const size_t lds = (size_t)kmax * sizeof(float);
+    const bool vec = B % 4 == 0 && (node_offset * B) % 4 == 0 &&
+                     ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0);
+    if (vec)
+        cat_bwd_tied_kernel<PF, true><<<(unsigned)n_groups, 256, lds, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, goff, gnodes, node_offset, B, pf_mode);
+    else
+        cat_bwd_tied_kernel<PF, false><<<(unsigned)n_groups, 256, lds, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, goff, gnodes, node_offset, B, pf_mode);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_input_backward_tied(int kind, void* pflows, const float* node_flows, int pf64, const int* data,
+                                           const int* vids, const int64_t* s_pfids, const int* ncats,
+                                           const int* group_offsets, const int* group_nodes, int64_t n_groups,
+                                           int kmax, int64_t node_offset, int64_t B, int overwrite,
+                                           hipStream_t stream) {
+    PJAMD_REQUIRE(n_groups >= 0 && B > 0 && n_groups < ((int64_t)1 << 31) && kmax > 0);
+    if (kind != 0 || kmax > kTiedMaxK) return PJAMD_EUNSUPPORTED;
+    if (n_groups == 0) return PJAMD_OK;
+    const int pf_mode = overwrite ? kPfOverwrite : kPfAdd;
+    if (pf64)
+        return input_backward_tied_impl<double>((double*)pflows, node_flows, data, vids, s_pfids, ncats, group_offsets,
+                                                group_nodes, n_groups, kmax, node_offset, B, pf_mode, stream);
+    return input_backward_tied_impl<float>((float*)pflows, node_flows, data, vids, s_pfids, ncats, group_offsets,
+                                           group_nodes, n_groups, kmax, node_offset, B, pf_mode, stream);
 }
 
 // `layout`: bit 0 = some flow ranges are shared by several input vectors (atomic adds); bit 1 =

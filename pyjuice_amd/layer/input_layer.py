@@ -249,8 +249,34 @@ class InputLayer(Layer, nn.Module):
             # Several vectors accumulate into one flow range when tied vectors share it.
             "shared_pf": any(ns.is_tied() for ns in self.nodes),
             "sorted_tiles": self._uniform_tiles(8),
+            "tied_groups": self._tied_groups(),
         }
         assert self.num_vars_per_node == 1, "multivariate input distributions are not supported by the HIP layers"
+
+    # largest category count of the tied-range backward kernel (its LDS histogram)
+    tied_max_cats = 16384
+
+    def _tied_groups(self):
+        """Groups of nodes sharing one parameter-flow range, for the tied-range backward
+        (``pjamd_input_backward_tied``): ``(offsets, nodes, n_groups, covers)`` as int32 device
+        tensors, ``covers`` True when the ranges tile the whole accumulator (then a zeroed
+        accumulator may be stored into).  None unless the layer is Categorical with tied vectors
+        and every group has one category count within ``tied_max_cats``."""
+        if self.num_nodes == 0 or self._kind() != 0 or not any(ns.is_tied() for ns in self.nodes):
+            return None
+        s = self.s_pfids.detach().cpu().numpy().astype(np.int64)
+        k = self._ncats.detach().cpu().numpy().astype(np.int64)
+        order = np.argsort(s, kind="stable")
+        bases, first, counts = np.unique(s[order], return_index=True, return_counts=True)
+        ks = k[order]
+        if int(k.max()) > self.tied_max_cats or np.any(ks != np.repeat(ks[first], counts)):
+            return None
+        if np.any(bases[1:] < bases[:-1] + ks[first][:-1]):
+            return None   # ranges overlap: not a partition of the slots
+        offsets = np.concatenate([first, [len(s)]]).astype(np.int32)
+        covers = int(ks[first].sum()) == self.num_param_flows
+        d = self.device
+        return (torch.from_numpy(offsets).to(d), torch.from_numpy(order.astype(np.int32)).to(d), len(bases), covers)
 
     def _uniform_tiles(self, tile: int) -> bool:
         """True if every tile of ``tile`` consecutive nodes (from node 0) has one variable and one
@@ -329,16 +355,23 @@ class InputLayer(Layer, nn.Module):
         t = self._dev
         B = node_flows.size(1)
         # store instead of add into a buffer marked zero when every slot has one owner
-        overwrite = self._pf_fresh and not t["shared_pf"]
+        tg = t["tied_groups"]
+        overwrite = self._pf_fresh and (not t["shared_pf"] or (tg is not None and tg[3]))
         pf = self._pf if overwrite else self.param_flows
         pf64 = 1 if pf.dtype == torch.float64 else 0
-        flags = (t["kmax"] << 3) | (4 if overwrite else 0) | (2 if t["sorted_tiles"] else 0) | \
-            (1 if t["shared_pf"] else 0)
         self._pf_fresh = False
-        N.call("pjamd_input_backward", self._kind(), N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data),
-               N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0],
-               B, flags, N.stream(), algo=(4 * (self.num_nodes + len(self.layer_scope)) * B
-                                            + (1 if overwrite else 2) * pf.element_size() * self.num_param_flows, 0))
+        algo = (4 * (self.num_nodes + len(self.layer_scope)) * B
+                + (1 if overwrite else 2) * pf.element_size() * self.num_param_flows, 0)
+        if tg is not None:
+            N.call("pjamd_input_backward_tied", self._kind(), N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data),
+                   N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), N.ptr(tg[0]), N.ptr(tg[1]), tg[2],
+                   t["kmax"], self._output_ind_range[0], B, 1 if overwrite else 0, N.stream(), algo=algo)
+        else:
+            flags = (t["kmax"] << 3) | (4 if overwrite else 0) | (2 if t["sorted_tiles"] else 0) | \
+                (1 if t["shared_pf"] else 0)
+            N.call("pjamd_input_backward", self._kind(), N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data),
+                   N.ptr(t["vids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes,
+                   self._output_ind_range[0], B, flags, N.stream(), algo=algo)
         if marginalized:
             N.call("pjamd_input_missing_flows", self._kind(), N.ptr(pf), pf64, N.ptr(node_flows), N.ptr(self.params),
                    N.ptr(data), N.ptr(t["vids"]), N.ptr(t["s_pids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]),

@@ -171,3 +171,31 @@ def test_product_fusion_tables(num_vars, L, K):
             for nids, cids, _ in layer._dev["fw"]:
                 for e in nids.numpy().tolist():
                     assert set(range(e, e + layer.block_size)) <= seen
+
+
+@pytest.mark.parametrize("T,L,V,tie", [(12, 16, 6, 8), (5, 32, 10, 2), (9, 16, 4, 3)])
+def test_tied_input_groups(T, L, V, tie):
+    """The tied-range input backward's groups: every node in exactly one group, a group is
+    exactly the nodes with its flow range, one category count per group, and full coverage of the
+    accumulator reported only when the ranges tile it."""
+    import torch
+    ns, cards = U.hmm_small(T=T, L=L, V=V)
+    pc = TensorCircuit(ns, max_tied_ns_per_parflow_block=tie)
+    layer = pc.input_layer_group[0]
+    layer.device = torch.device("cpu")
+    offsets, nodes, n_groups, covers = layer._tied_groups()
+    off, nd = offsets.numpy(), nodes.numpy()
+    s = layer.s_pfids.numpy()
+    k = layer._ncats.numpy()
+    assert off[0] == 0 and off[-1] == layer.num_nodes and len(off) == n_groups + 1
+    assert sorted(nd.tolist()) == list(range(layer.num_nodes))
+    slots = np.zeros(layer.num_param_flows, dtype=np.int64)
+    for g in range(n_groups):
+        members = nd[off[g]:off[g + 1]]
+        assert len(members) >= 1
+        assert len(set(s[members].tolist())) == 1 and len(set(k[members].tolist())) == 1
+        assert set(np.flatnonzero(s == s[members[0]]).tolist()) == set(members.tolist())
+        slots[s[members[0]]:s[members[0]] + k[members[0]]] += 1
+    assert slots.max() == 1
+    assert covers == bool(np.all(slots == 1))
+    assert max(off[1:] - off[:-1]) <= tie
