@@ -899,26 +899,65 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_split(float* __restrict__ nod
     }
 }
 
-// node_mars[nids[j] + i][b] from the partials of sum_fwd_vec_split; one thread per (row, column).
+// Largest split count of the split-reduction kernels.
+constexpr int kMaxSplits = 16;
+
+// Log-sum-exp of the nsplit partials (running max, sum scaled to it) of one row at the four
+// columns b .. b + 3: every partial is loaded before any is used, so a thread waits for one round
+// trip of 16-byte loads.  M = -inf when every split is empty.
+__device__ __forceinline__ void split_parts4(const float* __restrict__ ws_acc, const float* __restrict__ ws_max,
+                                             int nsplit, int64_t rows, int64_t row, int64_t B, int64_t b, float M[4],
+                                             float S[4]) {
+    float4 mx[kMaxSplits], ac[kMaxSplits];
+    const int64_t mrow = row / 64;
+#pragma unroll
+    for (int s = 0; s < kMaxSplits; ++s) {
+        if (s < nsplit) {
+            mx[s] = *reinterpret_cast<const float4*>(ws_max + ((int64_t)s * (rows / 64) + mrow) * B + b);
+            ac[s] = *reinterpret_cast<const float4*>(ws_acc + ((int64_t)s * rows + row) * B + b);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float m = kNegInf;
+#pragma unroll
+        for (int s = 0; s < kMaxSplits; ++s)
+            if (s < nsplit) m = fmaxf(m, c == 0 ? mx[s].x : c == 1 ? mx[s].y : c == 2 ? mx[s].z : mx[s].w);
+        float sum = 0.0f;
+        if (m != kNegInf) {
+#pragma unroll
+            for (int s = 0; s < kMaxSplits; ++s) {
+                if (s < nsplit) {
+                    const float ms = c == 0 ? mx[s].x : c == 1 ? mx[s].y : c == 2 ? mx[s].z : mx[s].w;
+                    const float a = c == 0 ? ac[s].x : c == 1 ? ac[s].y : c == 2 ? ac[s].z : ac[s].w;
+                    sum += (ms == kNegInf ? 0.0f : __expf(ms - m)) * a;
+                }
+            }
+        }
+        M[c] = m;
+        S[c] = sum;
+    }
+}
+
+// node_mars[nids[j] + i][b] from the partials of sum_fwd_vec_split; one thread per (row, four
+// columns), B % 4 == 0.
 __global__ __launch_bounds__(256) void sum_fwd_split_combine(float* __restrict__ node_mars, const int* __restrict__ nids,
                                                              const float* __restrict__ ws_acc,
                                                              const float* __restrict__ ws_max, int nsplit, int bs,
                                                              int64_t rows, int64_t B) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t row = blockIdx.y;
-    if (b >= B) return;
-    const int64_t mrow = row / 64;
-    float M = kNegInf;
-    for (int s = 0; s < nsplit; ++s) M = fmaxf(M, ws_max[((int64_t)s * (rows / 64) + mrow) * B + b]);
-    float S = 0.0f;
-    if (M != kNegInf) {
-        for (int s = 0; s < nsplit; ++s) {
-            const float ms = ws_max[((int64_t)s * (rows / 64) + mrow) * B + b];
-            S += (ms == kNegInf ? 0.0f : __expf(ms - M)) * ws_acc[((int64_t)s * rows + row) * B + b];
-        }
-    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nq = B / 4;
+    if (t >= rows * nq) return;
+    const int64_t row = t / nq, b = 4 * (t % nq);
+    float M[4], S[4];
+    split_parts4(ws_acc, ws_max, nsplit, rows, row, B, b, M, S);
     const int64_t j = row / bs, i = row % bs;
-    node_mars[((int64_t)nids[j] + i) * B + b] = (S > 0.0f && M != kNegInf) ? __logf(S) + M : kNegInf;
+    float4 o;
+    o.x = (S[0] > 0.0f && M[0] != kNegInf) ? __logf(S[0]) + M[0] : kNegInf;
+    o.y = (S[1] > 0.0f && M[1] != kNegInf) ? __logf(S[1]) + M[1] : kNegInf;
+    o.z = (S[2] > 0.0f && M[2] != kNegInf) ? __logf(S[2]) + M[2] : kNegInf;
+    o.w = (S[3] > 0.0f && M[3] != kNegInf) ? __logf(S[3]) + M[3] : kNegInf;
+    *reinterpret_cast<float4*>(node_mars + ((int64_t)nids[j] + i) * B + b) = o;
 }
 
 // Element flows with the reduction over parent rows split over workgroups (16-byte form, parent
@@ -1017,7 +1056,8 @@ __global__ __launch_bounds__(256) void sum_ele_vec_split(const float* nflows, co
 }
 
 // Child-row flows from the partials of sum_ele_vec_split, written into the product's children
-// (ch_nodes as pjamd_sum_backward_ele_to_children); one thread per (child row, column).
+// (ch_nodes as pjamd_sum_backward_ele_to_children); one thread per (child row, four columns),
+// B % 4 == 0.
 __global__ __launch_bounds__(256) void sum_ele_split_combine(float* __restrict__ child_flows,
                                                              const float* __restrict__ nmars,
                                                              const float* __restrict__ emars,
@@ -1026,33 +1066,32 @@ __global__ __launch_bounds__(256) void sum_ele_split_combine(float* __restrict__
                                                              bool mod_children, const float* __restrict__ ws_acc,
                                                              const float* __restrict__ ws_max, int nsplit, int cbs,
                                                              int64_t rows, int64_t B) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t row = blockIdx.y;
-    if (b >= B) return;
-    const int64_t mrow = row / 64;
-    float M = kNegInf;
-    for (int s = 0; s < nsplit; ++s) M = fmaxf(M, ws_max[((int64_t)s * (rows / 64) + mrow) * B + b]);
-    float S = 0.0f;
-    if (M != kNegInf) {
-        for (int s = 0; s < nsplit; ++s) {
-            const float ms = ws_max[((int64_t)s * (rows / 64) + mrow) * B + b];
-            S += (ms == kNegInf ? 0.0f : __expf(ms - M)) * ws_acc[((int64_t)s * rows + row) * B + b];
-        }
-    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nq = B / 4;
+    if (t >= rows * nq) return;
+    const int64_t row = t / nq, b = 4 * (t % nq);
     const int64_t j = row / cbs, i = row % cbs;
-    const float e = emars[((int64_t)chids[j] + i) * B + b];
-    const float v = (M == kNegInf || e == kNegInf) ? 0.0f : S * __expf(e + M);
+    const float4 e = *reinterpret_cast<const float4*>(emars + ((int64_t)chids[j] + i) * B + b);
+    float M[4], S[4];
+    split_parts4(ws_acc, ws_max, nsplit, rows, row, B, b, M, S);
+    const float ev[4] = {e.x, e.y, e.z, e.w};
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = (M[c] == kNegInf || ev[c] == kNegInf) ? 0.0f : S[c] * __expf(ev[c] + M[c]);
     const int* crow = ch_nodes + j * Kc;
     for (int k = 0; k < Kc; ++k) {
         const int code = crow[k];
         if (code == 0) continue;
         const int64_t o = ((int64_t)(code >> 1) + i) * B + b;
+        float4 w = make_float4(v[0], v[1], v[2], v[3]);
         if ((code & 1) && mod_children) {
-            const float m = nmars[o];
-            child_flows[o] = (m == kNegInf) ? kNegInf : __logf(v) - m;
-        } else {
-            child_flows[o] = v;
+            const float4 m = *reinterpret_cast<const float4*>(nmars + o);
+            w.x = (m.x == kNegInf) ? kNegInf : __logf(v[0]) - m.x;
+            w.y = (m.y == kNegInf) ? kNegInf : __logf(v[1]) - m.y;
+            w.z = (m.z == kNegInf) ? kNegInf : __logf(v[2]) - m.z;
+            w.w = (m.w == kNegInf) ? kNegInf : __logf(v[3]) - m.w;
         }
+        *reinterpret_cast<float4*>(child_flows + o) = w;
     }
 }
 
@@ -1735,8 +1774,9 @@ int launch_fwd_split(float* node_mars, const float* emars, const float* params, 
     PJAMD_LAUNCH_CHECK();
     const int64_t rows = (int64_t)n_blocks * bs;
     PJAMD_REQUIRE(rows < 65536);
-    sum_fwd_split_combine<<<dim3(cdiv(B, 256), (unsigned)rows), 256, 0, stream>>>(node_mars, nids, ws_acc, ws_max,
-                                                                                  nsplit, bs, rows, B);
+    PJAMD_REQUIRE(nsplit <= kMaxSplits);
+    sum_fwd_split_combine<<<(unsigned)cdiv(rows * (B / 4), 256), 256, 0, stream>>>(node_mars, nids, ws_acc, ws_max,
+                                                                                   nsplit, bs, rows, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
@@ -1759,7 +1799,8 @@ int launch_ele_split(float* node_flows, const float* nmars, const float* emars, 
                                                                           npar, KP, lbs, B, mtiles, (int)btiles, mod,
                                                                           nsplit, ws_acc, ws_max);
     PJAMD_LAUNCH_CHECK();
-    sum_ele_split_combine<<<dim3(cdiv(B, 256), (unsigned)rows), 256, 0, stream>>>(
+    PJAMD_REQUIRE(nsplit <= kMaxSplits);
+    sum_ele_split_combine<<<(unsigned)cdiv(rows * (B / 4), 256), 256, 0, stream>>>(
         node_flows, nmars, emars, chids, ch_nodes, Kc, modc, ws_acc, ws_max, nsplit, cbs, rows, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
