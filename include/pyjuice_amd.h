@@ -283,6 +283,41 @@ int pjamd_sum_backward_par(void* param_flows, const float* node_flows, const flo
                            const int* pids, const int* pfids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
                            int flags, int precision, pjamd_stream_t stream);
 
+/* Other propagation algorithms of the sum layers (PyJuice's propagation_alg), on the vector ALUs
+ * over the same partition tables: alg 1 = "MPE", 2 = "GeneralLL" with exponent alpha > 0, 0 =
+ * "LL" (as GeneralLL with alpha = 1, an independent check of the main kernels).  With
+ * lt = log(params[pids[j][k] + i]) + element_mars[cids[j][k]][b]:
+ * forward: node_mars[nids[j] + i][b] = max_k lt (MPE) or log(sum_k exp(alpha lt)) / alpha.
+ * Replaces the propagation_alg_id 1 / 2 branches of SumLayer._fw_triton_block_sparse_*_kernel
+ * and _fw_triton_sparse_kernel (sum_layer.py:454-800, 972-1100), selected by
+ * Layer.propagation_alg_mapping (layer.py:11-15). */
+int pjamd_sum_forward_alg(float* node_mars, const float* element_mars, const float* params, const int* nids,
+                          const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
+                          int alg, float alpha, pjamd_stream_t stream);
+
+/* Element flows for alg 1 / 2 over one backward partition (node flows as stored: these
+ * algorithms take no in-place log transform): element_flows[chids[j] + ci][b] = sum over parent
+ * rows p of node_flows[p][b] where |lt - node_mars[p][b]| < 1e-6 (MPE), or node_flows[p][b] *
+ * exp(alpha (lt - node_mars[p][b])) (GeneralLL), lt = log(theta[p, ci]) + element_mars[chids[j] +
+ * ci][b]; accumulate != 0 adds to the stored element flows.  Replaces the propagation_alg_id 1 / 2
+ * branches of SumLayer._bk_triton_block_sparse_ele_kernel (sum_layer.py:1569-1730) and the modify
+ * pass variants (sum_layer.py:1352-1392). */
+int pjamd_sum_backward_ele_alg(float* element_flows, const float* node_flows, const float* node_mars,
+                               const float* element_mars, const float* params, const int* chids, const int* parids,
+                               const int* parpids, const int* n_pars, int n_cblocks, int KP, int cbs, int bs,
+                               int64_t B, int alg, float alpha, int accumulate, pjamd_stream_t stream);
+
+/* Parameter flows for alg 1 / 2 over one forward partition, added atomically:
+ * param_flows[pfids[j][k] + i] += sum_b node_flows[n][b] where |lt - node_mars[n][b]| < 1e-6
+ * (MPE, not scaled by theta), or theta * sum_b node_flows[n][b] exp(e[b] - node_mars[n][b])
+ * (GeneralLL: the LL form, as the reference's kernel computes it); n = nids[j] + i.  pf64 selects
+ * a double accumulator.  Replaces the propagation_alg_id branches of
+ * SumLayer._bk_triton_block_sparse_par_kernel (sum_layer.py:2049-2169). */
+int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const float* node_flows, const float* node_mars,
+                               const float* element_mars, const float* params, const int* nids, const int* cids,
+                               const int* pids, const int* pfids, const int* n_chs, int n_blocks, int K, int bs,
+                               int64_t B, int alg, pjamd_stream_t stream);
+
 /* Number of int32 fields per row of the `desc` table below. */
 #define PJAMD_PAR_DESC_FIELDS 9
 

@@ -14,6 +14,14 @@ Maths (reference citations):
   ``pf[n, c] = W[n, c] sum_b f_n exp(v_c - v_n)``                    (sum_layer.py:1569-1730, 2049-2169)
 * EM: ``theta' = (1 - s) theta + s (pf + pc / nch) / (sum pf + pc)`` (par_update.py:214-256,
   categorical.py:164-201, bernoulli.py:91-101)
+* other propagation algorithms (``alg`` of forward / backward; lt = log W[n, c] + v_c):
+  MPE ``v[n] = max_c lt``, flows and parameter flows (not scaled by W) where |lt - v[n]| < 1e-6;
+  GeneralLL ``v[n] = log sum_c exp(alpha lt) / alpha``, flows ``f_c += f_n exp(alpha (lt - v_n))``,
+  parameter flows in the LL form ``W sum_b f_n exp(v_c - v_n)``
+  (sum_layer.py:454-800, 1352-1392, 1569-1730, 2049-2169).  Pinned by the tests: GeneralLL with
+  alpha = 1 equals LL; GeneralLL tends to MPE as alpha grows; the log-terms along an MPE
+  path add up to the MPE value; flows into each variable's inputs sum to one per sample.
+  GeneralLL parameter flows with alpha != 1 are pinned only through alpha = 1 (partial).
 """
 from __future__ import annotations
 
@@ -91,8 +99,25 @@ def _lse_matmul(W: np.ndarray, child: np.ndarray) -> np.ndarray:
     return out
 
 
-def forward(root, data) -> Dict:
-    """Log-values of every vector: ``{ns: [num_nodes, B]}``; ``data`` is ``[B, num_vars]``."""
+def _alg_sum(W: np.ndarray, child: np.ndarray, alg: str, alpha: float) -> np.ndarray:
+    """One sum vector's values under MPE / GeneralLL (sum_layer.py:508-563)."""
+    with np.errstate(divide="ignore"):
+        lW = np.log(W)
+    lt = lW[:, :, None] + child[None, :, :]                       # [n, C, B]
+    if alg == "MPE":
+        return lt.max(axis=1)
+    x = alpha * lt
+    m = x.max(axis=1, keepdims=True)
+    mm = np.where(np.isfinite(m), m, 0.0)
+    with np.errstate(divide="ignore"):
+        out = (np.log(np.exp(x - mm).sum(axis=1)) + mm[:, 0, :]) / alpha
+    out[~np.isfinite(m[:, 0, :])] = -np.inf
+    return out
+
+
+def forward(root, data, alg: str = "LL", alpha: float = 1.0) -> Dict:
+    """Log-values of every vector: ``{ns: [num_nodes, B]}``; ``data`` is ``[B, num_vars]``.
+    ``alg``: "LL", "MPE" or "GeneralLL" (with ``alpha``) for the sum vectors."""
     data = _np(data)
     vals = dict()
     for ns in node_iterator(root):
@@ -105,11 +130,15 @@ def forward(root, data) -> Dict:
             vals[ns] = acc
         else:
             child = np.concatenate([vals[c] for c in ns.chs], axis=0)
-            vals[ns] = _lse_matmul(sum_dense_params(ns), child)
+            if alg == "LL":
+                vals[ns] = _lse_matmul(sum_dense_params(ns), child)
+            else:
+                vals[ns] = _alg_sum(sum_dense_params(ns), child, alg, alpha)
     return vals
 
 
-def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None):
+def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None, alg: str = "LL",
+             alpha: float = 1.0):
     """Flows of every vector and parameter flows.
 
     Returns ``(flows, pflows)``: ``flows[ns]`` is ``[num_nodes, B]``; ``pflows[ns]`` is the dense
@@ -124,7 +153,27 @@ def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None):
         f = flows.get(ns)
         if f is None:
             f = np.zeros([ns.num_nodes, B])
-        if ns.is_sum():
+        if ns.is_sum() and alg != "LL":
+            W = sum_dense_params(ns)
+            child = np.concatenate([vals[c] for c in ns.chs], axis=0)
+            v = vals[ns]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                lt = np.log(W)[:, :, None] + child[None, :, :]       # [n, C, B]
+                if alg == "MPE":
+                    share = (np.abs(lt - v[:, None, :]) < 1e-6).astype(np.float64)
+                    pflows[ns] = (share * f[:, None, :]).sum(axis=2)
+                else:
+                    share = np.where(np.isfinite(v)[:, None, :] & np.isfinite(lt),
+                                     np.exp(alpha * (lt - v[:, None, :])), 0.0)
+                    ratio = np.where(np.isfinite(v)[:, None, :] & np.isfinite(child)[None, :, :],
+                                     np.exp(child[None, :, :] - v[:, None, :]), 0.0)
+                    pflows[ns] = W * (ratio * f[:, None, :]).sum(axis=2)
+            cf = (share * f[:, None, :]).sum(axis=0)
+            s = 0
+            for c in ns.chs:
+                flows[c] = flows.get(c, np.zeros([c.num_nodes, B])) + cf[s:s + c.num_nodes]
+                s += c.num_nodes
+        elif ns.is_sum():
             W = sum_dense_params(ns)
             child = np.concatenate([vals[c] for c in ns.chs], axis=0)
             v = vals[ns]

@@ -53,6 +53,9 @@ _SIGNATURES = {
     "pjamd_sum_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_backward_ele_split": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i32,
                                      _i32, _i64, _i32, _i32, _c_p, _i32, _c_p],
+    "pjamd_sum_forward_alg": [_c_p] * 7 + [_i32, _i32, _i32, _i64, _i32, _f32, _c_p],
+    "pjamd_sum_backward_ele_alg": [_c_p] * 9 + [_i32, _i32, _i32, _i32, _i64, _i32, _f32, _i32, _c_p],
+    "pjamd_sum_backward_par_alg": [_c_p, _i32] + [_c_p] * 9 + [_i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_sum_forward_split": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32,
                                 _c_p, _i32, _c_p],
     "pjamd_sum_forward_prod": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,

@@ -495,6 +495,128 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_rowwise(PF* __restrict__
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Other propagation algorithms (sum_layer.py:454-800 forward, 1352-1392 modify pass, 1569-1730
+// element flows, 2049-2169 parameter flows), on the vector ALUs: alg 1 = MPE, 2 = GeneralLL with
+// exponent alpha; alg 0 (LL) is GeneralLL with alpha = 1.  lt = log theta + e is formed the
+// same way (same operations in the same order) in the forward and in both backward kernels, so
+// MPE's match test |lt - m| < 1e-6 finds the forward's own maximum.
+
+// node_mars[nid + i][b]: MPE max_k lt_k; GeneralLL log(sum_k exp(alpha lt_k)) / alpha.  One thread
+// per (node row, column).
+__global__ __launch_bounds__(kThreads) void sum_fwd_alg_kernel(float* __restrict__ node_mars,
+                                                               const float* __restrict__ emars,
+                                                               const float* __restrict__ params,
+                                                               const int* __restrict__ nids,
+                                                               const int* __restrict__ cids,
+                                                               const int* __restrict__ pids,
+                                                               const int* __restrict__ nch, int K, int bs, int64_t B,
+                                                               int alg, float alpha) {
+    const int64_t b = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int j = blockIdx.x / bs, i = blockIdx.x % bs;
+    const int n = nch[j];
+    const int* crow = cids + (int64_t)j * K;
+    const int* prow = pids + (int64_t)j * K;
+    float m = kNegInf, s = 0.0f;
+    for (int k = 0; k < n; ++k) {
+        const float lt = __logf(params[(int64_t)prow[k] + i]) + emars[(int64_t)crow[k] * B + b];
+        if (alg == 1) {
+            m = fmaxf(m, lt);
+        } else {
+            const float x = alpha * lt;
+            if (x == kNegInf) continue;
+            if (x > m) {
+                s = s * __expf(m - x) + 1.0f;
+                m = x;
+            } else {
+                s += __expf(x - m);
+            }
+        }
+    }
+    float v = m;
+    if (alg != 1) v = (m == kNegInf) ? kNegInf : (m + __logf(s)) / alpha;
+    node_mars[((int64_t)nids[j] + i) * B + b] = v;
+}
+
+// element_flows[chid + ci][b] = sum over parents p of: MPE nf[p][b] where |lt - nm[p][b]| < 1e-6;
+// GeneralLL nf[p][b] exp(alpha (lt - nm[p][b])), the child's share of the parent's power sum.
+// Flows are read as stored (no in-place transform for these algorithms).  One thread per (child
+// row, column); accumulate adds to the element flows instead of storing.
+__global__ __launch_bounds__(kThreads) void sum_bwd_ele_alg_kernel(float* __restrict__ eflows,
+                                                                   const float* __restrict__ nflows,
+                                                                   const float* __restrict__ nmars,
+                                                                   const float* __restrict__ emars,
+                                                                   const float* __restrict__ params,
+                                                                   const int* __restrict__ chids,
+                                                                   const int* __restrict__ parids,
+                                                                   const int* __restrict__ parpids,
+                                                                   const int* __restrict__ npar, int KP, int cbs, int bs,
+                                                                   int64_t B, int alg, float alpha, bool accumulate) {
+    const int64_t b = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int j = blockIdx.x / cbs, ci = blockIdx.x % cbs;
+    const int kflat = npar[j] * bs;
+    const int* parrow = parids + (int64_t)j * KP;
+    const int* pprow = parpids + (int64_t)j * KP;
+    const int64_t o = ((int64_t)chids[j] + ci) * B + b;
+    const float e = emars[o];
+    float acc = 0.0f;
+    for (int kk = 0; kk < kflat; ++kk) {
+        const int kb = kk / bs, pi = kk - kb * bs;
+        const int64_t po = ((int64_t)parrow[kb] + pi) * B + b;
+        const float nf = nflows[po], nm = nmars[po];
+        const float lt = __logf(params[(int64_t)pprow[kb] + (int64_t)ci * bs + pi]) + e;
+        if (alg == 1) {
+            if (fabsf(lt - nm) < 1e-6f) acc += nf;
+        } else if (nm != kNegInf && lt != kNegInf && nf != 0.0f) {
+            acc += nf * __expf(alpha * (lt - nm));
+        }
+    }
+    if (accumulate) acc += eflows[o];
+    eflows[o] = acc;
+}
+
+// pf[pfid_k + i] += MPE: sum_b nf[b] where |lt_k - nm[b]| < 1e-6 (not scaled by theta);
+// GeneralLL: theta_k sum_b nf[b] exp(e_k[b] - nm[b]) (the LL form, as the reference computes it).
+// One wavefront per (node row, child), lanes along the batch; slots added atomically.
+template <typename PF>
+__global__ __launch_bounds__(kThreads) void sum_bwd_par_alg_kernel(PF* __restrict__ pflows,
+                                                                   const float* __restrict__ nflows,
+                                                                   const float* __restrict__ nmars,
+                                                                   const float* __restrict__ emars,
+                                                                   const float* __restrict__ params,
+                                                                   const int* __restrict__ nids,
+                                                                   const int* __restrict__ cids,
+                                                                   const int* __restrict__ pids,
+                                                                   const int* __restrict__ pfids,
+                                                                   const int* __restrict__ nch, int K, int bs,
+                                                                   int64_t B, int alg) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.y * (kThreads / 64) + w;
+    const int j = blockIdx.x / bs, i = blockIdx.x % bs;
+    if (k >= nch[j]) return;
+    const int64_t nrow = (int64_t)nids[j] + i;
+    const int64_t erow = cids[(int64_t)j * K + k];
+    const float th = params[(int64_t)pids[(int64_t)j * K + k] + i];
+    const float lth = __logf(th);
+    float s = 0.0f;
+    for (int64_t b = lane; b < B; b += 64) {
+        const float nf = nflows[nrow * B + b], nm = nmars[nrow * B + b];
+        const float e = emars[erow * B + b];
+        if (alg == 1) {
+            if (fabsf(lth + e - nm) < 1e-6f) s += nf;
+        } else if (nm != kNegInf && e != kNegInf && nf != 0.0f) {
+            s += nf * __expf(e - nm);
+        }
+    }
+    s = warp_sum(s);
+    if (lane == 0) {
+        const float v = alg == 1 ? s : th * s;
+        if (v != 0.0f) atomicAdd(&pflows[(int64_t)pfids[(int64_t)j * K + k] + i], (PF)v);
+    }
+}
 }  // namespace
 
 // precision: 0 = FP32 operands on MFMA, 1 = BF16 MFMA, 2 = FP16 MFMA (pjamd_sum_mfma.hip, block sizes
@@ -635,4 +757,58 @@ PJAMD_EXPORT int pjamd_sum_backward_par(void* param_flows, const float* node_flo
                                         pids, pfids, nch, n_blocks, K, bs, B, mod, atomic_out, sign, stream);
     return sum_bwd_par_impl<float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids, cids, pids,
                                    pfids, nch, n_blocks, K, bs, B, mod, atomic_out, sign, stream);
+}
+
+// alg: 0 = LL (as GeneralLL with alpha = 1), 1 = MPE, 2 = GeneralLL.
+PJAMD_EXPORT int pjamd_sum_forward_alg(float* node_mars, const float* element_mars, const float* params,
+                                       const int* nids, const int* cids, const int* pids, const int* nch,
+                                       int n_blocks, int K, int bs, int64_t B, int alg, float alpha,
+                                       hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && alg >= 0 && alg <= 2 &&
+                  (int64_t)n_blocks * bs < ((int64_t)1 << 31));
+    PJAMD_REQUIRE(alg != 2 || alpha > 0.0f);
+    if (n_blocks == 0) return PJAMD_OK;
+    dim3 grid((unsigned)(n_blocks * bs), cdiv(B, kThreads));
+    sum_fwd_alg_kernel<<<grid, kThreads, 0, stream>>>(node_mars, element_mars, params, nids, cids, pids, nch, K, bs, B,
+                                                      alg, alg == 0 ? 1.0f : alpha);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sum_backward_ele_alg(float* element_flows, const float* node_flows, const float* node_mars,
+                                            const float* element_mars, const float* params, const int* chids,
+                                            const int* parids, const int* parpids, const int* n_pars, int n_cblocks,
+                                            int KP, int cbs, int bs, int64_t B, int alg, float alpha, int accumulate,
+                                            hipStream_t stream) {
+    PJAMD_REQUIRE(n_cblocks >= 0 && KP >= 1 && cbs >= 1 && bs >= 1 && B > 0 && alg >= 0 && alg <= 2 &&
+                  (int64_t)n_cblocks * cbs < ((int64_t)1 << 31));
+    PJAMD_REQUIRE(alg != 2 || alpha > 0.0f);
+    if (n_cblocks == 0) return PJAMD_OK;
+    dim3 grid((unsigned)(n_cblocks * cbs), cdiv(B, kThreads));
+    sum_bwd_ele_alg_kernel<<<grid, kThreads, 0, stream>>>(element_flows, node_flows, node_mars, element_mars, params,
+                                                          chids, parids, parpids, n_pars, KP, cbs, bs, B, alg,
+                                                          alg == 0 ? 1.0f : alpha, accumulate != 0);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const float* node_flows,
+                                            const float* node_mars, const float* element_mars, const float* params,
+                                            const int* nids, const int* cids, const int* pids, const int* pfids,
+                                            const int* nch, int n_blocks, int K, int bs, int64_t B, int alg,
+                                            hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && alg >= 0 && alg <= 2 &&
+                  (int64_t)n_blocks * bs < ((int64_t)1 << 31));
+    if (n_blocks == 0) return PJAMD_OK;
+    dim3 grid((unsigned)(n_blocks * bs), cdiv(K, kThreads / 64));
+    if (pf64)
+        sum_bwd_par_alg_kernel<double><<<grid, kThreads, 0, stream>>>((double*)param_flows, node_flows, node_mars,
+                                                                      element_mars, params, nids, cids, pids, pfids,
+                                                                      nch, K, bs, B, alg);
+    else
+        sum_bwd_par_alg_kernel<float><<<grid, kThreads, 0, stream>>>((float*)param_flows, node_flows, node_mars,
+                                                                     element_mars, params, nids, cids, pids, pfids,
+                                                                     nch, K, bs, B, alg);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
 }

@@ -231,8 +231,10 @@ class SumLayer(Layer, nn.Module):
         """``node_mars[n] = log sum_c theta[n, c] exp(element_mars[c])`` for every node of the
         layer (``sum_layer.py:220-262``), with per-tile max subtraction.  ``prod_tables`` (from
         ``TensorCircuit._build_prod_fusion``, keyed by layer): compute the element rows from the
-        product layer's children in the same launch and store them (``pjamd_sum_forward_prod``)."""
-        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
+        product layer's children in the same launch and store them (``pjamd_sum_forward_prod``).
+        ``propagation_alg`` "MPE" / "GeneralLL" (with ``alpha``) run ``pjamd_sum_forward_alg``."""
+        if propagation_alg != "LL":
+            return self._forward_alg(node_mars, element_mars, params, propagation_alg, kwargs.get("alpha"))
         prec = PRECISIONS[precision] if precision is not None else (1 if force_use_bf16 else 0)
         if force_use_fp32:
             prec = 0
@@ -307,6 +309,51 @@ class SumLayer(Layer, nn.Module):
             cache[key] = plan
         return cache[key]
 
+    @staticmethod
+    def _alg_id(propagation_alg: str, alpha: Optional[float]):
+        """(alg code, alpha) of ``Layer.propagation_alg_mapping`` (``layer.py:11-15, 89-97``)."""
+        if propagation_alg == "MPE":
+            return 1, 0.0
+        if propagation_alg == "GeneralLL":
+            if alpha is None:
+                raise KeyError("alpha")
+            if not alpha > 0.0:
+                raise ValueError("GeneralLL needs alpha > 0")
+            return 2, float(alpha)
+        if propagation_alg == "LL":
+            return 0, 1.0
+        raise ValueError(f"Unknown propagation algorithm {propagation_alg}.")
+
+    def _forward_alg(self, node_mars, element_mars, params, propagation_alg, alpha):
+        alg, alpha = self._alg_id(propagation_alg, alpha)
+        B = node_mars.size(1)
+        bs = self.block_size
+        for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
+            N.call("pjamd_sum_forward_alg", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
+                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, alg, alpha, N.stream(),
+                   algo=(4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B))
+
+    def _backward_alg(self, node_flows, element_flows, node_mars, element_mars, params, param_flows,
+                      propagation_alg, alpha, accumulate):
+        """Element flows into ``element_flows`` (then the product layer's backward) and parameter
+        flows for MPE / GeneralLL; node flows are read as stored."""
+        alg, alpha = self._alg_id(propagation_alg, alpha)
+        B = node_flows.size(1)
+        bs = self.block_size
+        s = N.stream()
+        for chids, parids, parpids, npar, cbs, totp in self._dev["bk"]:
+            N.call("pjamd_sum_backward_ele_alg", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
+                   N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar),
+                   chids.numel(), parids.size(1), cbs, bs, B, alg, alpha, 1 if accumulate else 0, s,
+                   algo=(4 * (2 * totp * bs + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs, 0))
+        if param_flows is not None:
+            pf64 = 1 if param_flows.dtype == torch.float64 else 0
+            for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
+                N.call("pjamd_sum_backward_par_alg", N.ptr(param_flows), pf64, N.ptr(node_flows), N.ptr(node_mars),
+                       N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),
+                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, alg, s,
+                       algo=(4 * (2 * nids.numel() * bs + tot) * B + 4 * tot * bs, 0))
+
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
                  element_mars: torch.Tensor, params: torch.Tensor, param_flows: Optional[torch.Tensor] = None,
                  allow_modify_flows: bool = False, propagation_alg: str = "LL", logspace_flows: bool = False,
@@ -323,7 +370,11 @@ class SumLayer(Layer, nn.Module):
         backward partition, see ``pjamd_sum_backward_ele_to_children``): write the flows of the
         product layer's children directly into ``node_flows`` instead of element flows.
         """
-        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented by the HIP sum layers"
+        if propagation_alg != "LL":
+            assert not logspace_flows and not allow_neg_flows and not negate_pflows, \
+                "unsupported flow mode for the HIP sum layers"
+            return self._backward_alg(node_flows, element_flows, node_mars, element_mars, params, param_flows,
+                                      propagation_alg, kwargs.get("alpha"), accumulate_ch_flows)
         assert not logspace_flows and not allow_neg_flows, "unsupported flow mode for the HIP sum layers"
         assert not (accumulate_ch_flows and allow_modify_flows)
         prec = PRECISIONS[precision] if precision is not None else 0

@@ -610,9 +610,55 @@ class TensorCircuit(nn.Module):
     # ------------------------------------------------------------------ passes
 
     def set_propagation_alg(self, propagation_alg: str, **kwargs):
-        if propagation_alg != "LL":
-            raise NotImplementedError(f"Propagation algorithm {propagation_alg} is not implemented by the HIP layers.")
-        self.default_propagation_alg = "LL"
+        """Default propagation algorithm of ``forward`` / ``backward``: "LL", "MPE", or
+        "GeneralLL" with ``alpha`` (``tensorcircuit.py:192-205``)."""
+        if propagation_alg in ("LL", "MPE"):
+            self.default_propagation_alg = propagation_alg
+            self.propagation_alg_kwargs.clear()
+        elif propagation_alg == "GeneralLL":
+            assert "alpha" in kwargs, "Argument `alpha` should be provided for the `GeneralLL` propagation algorithm."
+            self.default_propagation_alg = "GeneralLL"
+            self.propagation_alg_kwargs.clear()
+            self.propagation_alg_kwargs["alpha"] = kwargs["alpha"]
+        else:
+            raise NotImplementedError(f"Unknown propagation algorithm {propagation_alg}.")
+
+    def forward_ll(self, *args, **kwargs):
+        return self.forward(*args, propagation_alg="LL", **kwargs)
+
+    def forward_mpe(self, *args, **kwargs):
+        return self.forward(*args, propagation_alg="MPE", **kwargs)
+
+    def forward_general_ll(self, *args, alpha: float = 1.0, **kwargs):
+        """GeneralLL forward with exponent ``alpha`` (``tensorcircuit.py:546-547``, which drops
+        ``alpha`` before calling forward; here it is passed on)."""
+        return self.forward(*args, propagation_alg="GeneralLL", alpha=alpha, **kwargs)
+
+    def _run_inner_forward_alg(self, propagation_alg: str, alpha):
+        """Layer-wise forward for MPE / GeneralLL: product layers as for LL, sum layers through
+        ``pjamd_sum_forward_alg`` (no chains, fusion, split reductions or graph capture)."""
+        for gid, g in enumerate(self.inner_layer_groups):
+            if g.is_prod():
+                g(self.node_mars, self._slab(gid))
+            else:
+                for layer in g:
+                    layer(self.node_mars, self._slab(gid), self.params, propagation_alg=propagation_alg, alpha=alpha)
+
+    def _run_inner_backward_alg(self, compute_param_flows: bool, propagation_alg: str, alpha):
+        """Layer-wise flow backward for MPE / GeneralLL: each sum layer writes its element flows
+        (``pjamd_sum_backward_ele_alg``) and parameter flows (``pjamd_sum_backward_par_alg``), the
+        product layer below spreads the element flows to its children."""
+        for gid in range(len(self.inner_layer_groups) - 1, -1, -1):
+            g = self.inner_layer_groups[gid]
+            if g.is_prod():
+                g.backward(self.node_flows, self.element_flows)
+                continue
+            if self.recompute_elements:
+                self.inner_layer_groups[gid - 1](self.node_mars, self._slab(gid - 1))
+            for layer in g:
+                layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(gid), self.params,
+                               param_flows=self._pf if compute_param_flows else None,
+                               propagation_alg=propagation_alg, alpha=alpha)
 
     def _slab(self, group_id: int) -> torch.Tensor:
         """``element_mars`` rows of the product group ``group_id`` (or of the sum group right above it)."""
@@ -658,7 +704,9 @@ class TensorCircuit(nn.Module):
         """Log-likelihoods ``[B, num_roots]`` of ``inputs`` ``[B, num_vars]`` (``tensorcircuit.py:207-342``)."""
         if propagation_alg is None:
             propagation_alg = self.default_propagation_alg
-        assert propagation_alg == "LL", "only the LL propagation algorithm is implemented"
+            kwargs.update(self.propagation_alg_kwargs)
+        assert isinstance(propagation_alg, str), "per-layer propagation algorithms are not supported"
+        alpha = kwargs.pop("alpha", None)
         prec = precision if precision is not None else ("bf16" if force_use_bf16 else self.precision)
         if force_use_fp32:
             prec = "fp32"
@@ -685,7 +733,9 @@ class TensorCircuit(nn.Module):
 
             sig = (0, id(self.node_mars), id(self.element_mars), id(self.params), B, prec)
             run = lambda: self._run_inner_forward(force_use_bf16, force_use_fp32, prec)
-            if record_cudagraph and sig not in self._recorded_cuda_graphs:
+            if propagation_alg != "LL":
+                self._run_inner_forward_alg(propagation_alg, alpha)
+            elif record_cudagraph and sig not in self._recorded_cuda_graphs:
                 run()  # this call's result; the capture below does not execute kernels
                 self._recorded_cuda_graphs[sig] = self._capture(run)
             elif apply_cudagraph and sig in self._recorded_cuda_graphs:
@@ -699,6 +749,8 @@ class TensorCircuit(nn.Module):
             cache["node_mars"] = self.node_mars.clone()
         if torch.is_grad_enabled():
             lls.requires_grad = True
+            if alpha is not None:
+                kwargs["alpha"] = alpha
             lls.register_hook(partial(_pc_model_backward_hook, pc=self, inputs=origin_inputs,
                                       record_cudagraph=record_cudagraph, apply_cudagraph=apply_cudagraph,
                                       propagation_alg=propagation_alg, precision=prec, **kwargs))
@@ -775,8 +827,9 @@ class TensorCircuit(nn.Module):
         marginalized input entry's flow is spread over its categories by ``theta``."""
         missing_mask = kwargs.get("missing_mask")
         batch_first = kwargs.get("_batch_first", True)
-        assert propagation_alg == "LL" and not logspace_flows and not negate_pflows, \
-            "only LL flows in linear space are implemented"
+        assert isinstance(propagation_alg, str), "per-layer propagation algorithms are not supported"
+        assert not logspace_flows and not negate_pflows, "only flows in linear space are implemented"
+        alpha = kwargs.pop("alpha", None)
         assert sum_layer_pre_backward_callback is None and sum_layer_post_backward_callback is None, \
             "sum-layer callbacks are not supported"
         assert self.node_mars is not None and self.element_mars is not None, "Should run forward path first."
@@ -788,7 +841,7 @@ class TensorCircuit(nn.Module):
         with torch.no_grad():
             B = self.node_mars.size(1)
             self._ensure_buffers(B, need_flows=True)
-            if prec == "fp32_valu" or "zero_ranges" not in self._fused:
+            if prec == "fp32_valu" or propagation_alg != "LL" or "zero_ranges" not in self._fused:
                 self._fill(self.node_flows, 0.0)
             else:
                 for a, b in self._fused["zero_ranges"]:
@@ -808,13 +861,16 @@ class TensorCircuit(nn.Module):
             # The first backward after init_param_flows(0.0) stores the sum parameter flows instead
             # of adding them when every slot has exactly one writer; otherwise the buffer is
             # cleared first.
-            overwrite = compute_param_flows and self._pf_fresh and self._pf_overwrite_ok
+            overwrite = (compute_param_flows and self._pf_fresh and self._pf_overwrite_ok
+                         and propagation_alg == "LL")
             if compute_param_flows and not overwrite:
                 _ = self.param_flows
             sig = (1, id(self.node_flows), id(self.element_flows), id(self.node_mars), id(self.element_mars),
                    id(self.params), id(self._pf), B, allow_modify_flows, compute_param_flows, prec, overwrite)
             run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec, overwrite)
-            if record_cudagraph and sig not in self._recorded_cuda_graphs:
+            if propagation_alg != "LL":
+                self._run_inner_backward_alg(compute_param_flows, propagation_alg, alpha)
+            elif record_cudagraph and sig not in self._recorded_cuda_graphs:
                 run()  # this call's result; the capture below does not execute kernels
                 self._recorded_cuda_graphs[sig] = self._capture(run)
             elif apply_cudagraph and sig in self._recorded_cuda_graphs:

@@ -638,3 +638,73 @@ def test_tied_copies_share_parameter_flow_tiles(shape, precision, device):
             assert U.rel_err(got, ref) <= tol, (n, U.rel_err(got, ref))
             checked += 1
     assert checked > 0
+
+
+@pytest.mark.parametrize("name", ["simple_model", "tying_block1", "hclt_cat_bs32", "hclt_cat_bs64", "hmm_small",
+                                  "pd_small", "pd_sparse"])
+@pytest.mark.parametrize("alg,alpha", [("MPE", None), ("GeneralLL", 0.5), ("GeneralLL", 2.0)])
+def test_propagation_algorithms(name, alg, alpha, device):
+    """MPE and GeneralLL forward / flow backward / parameter flows (pjamd_sum_*_alg) against the
+    FP64 oracle: log-values of every vector, node flows, sum and input parameter flows."""
+    U.say(f"[propagation] {name} {alg} {alpha}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.to(device)
+    X = U.random_data(cards, 32, seed=23)
+    xt = torch.from_numpy(X).to(device)
+    extra = {} if alpha is None else {"alpha": alpha}
+    lls = pc(xt, propagation_alg=alg, **extra).detach().cpu().numpy()[:, 0]
+    pc.backward(xt, propagation_alg=alg, flows_memory=0.0, **extra)
+    pc.update_parameters()
+    a = 1.0 if alpha is None else alpha
+    vals = dag_eval.forward(ns, X, alg=alg, alpha=a)
+    flows, pflows = dag_eval.backward(ns, X, vals, alg=alg, alpha=a)
+    V = len(cards)
+    assert np.abs(lls - vals[ns][0]).max() <= 1e-4 * V
+    nm = pc.node_mars.detach().cpu().numpy()
+    nf = pc.node_flows.detach().cpu().numpy()
+    for n in node_iterator(ns):
+        if n.is_sum() or n.is_input():
+            a0, a1 = n._output_ind_range
+            d = np.abs(nm[a0:a1] - vals[n])
+            d[np.isinf(vals[n]) & np.isinf(nm[a0:a1])] = 0
+            assert d.max() <= 1e-4 * len(n.scope) + 1e-5, (n, d.max())
+            assert U.rel_err(nf[a0:a1], flows[n]) <= 1e-4, (n, U.rel_err(nf[a0:a1], flows[n]))
+    pc.update_param_flows(origin_ns_only=False)
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            ref = pflows[n]
+            for m in node_iterator(ns):
+                if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
+                    ref = ref + pflows[m]
+            assert U.rel_err(got, ref) <= 1e-4, (n, U.rel_err(got, ref))
+    for layer in pc.input_layer_group:
+        pfi = layer.param_flows.detach().cpu().numpy().astype(np.float64)
+        for n in layer.nodes:
+            a0, a1 = n._param_flow_range
+            got = pfi[a0:a1].reshape(n.num_nodes, n.dist.num_param_flows())
+            ref = sum(pflows[m] for m in layer.nodes if m._param_flow_range == n._param_flow_range)
+            assert U.rel_err(got, ref) <= 1e-4, (n, U.rel_err(got, ref))
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs32", "hmm_small", "pd_small"])
+def test_general_ll_alpha_one_matches_ll_path(name, device):
+    """The vector-ALU propagation kernels with GeneralLL alpha = 1 against the main LL path
+    (MFMA / tiled kernels, fused backward): same log-likelihoods and parameter flows to FP32
+    rounding."""
+    U.say(f"[propagation vs LL] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision="fp32", **kw)
+    pc.to(device)
+    xt = torch.from_numpy(U.random_data(cards, 32, seed=29)).to(device)
+    out = dict()
+    for alg in ("LL", "GeneralLL"):
+        lls = pc(xt, propagation_alg=alg, alpha=1.0).detach().double().cpu().numpy()
+        pc.backward(xt, propagation_alg=alg, alpha=1.0, flows_memory=0.0)
+        out[alg] = (lls, pc.param_flows.detach().double().cpu().numpy(),
+                    [l.param_flows.detach().double().cpu().numpy() for l in pc.input_layer_group])
+    assert np.abs(out["LL"][0] - out["GeneralLL"][0]).max() <= 1e-5 * len(cards)
+    assert U.rel_err(out["GeneralLL"][1], out["LL"][1]) <= 1e-4
+    for a, b in zip(out["GeneralLL"][2], out["LL"][2]):
+        assert U.rel_err(a, b) <= 1e-4
