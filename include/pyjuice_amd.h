@@ -72,8 +72,8 @@ const char* pjamd_error_string(int code);
  * hipMemsetAsync, so it can be recorded into HIP graphs. */
 int pjamd_fill_f32(float* p, float v, int64_t n, pjamd_stream_t stream);
 
-/* out[v][b] = (int32) src[b][v] for a [B][V] batch of one-byte (uint8 / bool, src_code 0),
- * int32 (src_code 1) or int64 (src_code 2) values.  Replaces `inputs.permute(1, 0)` of TensorCircuit.forward
+/* out[v][b] = (int32) src[b][v] for a [B][V] batch of one-byte (uint8 / bool, src_code 0; int8,
+ * src_code 3), int32 (src_code 1) or int64 (src_code 2) values.  Replaces `inputs.permute(1, 0)` of TensorCircuit.forward
  * (tensorcircuit.py:226-230). */
 int pjamd_data_to_vb(int* out, int src_code, const void* src, int64_t B, int64_t V, pjamd_stream_t stream);
 

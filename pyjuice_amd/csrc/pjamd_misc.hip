@@ -215,6 +215,7 @@ PJAMD_EXPORT int pjamd_data_to_vb(int* out, int src_code, const void* src, int64
         case 0: to_vb_kernel<uint8_t><<<grid, 256, 0, stream>>>(out, (const uint8_t*)src, B, V); break;
         case 1: to_vb_kernel<int32_t><<<grid, 256, 0, stream>>>(out, (const int32_t*)src, B, V); break;
         case 2: to_vb_kernel<int64_t><<<grid, 256, 0, stream>>>(out, (const int64_t*)src, B, V); break;
+        case 3: to_vb_kernel<int8_t><<<grid, 256, 0, stream>>>(out, (const int8_t*)src, B, V); break;
         default: return PJAMD_EINVAL;
     }
     PJAMD_LAUNCH_CHECK();

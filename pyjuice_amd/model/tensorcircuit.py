@@ -33,7 +33,7 @@ from .backend import compile_cum_par_flows_fn, compile_par_update_fn, normalize_
 
 __all__ = ["TensorCircuit", "compile"]
 
-_DTYPE_CODE = {torch.uint8: 0, torch.bool: 0, torch.int8: 0, torch.int32: 1, torch.int64: 2}
+_DTYPE_CODE = {torch.uint8: 0, torch.bool: 0, torch.int8: 3, torch.int32: 1, torch.int64: 2}
 
 
 def _pc_model_backward_hook(grad, pc, inputs, record_cudagraph, apply_cudagraph, propagation_alg, **kwargs):

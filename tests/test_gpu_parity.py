@@ -708,3 +708,58 @@ def test_general_ll_alpha_one_matches_ll_path(name, device):
     assert U.rel_err(out["GeneralLL"][1], out["LL"][1]) <= 1e-4
     for a, b in zip(out["GeneralLL"][2], out["LL"][2]):
         assert U.rel_err(a, b) <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["hclt_bern16", "hclt_cat_bs32"])
+def test_input_dtypes(name, device):
+    """Data given as bool / uint8 / int8 / int32 / int64 (pjamd_data_to_vb) gives bit-identical
+    log-likelihoods, equal to the FP64 oracle; a negative int8 entry stays negative (marginalized)
+    instead of wrapping to 255."""
+    U.say(f"[dtypes] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision="fp32", **kw)
+    pc.to(device)
+    X = U.random_data(cards, 24, seed=31)
+    dts = [torch.uint8, torch.int8, torch.int32, torch.int64] + ([torch.bool] if max(cards) == 2 else [])
+    out = [pc(torch.from_numpy(X).to(dt).to(device)).detach().cpu().numpy()[:, 0] for dt in dts]
+    for o in out[1:]:
+        assert np.array_equal(o, out[0])
+    pc.update_parameters()
+    assert np.abs(out[0] - dag_eval.forward(ns, X)[ns][0]).max() <= 1e-4 * len(cards)
+    Xm = X.copy()
+    Xm[:, 0] = -1
+    a = pc(torch.from_numpy(Xm).to(torch.int8).to(device)).detach().cpu().numpy()[:, 0]
+    mask = torch.zeros(len(cards), dtype=torch.bool)
+    mask[0] = True
+    b = pc(torch.from_numpy(X).to(device), missing_mask=mask.to(device)).detach().cpu().numpy()[:, 0]
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs32", "hclt_cat_bs64", "hmm_small", "pd_sparse"])
+@pytest.mark.parametrize("B", [1, 4])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_tiny_batches(name, B, precision, device):
+    """Batches of 1 and 4 samples (partial tiles and one 16-byte column group) through forward,
+    flow backward and parameter flows, against the FP64 oracle."""
+    U.say(f"[tiny batch] {name} B={B} {precision}")
+    t_ll, t_mar, t_off, t_flow, t_em = TOL[precision]
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = U.random_data(cards, B, seed=37)
+    xt = torch.from_numpy(X).to(device)
+    lls = pc(xt).detach().cpu().numpy()[:, 0]
+    pc.backward(xt, flows_memory=0.0)
+    pc.update_parameters()
+    vals = dag_eval.forward(ns, X)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    assert np.abs(lls - vals[ns][0]).max() <= t_ll * len(cards)
+    pc.update_param_flows(origin_ns_only=False)
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            ref = pflows[n]
+            for m in node_iterator(ns):
+                if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
+                    ref = ref + pflows[m]
+            assert U.rel_err(got, ref) <= t_flow, (n, U.rel_err(got, ref))
