@@ -768,6 +768,7 @@ PJAMD_EXPORT int pjamd_sum_forward_alg(float* node_mars, const float* element_ma
                   (int64_t)n_blocks * bs < ((int64_t)1 << 31));
     PJAMD_REQUIRE(alg != 2 || alpha > 0.0f);
     if (n_blocks == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(cdiv(B, kThreads) < 65536u);
     dim3 grid((unsigned)(n_blocks * bs), cdiv(B, kThreads));
     sum_fwd_alg_kernel<<<grid, kThreads, 0, stream>>>(node_mars, element_mars, params, nids, cids, pids, nch, K, bs, B,
                                                       alg, alg == 0 ? 1.0f : alpha);
@@ -784,6 +785,7 @@ PJAMD_EXPORT int pjamd_sum_backward_ele_alg(float* element_flows, const float* n
                   (int64_t)n_cblocks * cbs < ((int64_t)1 << 31));
     PJAMD_REQUIRE(alg != 2 || alpha > 0.0f);
     if (n_cblocks == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(cdiv(B, kThreads) < 65536u);
     dim3 grid((unsigned)(n_cblocks * cbs), cdiv(B, kThreads));
     sum_bwd_ele_alg_kernel<<<grid, kThreads, 0, stream>>>(element_flows, node_flows, node_mars, element_mars, params,
                                                           chids, parids, parpids, n_pars, KP, cbs, bs, B, alg,
@@ -799,6 +801,7 @@ PJAMD_EXPORT int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const f
                                             hipStream_t stream) {
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && alg >= 0 && alg <= 2 &&
                   (int64_t)n_blocks * bs < ((int64_t)1 << 31));
+    PJAMD_REQUIRE(cdiv(K, kThreads / 64) < 65536u);
     if (n_blocks == 0) return PJAMD_OK;
     dim3 grid((unsigned)(n_blocks * bs), cdiv(K, kThreads / 64));
     if (pf64)

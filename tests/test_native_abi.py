@@ -95,3 +95,21 @@ def test_argument_checks_fail_before_launch(lib):
     assert lib.pjamd_prod_forward(null, null, null, null, 0, 2, 4, 8, 0, null) == 0
     assert lib.pjamd_sum_backward_par_batched(null, null, null, null, null, null, null, null, null, null, null,
                                               null, 0, 64, 8, 0, 1, null) == 0
+
+
+def test_argument_checks_of_query_and_split_entry_points(lib):
+    null = ctypes.c_void_p(0)
+    p7, p9 = [null] * 7, [null] * 9
+    # propagation algorithms: unknown alg, GeneralLL without a positive alpha, zero batch
+    assert lib.pjamd_sum_forward_alg(*p7, 1, 4, 32, 8, 3, 1.0, null) == -1
+    assert lib.pjamd_sum_forward_alg(*p7, 1, 4, 32, 8, 2, 0.0, null) == -1
+    assert lib.pjamd_sum_forward_alg(*p7, 0, 4, 32, 8, 1, 0.0, null) == 0
+    assert lib.pjamd_sum_backward_ele_alg(*p9, 1, 2, 32, 32, 8, 2, -1.0, 0, null) == -1
+    assert lib.pjamd_sum_backward_par_alg(null, 0, *p9, 1, 4, 32, 0, 1, null) == -1
+    # tied-range input backward: Categorical only, category counts up to 16384
+    assert lib.pjamd_input_backward_tied(1, null, null, 0, *([null] * 6), 4, 2, 0, 8, 0, null) == -2
+    assert lib.pjamd_input_backward_tied(0, null, null, 0, *([null] * 6), 4, 20000, 0, 8, 0, null) == -2
+    assert lib.pjamd_input_backward_tied(0, null, null, 0, *([null] * 6), 0, 16, 0, 8, 0, null) == 0
+    # split reductions: 2 .. 16 splits
+    assert lib.pjamd_sum_forward_split(*p7, 1, 1024, 1024, 256, 1024, 1, null, 1, null) == -1
+    assert lib.pjamd_sum_forward_split(*p7, 1, 1024, 1024, 256, 1024, 17, null, 1, null) == -1
