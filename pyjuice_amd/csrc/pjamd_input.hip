@@ -321,7 +321,8 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
 // one uniform tile of the caller's promise): a variable's counting sort is done once for all the
 // group's tiles of that variable (the tiles of one variable are consecutive), each tile's flow
 // rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
-// staging takes TILE * SB * 4 bytes of LDS (32 KiB for both instantiations).
+// staging takes TILE * SB * 4 bytes of LDS: 16 KiB (8 x 512 columns, 2 x 2048 columns) or 32 KiB
+// (8 x 1024 columns); the smaller forms keep more workgroups resident.
 constexpr int kGroupNodes = 64;
 
 template <typename PF, int TILE, int SB>
@@ -735,11 +736,14 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
     } else if (sorted_tiles && kmax <= kSortK && B <= 2048 && B % 4 == 0 &&
                ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0) && (node_offset * B) % 4 == 0) {
         const unsigned grid = (unsigned)cdiv(num_nodes, kGroupNodes);
-        if (B <= 1024)
+        if (B <= 512)   // 16 KiB of staged flows: twice the resident workgroups of the 1024-column form
+            cat_bwd_group_kernel<PF, 8, 512><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
+                                                                        num_nodes, node_offset, B, pf_mode);
+        else if (B <= 1024)
             cat_bwd_group_kernel<PF, 8, 1024><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
                                                                         num_nodes, node_offset, B, pf_mode);
         else
-            cat_bwd_group_kernel<PF, 4, 2048><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
+            cat_bwd_group_kernel<PF, 2, 2048><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
                                                                         num_nodes, node_offset, B, pf_mode);
     } else if (sorted_tiles && kmax <= kSortK) {
         cat_bwd_sorted_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile), 256, 0, stream>>>(
