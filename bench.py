@@ -36,9 +36,9 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0
 # profiles/hclt784_L256_B512_bf16_hbm_traffic.txt.  Reported as roofline.traffic for the dominant
 # kernel (counters cannot be read from inside the timed process).
 TRAFFIC_MB_PER_LAUNCH = {
-    "pjamd_sum_backward_ele_to_children": 34.02, "pjamd_sum_forward": 15.15,
-    "pjamd_sum_backward_par_batched": 1205.71, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 833.87,
-    "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.85, "pjamd_input_em": 847.84,
+    "pjamd_sum_backward_ele_to_children": 34.04, "pjamd_sum_forward": 15.14,
+    "pjamd_sum_backward_par_batched": 1220.77, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 833.97,
+    "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.85, "pjamd_input_em": 847.86,
 }
 HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
                "pjamd_input_em", "pjamd_sum_em", "pjamd_sum_modify_flows", "pjamd_pflow_fuse")
