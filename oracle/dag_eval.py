@@ -13,7 +13,8 @@ Maths (reference citations):
 * flows: ``f_c += sum_n f_n W[n, c] exp(v_c - v_n)``; parameter flows
   ``pf[n, c] = W[n, c] sum_b f_n exp(v_c - v_n)``                    (sum_layer.py:1569-1730, 2049-2169)
 * EM: ``theta' = (1 - s) theta + s (pf + pc / nch) / (sum pf + pc)`` (par_update.py:214-256,
-  categorical.py:164-201, bernoulli.py:91-101)
+  categorical.py:164-201, bernoulli.py:91-101); with ``keep_zero_params`` zero parameters stay
+  zero and the normaliser runs over the others (par_update.py:196-205, 236-253)
 * other propagation algorithms (``alg`` of forward / backward; lt = log W[n, c] + v_c):
   MPE ``v[n] = max_c lt``, flows and parameter flows (not scaled by W) where |lt - v[n]| < 1e-6;
   GeneralLL ``v[n] = log sum_c exp(alpha lt) / alpha``, flows ``f_c += f_n exp(alpha (lt - v_n))``,
@@ -212,31 +213,46 @@ def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None, al
     return flows, pflows
 
 
-def em_update(root, pflows: Dict, step_size: float, pseudocount: float = 0.0) -> Dict:
+def em_update(root, pflows: Dict, step_size: float, pseudocount: float = 0.0,
+              keep_zero_params: bool = False) -> Dict:
     """New parameters of every source vector after one EM step: sum vectors as dense
-    ``[num_nodes, num_ch_nodes]`` (zeros off the edges), inputs as ``[num_nodes, K]``."""
+    ``[num_nodes, num_ch_nodes]`` (zeros off the edges), inputs as ``[num_nodes, K]``.
+
+    A node with n children (sum edges, or K categories) and flows pf_c is moved towards
+    ``(pf_c + pc / n) / (sum_c pf_c + pc)`` (``par_update.py:214-256``, ``categorical.py:164-201``,
+    ``bernoulli.py:91-101`` with ``pc / 2``).  With ``keep_zero_params`` (sum and Categorical
+    vectors; ``par_update.py:196-205, 236-253``, ``categorical.py:171-201``) parameters below
+    1e-12 stay 0 and the normaliser sums ``pf_c + pc / n`` over the other children only (n still
+    counts every child)."""
     tot = dict()
     for ns, pf in pflows.items():
         s = _src(ns)
         tot[s] = tot.get(s, 0.0) + pf
     out = dict()
+
+    def _step(th, pf, mask, n):
+        num = pf + pseudocount / n
+        if keep_zero_params:
+            live = mask & (th >= 1e-12)
+            den = np.where(live, num, 0.0).sum(axis=1, keepdims=True)
+            new = num / np.where(den > 0, den, 1.0)
+            upd = (1.0 - step_size) * th + step_size * new
+            return np.where(live, upd, 0.0)
+        den = np.where(mask, pf, 0.0).sum(axis=1, keepdims=True) + pseudocount
+        return np.where(mask, (1.0 - step_size) * th + step_size * num / den, 0.0)
+
     for s, pf in tot.items():
         if s.is_sum():
             W = sum_dense_params(s)
-            mask = np.zeros_like(W, dtype=bool)
             E = _np(s.edge_ids)
             mb = np.zeros([s.num_node_blocks, s.block_size, s.num_ch_node_blocks, s.ch_block_size], dtype=bool)
             mb[E[0], :, E[1], :] = True
             mask = mb.reshape(W.shape)
             nch = mask.sum(axis=1, keepdims=True).astype(np.float64)
-            pfm = np.where(mask, pf, 0.0)
-            new = (pfm + pseudocount / nch) / (pfm.sum(axis=1, keepdims=True) + pseudocount)
-            out[s] = np.where(mask, (1.0 - step_size) * W + step_size * new, 0.0)
+            out[s] = _step(W, np.where(mask, pf, 0.0), mask, nch)
         elif s.dist.get_signature() == "Categorical":
             th = input_params(s)
-            K = th.shape[1]
-            new = (pf + pseudocount / K) / (pf.sum(axis=1, keepdims=True) + pseudocount)
-            out[s] = (1.0 - step_size) * th + step_size * new
+            out[s] = _step(th, pf, np.ones_like(th, dtype=bool), float(th.shape[1]))
         else:
             th = input_params(s)[:, 0]
             new = (pf[:, 0] + 0.5 * pseudocount) / (pf[:, 0] + pf[:, 1] + pseudocount)

@@ -69,19 +69,28 @@ void pcc_input_backward(int kind, real* pflows, const real* node_flows, const in
     }
 }
 
-/* theta <- (1 - s) theta + s (f + pc / K) / (sum f + pc) for the source nodes src[]. */
+/* theta <- (1 - s) theta + s (f + pc / K) / (sum f + pc) for the source nodes src[].  keep_zero
+ * (Categorical): parameters below 1e-12 stay 0 and the normaliser is sum over the others of
+ * (f + pc / K) (categorical.py:171-201). */
 void pcc_input_em(int kind, real* params, const real* pflows, const int64_t* src, const int64_t* s_pids,
-                  const int64_t* s_pfids, const int64_t* ncats, int64_t num_src, double step, double pc) {
+                  const int64_t* s_pfids, const int64_t* ncats, int64_t num_src, double step, double pc,
+                  int keep_zero) {
 #pragma omp parallel for schedule(static)
     for (int64_t w = 0; w < num_src; ++w) {
         const int64_t i = src[w];
         if (kind == 0) {
             const int64_t K = ncats[i];
-            double cum = pc;
-            for (int64_t k = 0; k < K; ++k) cum += pflows[s_pfids[i] + k];
+            const double npc = pc / (double)K;
+            double cum = keep_zero ? 0.0 : pc;
             for (int64_t k = 0; k < K; ++k) {
-                const double nv = ((double)pflows[s_pfids[i] + k] + pc / (double)K) / cum;
-                params[s_pids[i] + k] = (real)((1.0 - step) * params[s_pids[i] + k] + step * nv);
+                const double f = pflows[s_pfids[i] + k];
+                if (!keep_zero) cum += f;
+                else if (params[s_pids[i] + k] >= 1e-12) cum += f + npc;
+            }
+            for (int64_t k = 0; k < K; ++k) {
+                const double th = params[s_pids[i] + k];
+                const double nv = ((double)pflows[s_pfids[i] + k] + npc) / cum;
+                params[s_pids[i] + k] = (keep_zero && th < 1e-12) ? (real)0 : (real)((1.0 - step) * th + step * nv);
             }
         } else {
             const double f1 = pflows[s_pfids[i]], f0 = pflows[s_pfids[i] + 1];
@@ -189,19 +198,27 @@ void pcc_sum_par(real* pflows, const real* nflows, const real* nmars, const real
 }
 
 /* Sum EM over node blocks rec = {par_start, pflow_start, node_start, bs, nch}: the parameter of
- * child c of node i is at par_start + c bs + i. */
-void pcc_sum_em(real* params, const real* pflows, const int64_t* recs, int64_t n_recs, double step, double pc) {
+ * child c of node i is at par_start + c bs + i.  keep_zero: parameters below 1e-12 stay 0 and the
+ * normaliser is the sum over the others of (pf + pc / nch) (par_update.py:196-205, 236-253). */
+void pcc_sum_em(real* params, const real* pflows, const int64_t* recs, int64_t n_recs, double step, double pc,
+                int keep_zero) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t q = 0; q < n_recs; ++q) {
         const int64_t* R = recs + q * 5;
         const int64_t par0 = R[0], pf0 = R[1], bs = R[3], nch = R[4];
+        const double npc = pc / (double)nch;
         for (int64_t i = 0; i < bs; ++i) {
-            double cum = pc;
-            for (int64_t c = 0; c < nch; ++c) cum += pflows[pf0 + c * bs + i];
+            double cum = keep_zero ? 0.0 : pc;
+            for (int64_t c = 0; c < nch; ++c) {
+                const double f = pflows[pf0 + c * bs + i];
+                if (!keep_zero) cum += f;
+                else if (params[par0 + c * bs + i] >= 1e-12) cum += f + npc;
+            }
             for (int64_t c = 0; c < nch; ++c) {
                 const int64_t o = c * bs + i;
-                const double nv = ((double)pflows[pf0 + o] + pc / (double)nch) / cum;
-                params[par0 + o] = (real)((1.0 - step) * params[par0 + o] + step * nv);
+                const double th = params[par0 + o];
+                const double nv = ((double)pflows[pf0 + o] + npc) / cum;
+                params[par0 + o] = (keep_zero && th < 1e-12) ? (real)0 : (real)((1.0 - step) * th + step * nv);
             }
         }
     }

@@ -23,12 +23,12 @@ _p, _i, _l, _d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_doubl
 _SIGS = {
     "pcc_input_forward": [_i, _p, _p, _p, _p, _p, _p, _l, _l, _l],
     "pcc_input_backward": [_i, _p, _p, _p, _p, _p, _p, _l, _l, _l],
-    "pcc_input_em": [_i, _p, _p, _p, _p, _p, _p, _l, _d, _d],
+    "pcc_input_em": [_i, _p, _p, _p, _p, _p, _p, _l, _d, _d, _i],
     "pcc_prod": [_p, _p, _p, _p, _l, _l, _l, _l, _i],
     "pcc_sum_forward": [_p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _l],
     "pcc_sum_ele": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _l, _l, _i],
     "pcc_sum_par": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _l, _l, _l, _l],
-    "pcc_sum_em": [_p, _p, _p, _l, _d, _d],
+    "pcc_sum_em": [_p, _p, _p, _l, _d, _d, _i],
     "pcc_pflow_fuse": [_p, _l, _l, _p, _l],
 }
 _libs = dict()
@@ -167,15 +167,16 @@ class CompiledCPU:
         for d in self.inputs:
             d["pflows"][:] = 0.0
 
-    def mini_batch_em(self, step_size: float, pseudocount: float = 0.0):
+    def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False):
         L = self.L
         for d in self.inputs:
             for t, n, ch in d["tied"]:
                 L.pcc_pflow_fuse(_P(d["pflows"]), t, n, _P(ch), ch.size)
             L.pcc_input_em(d["kind"], _P(d["params"]), _P(d["pflows"]), _P(d["src"]), _P(d["s_pids"]),
-                           _P(d["s_pfids"]), _P(d["ncats"]), d["src"].size, step_size, pseudocount)
+                           _P(d["s_pfids"]), _P(d["ncats"]), d["src"].size, step_size, pseudocount,
+                           int(keep_zero_params and d["kind"] == 0))
         for t, n, ch in self.fuse:
             L.pcc_pflow_fuse(_P(self.param_flows), t, n, _P(ch), ch.size)
         if self.node_blocks.size > 0:
             L.pcc_sum_em(_P(self.params), _P(self.param_flows), _P(self.node_blocks), self.node_blocks.shape[0],
-                         step_size, pseudocount)
+                         step_size, pseudocount, int(keep_zero_params))

@@ -347,10 +347,13 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
         if (i0 >= num_nodes) break;
         const int n_here = (int)((num_nodes - i0) < TILE ? (num_nodes - i0) : TILE);
         const int v = vids[i0];
-        if (v != cur_v) {
+        const int k_tile = ncats[i0];
+        // Tiles of one variable may differ in category count (two Categorical vectors of
+        // different num_cats over the same variable): the sort is redone when either changes.
+        if (v != cur_v || k_tile != K) {
             // counting sort of the variable's categories: cnt[k] = first position of category k
             cur_v = v;
-            K = ncats[i0];
+            K = k_tile;
             const int* xrow = data + (int64_t)v * B;
             for (int k = tid; k <= K; k += 256) cnt[k] = 0;
             __syncthreads();
