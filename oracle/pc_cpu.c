@@ -118,60 +118,135 @@ void pcc_prod(real* out, const real* in, const int64_t* nids, const int64_t* cid
     }
 }
 
+/* The three sum-layer passes below factor every exp(x - y) through a shift per (node block,
+ * column): the exponentials are taken once per (child row, column) and once per (parent row,
+ * column), and the contraction over the block's rows is a plain multiply-add, so the cost is that
+ * of a dense product rather than one exponential per (edge, sample).  Columns are processed in
+ * chunks of PCC_BC; OpenMP runs over (block, chunk) tasks.  Where a shifted exponent would
+ * overflow (possible only next to zero parameters), the term is summed directly instead. */
+#define PCC_BC 64
+#define PCC_EXP_MAX ((sizeof(real) == sizeof(double)) ? 700.0 : 80.0)
+
+static inline real pcc_exp_or_zero(real x) { return (x == NEG_INF) ? (real)0 : (real)exp(x); }
+
 /* node_mars[nids[j] + r][b] = log sum_k theta[pids[j][k] + r] exp(e[cids[j][k]][b]) over the
- * children k < nch[j], with the per-(node, sample) maximum subtracted. */
+ * children k < nch[j]; per column the children's maximum is subtracted before the exponential
+ * (which is then independent of r). */
 void pcc_sum_forward(real* node_mars, const real* emars, const real* params, const int64_t* nids,
                      const int64_t* cids, const int64_t* pids, const int64_t* nch, int64_t n_blocks, int64_t K,
                      int64_t bs, int64_t B) {
+    const int64_t nc = (B + PCC_BC - 1) / PCC_BC;
 #pragma omp parallel
     {
-        real* mx = (real*)malloc(sizeof(real) * B);
-        real* acc = (real*)malloc(sizeof(real) * B);
-#pragma omp for schedule(static)
-        for (int64_t t = 0; t < n_blocks * bs; ++t) {
-            const int64_t j = t / bs, r = t % bs;
+        real* acc = (real*)malloc(sizeof(real) * bs * PCC_BC);
+        real mx[PCC_BC], ex[PCC_BC];
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < n_blocks * nc; ++t) {
+            const int64_t j = t / nc, b0 = (t % nc) * PCC_BC;
+            const int64_t nb = (B - b0) < PCC_BC ? (B - b0) : PCC_BC;
             const int64_t n = nch[j];
-            for (int64_t b = 0; b < B; ++b) { mx[b] = NEG_INF; acc[b] = 0; }
+            for (int64_t b = 0; b < nb; ++b) mx[b] = NEG_INF;
             for (int64_t k = 0; k < n; ++k) {
-                const real* e = emars + cids[j * K + k] * B;
-                for (int64_t b = 0; b < B; ++b) if (e[b] > mx[b]) mx[b] = e[b];
+                const real* e = emars + cids[j * K + k] * B + b0;
+                for (int64_t b = 0; b < nb; ++b) if (e[b] > mx[b]) mx[b] = e[b];
             }
+            for (int64_t q = 0; q < bs * nb; ++q) acc[q] = 0;
             for (int64_t k = 0; k < n; ++k) {
-                const real* e = emars + cids[j * K + k] * B;
-                const real th = params[pids[j * K + k] + r];
-                for (int64_t b = 0; b < B; ++b)
-                    if (mx[b] != NEG_INF) acc[b] += th * exp(e[b] - mx[b]);
+                const real* e = emars + cids[j * K + k] * B + b0;
+                for (int64_t b = 0; b < nb; ++b) ex[b] = (mx[b] == NEG_INF) ? (real)0 : pcc_exp_or_zero(e[b] - mx[b]);
+                const real* th = params + pids[j * K + k];
+                for (int64_t r = 0; r < bs; ++r) {
+                    const real w = th[r];
+                    real* a = acc + r * nb;
+                    for (int64_t b = 0; b < nb; ++b) a[b] += w * ex[b];
+                }
             }
-            real* o = node_mars + (nids[j] + r) * B;
-            for (int64_t b = 0; b < B; ++b) o[b] = (acc[b] > 0 && mx[b] != NEG_INF) ? log(acc[b]) + mx[b] : NEG_INF;
+            for (int64_t r = 0; r < bs; ++r) {
+                real* o = node_mars + (nids[j] + r) * B + b0;
+                const real* a = acc + r * nb;
+                for (int64_t b = 0; b < nb; ++b)
+                    o[b] = (a[b] > 0 && mx[b] != NEG_INF) ? (real)log(a[b]) + mx[b] : NEG_INF;
+            }
         }
-        free(mx);
         free(acc);
     }
 }
 
+/* R[i][b] = sign(f) exp(log|f| - m - s[b]) for the rows i < bs starting at row0, with
+ * s[b] = max_i (log|f| - m) (-inf where no row has flow). */
+static void pcc_scaled_flows(real* R, real* s, const real* nflows, const real* nmars, int64_t row0, int64_t bs,
+                             int64_t b0, int64_t nb, int64_t B) {
+    for (int64_t b = 0; b < nb; ++b) s[b] = NEG_INF;
+    for (int64_t i = 0; i < bs; ++i) {
+        const real* f = nflows + (row0 + i) * B + b0;
+        const real* m = nmars + (row0 + i) * B + b0;
+        real* l = R + i * nb;
+        for (int64_t b = 0; b < nb; ++b) {
+            l[b] = (f[b] != 0 && m[b] != NEG_INF) ? (real)log(fabs(f[b])) - m[b] : NEG_INF;
+            if (l[b] > s[b]) s[b] = l[b];
+        }
+    }
+    for (int64_t i = 0; i < bs; ++i) {
+        const real* f = nflows + (row0 + i) * B + b0;
+        real* l = R + i * nb;
+        for (int64_t b = 0; b < nb; ++b) {
+            const real v = (l[b] == NEG_INF) ? (real)0 : (real)exp(l[b] - s[b]);
+            l[b] = f[b] < 0 ? -v : v;
+        }
+    }
+}
+
 /* eflows[chids[j] + c][b] (+)= sum_{k < npar[j], i < bs} theta[parpids[j][k] + c bs + i]
- *                              * f[parids[j][k] + i][b] * exp(e[chids[j] + c][b] - m[parids[j][k] + i][b]) */
+ *                              * f[parids[j][k] + i][b] * exp(e[chids[j] + c][b] - m[parids[j][k] + i][b])
+ * (node flows unmodified, any sign). */
 void pcc_sum_ele(real* eflows, const real* nflows, const real* nmars, const real* emars, const real* params,
                  const int64_t* chids, const int64_t* parids, const int64_t* parpids, const int64_t* npar, int64_t n,
                  int64_t KP, int64_t cbs, int64_t bs, int64_t B, int accum) {
-#pragma omp parallel for schedule(static)
-    for (int64_t t = 0; t < n * cbs; ++t) {
-        const int64_t j = t / cbs, c = t % cbs;
-        const int64_t row = chids[j] + c;
-        const real* e = emars + row * B;
-        real* o = eflows + row * B;
-        if (!accum)
-            for (int64_t b = 0; b < B; ++b) o[b] = 0;
-        for (int64_t k = 0; k < npar[j]; ++k) {
-            for (int64_t i = 0; i < bs; ++i) {
-                const real th = params[parpids[j * KP + k] + c * bs + i];
-                const real* f = nflows + (parids[j * KP + k] + i) * B;
-                const real* m = nmars + (parids[j * KP + k] + i) * B;
-                for (int64_t b = 0; b < B; ++b)
-                    if (f[b] != 0 && e[b] != NEG_INF) o[b] += th * f[b] * exp(e[b] - m[b]);
+    const int64_t nc = (B + PCC_BC - 1) / PCC_BC;
+    const real lim = (real)PCC_EXP_MAX;
+#pragma omp parallel
+    {
+        real* R = (real*)malloc(sizeof(real) * bs * PCC_BC);
+        real s[PCC_BC], acc[PCC_BC];
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < n * nc; ++t) {
+            const int64_t j = t / nc, b0 = (t % nc) * PCC_BC;
+            const int64_t nb = (B - b0) < PCC_BC ? (B - b0) : PCC_BC;
+            if (!accum)
+                for (int64_t c = 0; c < cbs; ++c)
+                    for (int64_t b = 0; b < nb; ++b) eflows[(chids[j] + c) * B + b0 + b] = 0;
+            for (int64_t k = 0; k < npar[j]; ++k) {
+                const int64_t prow = parids[j * KP + k];
+                pcc_scaled_flows(R, s, nflows, nmars, prow, bs, b0, nb, B);
+                for (int64_t c = 0; c < cbs; ++c) {
+                    const int64_t row = chids[j] + c;
+                    const real* e = emars + row * B + b0;
+                    const real* th = params + parpids[j * KP + k] + c * bs;
+                    for (int64_t b = 0; b < nb; ++b) acc[b] = 0;
+                    for (int64_t i = 0; i < bs; ++i) {
+                        const real w = th[i];
+                        const real* r = R + i * nb;
+                        for (int64_t b = 0; b < nb; ++b) acc[b] += w * r[b];
+                    }
+                    real* o = eflows + row * B + b0;
+                    for (int64_t b = 0; b < nb; ++b) {
+                        if (e[b] == NEG_INF || s[b] == NEG_INF) continue;
+                        if (e[b] + s[b] <= lim) {
+                            o[b] += acc[b] * (real)exp(e[b] + s[b]);
+                        } else {   /* direct sum: the shifted exponent would overflow */
+                            real d = 0;
+                            for (int64_t i = 0; i < bs; ++i) {
+                                const real f = nflows[(prow + i) * B + b0 + b];
+                                const real m = nmars[(prow + i) * B + b0 + b];
+                                if (f != 0 && m != NEG_INF) d += th[i] * f * (real)exp(e[b] - m);
+                            }
+                            o[b] += d;
+                        }
+                    }
+                }
             }
         }
+        free(R);
     }
 }
 
@@ -180,20 +255,53 @@ void pcc_sum_ele(real* eflows, const real* nflows, const real* nmars, const real
 void pcc_sum_par(real* pflows, const real* nflows, const real* nmars, const real* emars, const real* params,
                  const int64_t* nids, const int64_t* cids, const int64_t* pids, const int64_t* pfids,
                  const int64_t* nch, int64_t n_blocks, int64_t K, int64_t bs, int64_t B) {
-#pragma omp parallel for schedule(static)
-    for (int64_t t = 0; t < n_blocks * bs; ++t) {
-        const int64_t j = t / bs, r = t % bs;
-        const real* f = nflows + (nids[j] + r) * B;
-        const real* m = nmars + (nids[j] + r) * B;
-        for (int64_t k = 0; k < nch[j]; ++k) {
-            const real* e = emars + cids[j * K + k] * B;
-            real s = 0;
-            for (int64_t b = 0; b < B; ++b)
-                if (f[b] != 0 && e[b] != NEG_INF) s += f[b] * exp(e[b] - m[b]);
-            const real v = params[pids[j * K + k] + r] * s;
+    const int64_t nc = (B + PCC_BC - 1) / PCC_BC;
+    const real lim = (real)PCC_EXP_MAX;
+#pragma omp parallel
+    {
+        real* R = (real*)malloc(sizeof(real) * bs * PCC_BC);
+        real* loc = (real*)malloc(sizeof(real) * bs);
+        real s[PCC_BC], C[PCC_BC];
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < n_blocks * nc; ++t) {
+            const int64_t j = t / nc, b0 = (t % nc) * PCC_BC;
+            const int64_t nb = (B - b0) < PCC_BC ? (B - b0) : PCC_BC;
+            pcc_scaled_flows(R, s, nflows, nmars, nids[j], bs, b0, nb, B);
+            for (int64_t k = 0; k < nch[j]; ++k) {
+                const real* e = emars + cids[j * K + k] * B + b0;
+                int direct = 0;
+                for (int64_t b = 0; b < nb; ++b) {
+                    if (e[b] == NEG_INF || s[b] == NEG_INF) { C[b] = 0; continue; }
+                    if (e[b] + s[b] > lim) { C[b] = 0; direct = 1; continue; }
+                    C[b] = (real)exp(e[b] + s[b]);
+                }
+                for (int64_t r = 0; r < bs; ++r) {
+                    const real* rr = R + r * nb;
+                    real a = 0;
+                    for (int64_t b = 0; b < nb; ++b) a += rr[b] * C[b];
+                    loc[r] = a;
+                }
+                if (direct) {   /* columns whose shifted exponent would overflow: direct sum */
+                    for (int64_t b = 0; b < nb; ++b) {
+                        if (e[b] == NEG_INF || s[b] == NEG_INF || e[b] + s[b] <= lim) continue;
+                        for (int64_t r = 0; r < bs; ++r) {
+                            const real f = nflows[(nids[j] + r) * B + b0 + b];
+                            const real m = nmars[(nids[j] + r) * B + b0 + b];
+                            if (f != 0 && m != NEG_INF) loc[r] += f * (real)exp(e[b] - m);
+                        }
+                    }
+                }
+                const real* th = params + pids[j * K + k];
+                real* pf = pflows + pfids[j * K + k];
+                for (int64_t r = 0; r < bs; ++r) {
+                    const real v = th[r] * loc[r];
 #pragma omp atomic
-            pflows[pfids[j * K + k] + r] += v;
+                    pf[r] += v;
+                }
+            }
         }
+        free(loc);
+        free(R);
     }
 }
 

@@ -2,8 +2,10 @@
 
 ``log_likelihood(pc, x)`` returns ``lls [B, num_roots]`` from the HIP forward; its backward is
 the hand-written adjoint: the flow backward (``TensorCircuit.backward``) with the incoming
-gradient as root flows, whose parameter flows are ``pf = theta * d(sum_b w_b LL_b) / d theta``
-because a circuit is multilinear in every parameter.  So
+gradient w as root flows.  Its parameter flows are ``pf = theta * d(sum_b w_b LL_b) / d theta``:
+every parameter occurrence (each tied copy counted separately) enters the circuit's value
+linearly, so theta * d(value)/d(theta) of one occurrence is the flow through that edge, and the
+flows of tied copies add up to the total derivative.  So
 
 * sum and Categorical parameters: ``dL/dtheta = pf / theta`` (tied copies summed first);
 * Bernoulli ``p``: ``dL/dp = f(x=1) / p - f(x=0) / (1 - p)``.
@@ -12,7 +14,9 @@ The gradient is taken with respect to the parameters as stored (no renormalisati
 quantity the EM update and the reference's ``param_flows`` are built from
 (``pyjuice/model/tensorcircuit.py:24-37, 344-538``).  The backward overwrites the circuit's
 parameter-flow accumulators and needs the node marginals of this forward (run it before the
-next forward).  A parameter equal to 0 gets gradient 0.
+next forward).  A parameter equal to 0 gets gradient 0.  The flow kernels take logarithms of
+flows, so a gradient with entries of both signs runs the flow backward twice (positive and
+negative parts) and subtracts.
 """
 from __future__ import annotations
 
@@ -100,11 +104,31 @@ class _CircuitLL(torch.autograd.Function):
     def backward(ctx, grad):
         pc = ctx.pc
         params, *input_params = ctx.saved_tensors
+        want_sum = ctx.needs_input_grad[3]
+        want_in = [ctx.needs_input_grad[4 + i] for i in range(len(input_params))]
+        g, g_in = None, [None] * len(input_params)
         with torch.no_grad():
-            pc.backward(ctx.inputs, ll_weights=grad.t().contiguous(), flows_memory=0.0, precision=ctx.precision)
-            g = _sum_grad(pc, params) if ctx.needs_input_grad[3] else None
-            g_in = [(_input_grad(layer, p) if ctx.needs_input_grad[4 + i] else None)
-                    for i, (layer, p) in enumerate(zip(pc.input_layer_group, input_params))]
+            # Flows are linear in the root weights but the flow kernels work on log(flow), so a
+            # signed gradient runs the adjoint once per sign: weights max(g, 0) and max(-g, 0),
+            # and the second result is subtracted.
+            w = grad.t().contiguous()
+            parts = [(1.0, w.clamp_min(0.0)), (-1.0, (-w).clamp_min(0.0))]
+            for sign, wp in parts:
+                if not bool((wp > 0).any()):
+                    continue
+                pc.backward(ctx.inputs, ll_weights=wp, flows_memory=0.0, precision=ctx.precision)
+                if want_sum:
+                    gs = _sum_grad(pc, params)
+                    g = gs * sign if g is None else g + sign * gs
+                for i, (layer, p) in enumerate(zip(pc.input_layer_group, input_params)):
+                    if want_in[i]:
+                        gi = _input_grad(layer, p)
+                        g_in[i] = gi * sign if g_in[i] is None else g_in[i] + sign * gi
+            if want_sum and g is None:
+                g = torch.zeros_like(params)
+            for i, p in enumerate(input_params):
+                if want_in[i] and g_in[i] is None:
+                    g_in[i] = torch.zeros_like(p)
         return (None, None, None, g, *g_in)
 
 

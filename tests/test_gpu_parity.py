@@ -6,6 +6,11 @@ Tolerances against the FP64 oracle (``TOL`` below), FP32 / reduced-precision MFM
 * node flows and parameter flows: max error <= 1e-4 / 2e-2 relative to the largest value;
 * EM parameters: |d| <= 2e-6 / 5e-4 absolute;
 * samples: bit-identical given the same uniforms.
+
+Batches: 64 and 13 on every circuit, 300 and 512 on the circuits that run the MFMA kernels.
+Edge cases: zero parameters (``zero_param_mask``), EM with ``keep_zero_params``, impossible
+observations (out-of-range categories, so whole LL columns are -inf), and two Categorical vectors
+with different category counts over one variable.
 """
 import numpy as np
 import pytest
@@ -76,8 +81,13 @@ def device():
     return torch.device("cuda:0")
 
 
-@pytest.mark.parametrize("name", CIRCUITS)
-@pytest.mark.parametrize("B", [64, 13])
+# The benchmark batches: 300 (a partial 64-column tile, several batch tiles and parameter-flow
+# reduction stages) and 512 (the headline batch) on the circuits that take the MFMA kernels.
+BIG = ["hclt_cat_bs64", "hclt_cat_bs128", "pd_small", "hmm_small", "hclt_pruned"]
+CASES = [(n, B) for n in CIRCUITS for B in (64, 13)] + [(n, B) for n in BIG for B in (300, 512)]
+
+
+@pytest.mark.parametrize("name,B", CASES)
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16", "fp32_valu"])
 def test_forward_backward_em(name, B, precision, device):
     U.say(f"[parity] {name} B={B} {precision}")
@@ -763,3 +773,121 @@ def test_tiny_batches(name, B, precision, device):
                 if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
                     ref = ref + pflows[m]
             assert U.rel_err(got, ref) <= t_flow, (n, U.rel_err(got, ref))
+
+
+def _edge_circuit(name):
+    """A circuit with a seeded quarter of its sum parameters masked to zero (``zero_param_mask``,
+    at least one child kept per node) and a seeded fifth of its Categorical parameters set to 0."""
+    ns, cards, kw = _circuit(name)
+    TensorCircuit(ns, **kw).update_parameters()     # every vector holds its parameters
+    g = torch.Generator().manual_seed(5)
+    for n in node_iterator(ns):
+        if n.is_tied():
+            continue
+        if n.is_sum() and n.num_ch_nodes > 1:
+            m = torch.rand(n._params.shape, generator=g) < 0.25
+            m[..., 0] &= False   # keep child 0 of each child block: no node loses every child
+            n.set_zero_param_mask(m)
+            n._params[m] = 0.0
+        elif n.is_input() and n.dist.get_signature() == "Categorical":
+            K = n.dist.num_cats
+            P = n._params.reshape(-1, K).clone()
+            z = torch.rand(P.shape, generator=g) < 0.2
+            z[:, 0] = False
+            P[z] = 0.0
+            n._params = (P / P.sum(dim=1, keepdim=True)).reshape(-1)
+    return ns, cards, kw
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs32", "hclt_cat_bs64", "pd_small", "hmm_small"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp32_valu"])
+@pytest.mark.parametrize("B", [64, 300])
+def test_zero_params_impossible_data_keep_zero_em(name, precision, B, device):
+    """Zero parameters and out-of-range categories (-inf LL columns: the reference's
+    ``emars_max == -inf`` guards, ``sum_layer.py:519-549``) through forward, flows and EM with
+    ``keep_zero_params`` (``par_update.py:196-205, 236-253``, ``categorical.py:171-201``)."""
+    U.say(f"[edge] {name} {precision} B={B}")
+    t_ll, t_mar, t_off, t_flow, t_em = TOL[precision]
+    ns, cards, kw = _edge_circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = U.random_data(cards, B, seed=13)
+    X[3, 0] = cards[0]          # out of range: -inf
+    X[B - 2, -1] = cards[-1] + 7
+    xt = torch.from_numpy(X).to(device)
+    lls = pc(xt).detach().cpu().numpy()[:, 0]
+    pc.update_parameters()
+    vals = dag_eval.forward(ns, X)
+    ref = vals[ns][0]
+    inf = np.isinf(ref)
+    assert inf[3] and inf[B - 2]
+    assert np.array_equal(np.isinf(lls), inf) and np.all(lls[inf] < 0)
+    assert np.abs(lls[~inf] - ref[~inf]).max() <= t_ll * len(cards)
+    pc.backward(xt, allow_modify_flows=False, flows_memory=0.0)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    nf = pc.node_flows.detach().cpu().numpy()
+    assert np.isfinite(nf).all()
+    for n in node_iterator(ns):
+        if n.is_sum() or n.is_input():
+            a, b = n._output_ind_range
+            assert U.rel_err(nf[a:b], flows[n]) <= t_flow, n
+    pc.update_param_flows(origin_ns_only=False)
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            ref_pf = pflows[n]
+            for m in node_iterator(ns):
+                if m is not n and m.is_tied() and m.get_source_ns() is n and m._param_flow_range == n._param_flow_range:
+                    ref_pf = ref_pf + pflows[m]
+            assert U.rel_err(got, ref_pf) <= t_flow, n
+    step, pcount = 0.5, 0.1
+    pc.mini_batch_em(step_size=step, pseudocount=pcount, keep_zero_params=True)
+    new = dag_eval.em_update(ns, pflows, step_size=step, pseudocount=pcount, keep_zero_params=True)
+    pc.update_parameters()
+    for s, p in new.items():
+        if s.is_sum():
+            got = U.dense_pf_from_blocks(s, s._params.numpy())
+        else:
+            got = s._params.numpy().astype(np.float64).reshape(p.shape)
+        assert np.array_equal(got == 0, p == 0) or s.is_input() and s.dist.get_signature() != "Categorical"
+        assert np.abs(got - p).max() <= t_em, (s, np.abs(got - p).max())
+
+
+@pytest.mark.parametrize("B", [64, 512, 1024, 2048])
+def test_mixed_category_counts_on_one_variable(B, device):
+    """Two Categorical vectors over the same variable with different category counts, laid out in
+    one input layer so that 8-node tiles of one 64-node backward group alternate between them:
+    each tile's sort must use its own count."""
+    from pyjuice_amd.nodes import inputs, multiply, summate
+    import pyjuice_amd as juice
+    U.say(f"[mixed K] B={B}")
+    torch.manual_seed(2)
+    with juice.set_block_size(8):
+        vecs = []
+        for v in range(3):
+            a = inputs(v, num_node_blocks=2, dist=juice.distributions.Categorical(num_cats=12))
+            b = inputs(v, num_node_blocks=2, dist=juice.distributions.Categorical(num_cats=5))
+            vecs.append(summate(a, b, num_node_blocks=2))
+        top = multiply(*vecs)
+    root = summate(top, num_node_blocks=1, block_size=1)
+    root.init_parameters()
+    pc = TensorCircuit(root)
+    pc.to(device)
+    layer = pc.input_layer_group[0]
+    assert layer._uniform_tiles(8)
+    k = layer._ncats.cpu().numpy()
+    v = layer.vids[:, 0].cpu().numpy()
+    tiles = [(int(v[i]), int(k[i])) for i in range(0, len(k), 8)]
+    assert any(t[0] == u[0] and t[1] != u[1] for t, u in zip(tiles, tiles[1:])), tiles
+    X = U.random_data([12] * 3, B, seed=4)     # categories >= 5 are impossible for the 5-category vectors
+    xt = torch.from_numpy(X).to(device)
+    pc(xt)
+    pc.backward(xt, allow_modify_flows=False, flows_memory=0.0)
+    pc.update_parameters()
+    vals = dag_eval.forward(root, X)
+    _, pflows = dag_eval.backward(root, X, vals)
+    pfi = layer.param_flows.detach().cpu().numpy().astype(np.float64)
+    for n in layer.nodes:
+        a, b = n._param_flow_range
+        got = pfi[a:b].reshape(n.num_nodes, n.dist.num_param_flows())
+        assert U.rel_err(got, pflows[n]) <= 1e-4, n

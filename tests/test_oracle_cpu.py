@@ -105,3 +105,42 @@ def test_fp32_build_tracks_fp64_build():
     a.backward()
     b.backward()
     assert U.rel_err(b.param_flows.astype(np.float64), a.param_flows) <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["simple_model", "hclt_bern16", "hmm_small"])
+def test_compiled_cpu_signed_root_flows_are_weighted_gradients(name):
+    """With signed root flows w, the C restatement's parameter flows are
+    ``theta * d(sum_b w_b LL_b) / d theta`` (FP64 central differences of the weighted sum of its
+    forward, which the test above pins to the DAG oracle): the oracle side of the signed-gradient
+    adjoint."""
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.update_parameters()
+    X = U.random_data(cards, 12, seed=8)
+    w = np.random.default_rng(4).normal(size=[1, 12])
+    cpu = CompiledCPU(pc)
+    cpu.forward(X)
+    cpu.backward(root_flows=w)
+    fused = cpu.param_flows.copy()
+    for t, k, ch in cpu.fuse:
+        fused[t:t + k] = sum(cpu.param_flows[c:c + k] for c in ch)
+    theta = pc.params.detach().cpu().numpy().astype(np.float64)
+    from pyjuice_amd.model.autograd import _sum_param_to_flow
+    idx = _sum_param_to_flow(pc).numpy()
+    g = np.random.default_rng(6)
+    cand = np.nonzero((idx >= 0) & (theta > 0))[0]
+    h = 1e-6
+
+    def wll(th):
+        pc.params.data = torch.from_numpy(th.astype(np.float32)).to(torch.float32)
+        c2 = CompiledCPU(pc)
+        c2.params = th.copy()
+        return float((c2.forward(X)[0] * w[0]).sum())
+
+    for k in g.choice(cand, size=min(8, cand.size), replace=False):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd = (wll(tp) - wll(tm)) / (2 * h)
+        assert abs(fd - fused[idx[k]]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, fused[idx[k]])
+    pc.params.data = torch.from_numpy(theta.astype(np.float32))
