@@ -5,7 +5,8 @@
  * is a plain C function over device pointers and sizes; it enqueues its kernels on `stream`
  * and returns without synchronising.  Return value: 0 on success, a positive hipError_t when
  * a launch failed, PJAMD_EINVAL (-1) for invalid sizes / argument combinations and
- * PJAMD_EUNSUPPORTED (-2) for a configuration the library does not implement
+ * PJAMD_EUNSUPPORTED (-2) for a configuration the library does not implement and
+ * PJAMD_ETOOLARGE (-3) for a request beyond an allocation-free entry point's fixed workspace
  * (pjamd_error_string() names each code).
  *
  * Memory layout shared by all entry points (device memory, row-major):
@@ -40,7 +41,8 @@ typedef struct ihipStream_t* pjamd_stream_t;
 enum {
     PJAMD_STATUS_OK = 0,
     PJAMD_STATUS_EINVAL = -1,
-    PJAMD_STATUS_EUNSUPPORTED = -2
+    PJAMD_STATUS_EUNSUPPORTED = -2,
+    PJAMD_STATUS_ETOOLARGE = -3      /* beyond the fixed workspace of an allocation-free entry point */
 };
 
 /* Sum-layer arithmetic (`precision` arguments). */
@@ -205,6 +207,24 @@ int pjamd_sum_forward_split(float* node_mars, const float* element_mars, const f
                             const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs,
                             int64_t B, int child_run, int n_splits, float* workspace, int precision,
                             pjamd_stream_t stream);
+
+/* Size in floats of the fixed device-resident workspace of pjamd_sum_forward_split_noalloc
+ * (32 MiB per device). */
+#define PJAMD_SPLIT_SCRATCH_FLOATS (8 << 20)
+
+/* pjamd_sum_forward_split without a workspace argument, allocating nothing: no counterpart in
+ * PyJuice.  For callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The partials live in a fixed array of
+ * PJAMD_SPLIT_SCRATCH_FLOATS floats that each device holds until the process ends; a call needing
+ * n_splits * (n_blocks * bs + ceil(n_blocks * bs / 64)) * B floats or more than that returns
+ * PJAMD_ETOOLARGE before any launch.  The array is looked up on the current device (stream must
+ * belong to it) on the first call there.  Every call on a device shares that device's array, so
+ * calls that overlap on one device (other streams, other threads) race.  Arguments and results
+ * otherwise as pjamd_sum_forward_split, with the same kernels. */
+int pjamd_sum_forward_split_noalloc(float* node_mars, const float* element_mars, const float* params,
+                                    const int* nids, const int* cids, const int* pids, const int* n_chs,
+                                    int n_blocks, int K, int bs, int64_t B, int child_run, int n_splits,
+                                    int precision, pjamd_stream_t stream);
 
 /* pjamd_sum_forward with the product layer below it fused in (BF16 / FP16 operands: precision 1
  * or 2).  The partition's element rows are not read from element_mars but summed from the node

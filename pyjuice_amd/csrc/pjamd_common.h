@@ -14,7 +14,8 @@
 enum {
     PJAMD_OK = 0,
     PJAMD_EINVAL = -1,      // invalid size / unsupported combination of arguments
-    PJAMD_EUNSUPPORTED = -2 // feature not implemented by this library
+    PJAMD_EUNSUPPORTED = -2, // feature not implemented by this library
+    PJAMD_ETOOLARGE = -3     // request exceeds the fixed device workspace of an allocation-free entry point
 };
 
 #define PJAMD_LAUNCH_CHECK()                      \

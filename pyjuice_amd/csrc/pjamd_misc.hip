@@ -194,6 +194,9 @@ PJAMD_EXPORT const char* pjamd_error_string(int code) {
     if (code == PJAMD_OK) return "success";
     if (code == PJAMD_EINVAL) return "invalid argument (size, block size or unsupported combination)";
     if (code == PJAMD_EUNSUPPORTED) return "unsupported configuration";
+    if (code == PJAMD_ETOOLARGE)
+        return "request exceeds the fixed device workspace of the allocation-free entry point "
+               "(PJAMD_SPLIT_SCRATCH_FLOATS floats)";
     if (code > 0) return hipGetErrorString((hipError_t)code);
     return "unknown error";
 }

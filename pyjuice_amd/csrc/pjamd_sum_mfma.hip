@@ -17,6 +17,8 @@
 // block (which share parameters and child rows) meet in the same L2.
 #include "pjamd_mfma.h"
 
+#include <mutex>
+
 using namespace pjamd;
 
 namespace {
@@ -2158,4 +2160,53 @@ PJAMD_EXPORT int pjamd_sum_backward_par_batched(void* param_flows, const float* 
     return launch_par_batched<__bf16, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
                                              cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod,
                                              overwrite, sign, stream);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free split forward.  Its workspace is `scratch`, a fixed device-resident array of
+// PJAMD_SPLIT_SCRATCH_FLOATS floats (32 MiB) that every device the library runs on holds until the
+// process ends.  In host code `scratch` names only a host-side stand-in: its device address is
+// looked up with hipGetSymbolAddress once per device (with that device current) and kept in
+// g_scratch_ptr; only that pointer is ever handed to a kernel.  The split kernel writes every
+// partial and column maximum the combine reads, so nothing needs clearing between calls; calls
+// that overlap on one device share the array and race.
+static __device__ __align__(256) float scratch[PJAMD_SPLIT_SCRATCH_FLOATS];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device.
+int current_scratch(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return PJAMD_EINVAL;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PJAMD_OK;
+}
+}  // namespace
+
+PJAMD_EXPORT int pjamd_sum_forward_split_noalloc(float* node_mars, const float* element_mars, const float* params,
+                                                 const int* nids, const int* cids, const int* pids, const int* n_chs,
+                                                 int n_blocks, int K, int bs, int64_t B, int child_run, int n_splits,
+                                                 int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && n_splits >= 2 && n_splits <= 16);
+    if (n_blocks == 0) return PJAMD_OK;
+    const int64_t rows = (int64_t)n_blocks * bs;
+    const int64_t need = (int64_t)n_splits * (rows + (rows + 63) / 64) * B;
+    if (need > (int64_t)PJAMD_SPLIT_SCRATCH_FLOATS) return PJAMD_ETOOLARGE;
+    float* ws = nullptr;
+    const int st = current_scratch(&ws);
+    if (st != PJAMD_OK) return st;
+    return pjamd_sum_forward_split(node_mars, element_mars, params, nids, cids, pids, n_chs, n_blocks, K, bs, B,
+                                   child_run, n_splits, ws, precision, stream);
 }
