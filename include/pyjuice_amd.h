@@ -68,6 +68,12 @@ int pjamd_abi_version(void);
 /* Static text for a status code (no PyJuice counterpart). */
 const char* pjamd_error_string(int code);
 
+/* p[i] = log(p[i]) for i < n (-inf where p[i] == 0): hands the node-flow buffer back in log
+ * space after a linear-space flow backward.  Replaces the log-space storage of node flows under
+ * TensorCircuit.backward(logspace_flows=True) (src/pyjuice/model/tensorcircuit.py:393-417,
+ * layer/sum_layer.py:1376-1392). */
+int pjamd_flows_to_log(float* p, int64_t n, pjamd_stream_t stream);
+
 /* p[0 .. n) = v.  Replaces the buffer initialisation of TensorCircuit._init_buffer
  * (src/pyjuice/model/tensorcircuit.py:858-878) and the flow / dummy-row fills of
  * TensorCircuit.forward / backward (tensorcircuit.py:207-342, 344-538).  A kernel, not
@@ -331,8 +337,10 @@ int pjamd_sum_backward_ele_alg(float* element_flows, const float* node_flows, co
  * param_flows[pfids[j][k] + i] += sum_b node_flows[n][b] where |lt - node_mars[n][b]| < 1e-6
  * (MPE, not scaled by theta), or theta * sum_b node_flows[n][b] exp(e[b] - node_mars[n][b])
  * (GeneralLL: the LL form, as the reference's kernel computes it); n = nids[j] + i.  pf64 selects
- * a double accumulator.  Replaces the propagation_alg_id branches of
- * SumLayer._bk_triton_block_sparse_par_kernel (sum_layer.py:2049-2169). */
+ * a double accumulator; alg | PJAMD_ALG_NEGATE subtracts instead (negate_pflows).  Replaces the
+ * propagation_alg_id and negate_pflows branches of SumLayer._bk_triton_block_sparse_par_kernel
+ * (sum_layer.py:2049-2169). */
+#define PJAMD_ALG_NEGATE 16
 int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const float* node_flows, const float* node_mars,
                                const float* element_mars, const float* params, const int* nids, const int* cids,
                                const int* pids, const int* pfids, const int* n_chs, int n_blocks, int K, int bs,

@@ -116,9 +116,16 @@ def _alg_sum(W: np.ndarray, child: np.ndarray, alg: str, alpha: float) -> np.nda
     return out
 
 
-def forward(root, data, alg: str = "LL", alpha: float = 1.0) -> Dict:
+def _alg_of(alg, ns) -> str:
+    """The algorithm of sum vector ``ns``: ``alg`` itself, or its entry when ``alg`` maps vectors to
+    algorithms (a per-layer choice; vectors not listed take "LL")."""
+    return alg.get(ns, "LL") if isinstance(alg, dict) else alg
+
+
+def forward(root, data, alg="LL", alpha: float = 1.0) -> Dict:
     """Log-values of every vector: ``{ns: [num_nodes, B]}``; ``data`` is ``[B, num_vars]``.
-    ``alg``: "LL", "MPE" or "GeneralLL" (with ``alpha``) for the sum vectors."""
+    ``alg``: "LL", "MPE" or "GeneralLL" (with ``alpha``) for the sum vectors, or a dict from sum
+    vector to one of those."""
     data = _np(data)
     vals = dict()
     for ns in node_iterator(root):
@@ -131,14 +138,15 @@ def forward(root, data, alg: str = "LL", alpha: float = 1.0) -> Dict:
             vals[ns] = acc
         else:
             child = np.concatenate([vals[c] for c in ns.chs], axis=0)
-            if alg == "LL":
+            a = _alg_of(alg, ns)
+            if a == "LL":
                 vals[ns] = _lse_matmul(sum_dense_params(ns), child)
             else:
-                vals[ns] = _alg_sum(sum_dense_params(ns), child, alg, alpha)
+                vals[ns] = _alg_sum(sum_dense_params(ns), child, a, alpha)
     return vals
 
 
-def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None, alg: str = "LL",
+def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None, alg="LL",
              alpha: float = 1.0):
     """Flows of every vector and parameter flows.
 
@@ -154,13 +162,14 @@ def backward(root, data, vals: Dict, root_flows: Optional[np.ndarray] = None, al
         f = flows.get(ns)
         if f is None:
             f = np.zeros([ns.num_nodes, B])
-        if ns.is_sum() and alg != "LL":
+        if ns.is_sum() and _alg_of(alg, ns) != "LL":
+            a = _alg_of(alg, ns)
             W = sum_dense_params(ns)
             child = np.concatenate([vals[c] for c in ns.chs], axis=0)
             v = vals[ns]
             with np.errstate(divide="ignore", invalid="ignore"):
                 lt = np.log(W)[:, :, None] + child[None, :, :]       # [n, C, B]
-                if alg == "MPE":
+                if a == "MPE":
                     share = (np.abs(lt - v[:, None, :]) < 1e-6).astype(np.float64)
                     pflows[ns] = (share * f[:, None, :]).sum(axis=2)
                 else:

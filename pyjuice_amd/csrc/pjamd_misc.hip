@@ -10,6 +10,12 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, float 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// p[i] = log(p[i]) (-inf for 0), grid-stride.
+__global__ __launch_bounds__(256) void log_kernel(float* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = logf(p[i]);
+}
+
 // [B, V] integer data of 1, 4 or 8 bytes -> [V, B] int32 through a 32 x 32 LDS tile.
 template <typename T>
 __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const T* __restrict__ src, int64_t B, int64_t V) {
@@ -199,6 +205,16 @@ PJAMD_EXPORT const char* pjamd_error_string(int code) {
                "(PJAMD_SPLIT_SCRATCH_FLOATS floats)";
     if (code > 0) return hipGetErrorString((hipError_t)code);
     return "unknown error";
+}
+
+PJAMD_EXPORT int pjamd_flows_to_log(float* p, int64_t n, hipStream_t stream) {
+    PJAMD_REQUIRE(n >= 0);
+    if (n == 0) return PJAMD_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    log_kernel<<<(unsigned)blocks, 256, 0, stream>>>(p, n);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
 }
 
 PJAMD_EXPORT int pjamd_fill_f32(float* p, float v, int64_t n, hipStream_t stream) {

@@ -592,7 +592,7 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_alg_kernel(PF* __restric
                                                                    const int* __restrict__ pids,
                                                                    const int* __restrict__ pfids,
                                                                    const int* __restrict__ nch, int K, int bs,
-                                                                   int64_t B, int alg) {
+                                                                   int64_t B, int alg, float sign) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = blockIdx.y * (kThreads / 64) + w;
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kThreads) void sum_bwd_par_alg_kernel(PF* __restric
     }
     s = warp_sum(s);
     if (lane == 0) {
-        const float v = alg == 1 ? s : th * s;
+        const float v = sign * (alg == 1 ? s : th * s);
         if (v != 0.0f) atomicAdd(&pflows[(int64_t)pfids[(int64_t)j * K + k] + i], (PF)v);
     }
 }
@@ -799,6 +799,8 @@ PJAMD_EXPORT int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const f
                                             const int* nids, const int* cids, const int* pids, const int* pfids,
                                             const int* nch, int n_blocks, int K, int bs, int64_t B, int alg,
                                             hipStream_t stream) {
+    const float sign = (alg & PJAMD_ALG_NEGATE) ? -1.0f : 1.0f;
+    alg &= ~PJAMD_ALG_NEGATE;
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && alg >= 0 && alg <= 2 &&
                   (int64_t)n_blocks * bs < ((int64_t)1 << 31));
     PJAMD_REQUIRE(cdiv(K, kThreads / 64) < 65536u);
@@ -807,11 +809,11 @@ PJAMD_EXPORT int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const f
     if (pf64)
         sum_bwd_par_alg_kernel<double><<<grid, kThreads, 0, stream>>>((double*)param_flows, node_flows, node_mars,
                                                                       element_mars, params, nids, cids, pids, pfids,
-                                                                      nch, K, bs, B, alg);
+                                                                      nch, K, bs, B, alg, sign);
     else
         sum_bwd_par_alg_kernel<float><<<grid, kThreads, 0, stream>>>((float*)param_flows, node_flows, node_mars,
                                                                      element_mars, params, nids, cids, pids, pfids,
-                                                                     nch, K, bs, B, alg);
+                                                                     nch, K, bs, B, alg, sign);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }

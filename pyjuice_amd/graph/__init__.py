@@ -29,11 +29,11 @@ class PartitionNode(RegionGraph):
     """Region of a product: children scopes must be disjoint."""
 
     def __init__(self, children):
-        assert len(children) > 0, "PartitionNode receives no child."
+        assert len(children) > 0, "a PartitionNode needs at least one child region"
         scope = BitSet()
         for c in children:
             if not RegionGraph.ALLOW_NONDECOMPOSABLE:
-                assert not scope.contains_any(c.scope), "Children of a PartitionNode have overlapping scopes."
+                assert not scope.contains_any(c.scope), "the child regions of a PartitionNode must have disjoint scopes"
             scope = scope | c.scope
         super().__init__(scope, children)
 
@@ -42,11 +42,11 @@ class InnerRegionNode(RegionGraph):
     """Region of a sum: children scopes must be identical."""
 
     def __init__(self, children):
-        assert len(children) > 0, "InnerRegionNode receives no child."
+        assert len(children) > 0, "an InnerRegionNode needs at least one child partition"
         scope = BitSet(children[0].scope._mask)
         for c in children[1:]:
             if not RegionGraph.ALLOW_NONSMOOTH:
-                assert scope == c.scope, "Children of an InnerRegionNode must have the same scope."
+                assert scope == c.scope, "the child partitions of an InnerRegionNode must share one scope"
             else:
                 scope = scope | c.scope
         super().__init__(scope, children)

@@ -148,7 +148,7 @@ def save(fname: str, model: Union[CircuitNodes, "TensorCircuit"]):
         model.update_parameters()
         model = model.root_ns
     if not fname.endswith(".jpc"):
-        raise ValueError(f"Unknown file type `.{fname.split('.')[-1]}`.")
+        raise ValueError(f"circuit files end in .jpc (got {fname!r})")
     with open(fname, "wb") as f:
         pickle.dump(serialize_nodes(model), f, protocol=4)
 
@@ -156,7 +156,7 @@ def save(fname: str, model: Union[CircuitNodes, "TensorCircuit"]):
 def load(fname: str) -> CircuitNodes:
     """Reads a ``.jpc`` file (``io.py:36-53``) with the allow-list unpickler."""
     if not fname.endswith(".jpc"):
-        raise ValueError(f"Unknown file type `.{fname.split('.')[-1]}`.")
+        raise ValueError(f"circuit files end in .jpc (got {fname!r})")
     with open(fname, "rb") as f:
         records = _AllowListUnpickler(f).load()
     return deserialize_nodes(records)

@@ -26,7 +26,7 @@ def _apply_missing_mask(data_vb, missing_mask, batch_first=True):
 class InputLayer(Layer, nn.Module):
     def __init__(self, nodes: Sequence, cum_nodes: int = 0, pc_num_vars: int = 0,
                  max_tied_ns_per_parflow_block: int = 4):
-        assert len(nodes) == len(set(nodes)), "Input node list contains duplicates."
+        assert len(nodes) == len(set(nodes)), "a vector appears twice in one layer"
         nodes = self._reorder_nodes(nodes)
         nn.Module.__init__(self)
         Layer.__init__(self, nodes, disable_block_size_check=True)
@@ -44,7 +44,7 @@ class InputLayer(Layer, nn.Module):
         cum_params = cum_pflows = cum_source = 0
         tied = dict()
         for ns in self.nodes:
-            assert ns.dist.get_signature() == sig, "Nodes of an InputLayer must have the same distribution type."
+            assert ns.dist.get_signature() == sig, "an input layer holds one distribution type"
             ns._output_ind_range = (cum_nodes, cum_nodes + ns.num_nodes)
             cum_nodes += ns.num_nodes
             layer_num_nodes += ns.num_nodes
@@ -171,9 +171,9 @@ class InputLayer(Layer, nn.Module):
                     added.add(ns)
                     added.add(src)
                 else:
-                    raise ValueError("A tied `InputNodes` should be in the same layer with its source nodes.")
+                    raise ValueError("a tied input vector must share a layer with its source")
         out = untied + tied_l
-        assert len(out) == len(nodes), "Total node length should not change after reordering."
+        assert len(out) == len(nodes), "reordering the input vectors changed the node count"
         return out
 
     def _get_num_nodes_gcd(self):
@@ -217,7 +217,7 @@ class InputLayer(Layer, nn.Module):
             ns._params = self.params.data[a:b].detach().cpu().clone()
 
     def __repr__(self):
-        return f"InputLayer(nid_range=({self._output_ind_range[0]}, {self._output_ind_range[1]}), num_vars={len(self.layer_scope)})"
+        return f"InputLayer[{self._output_ind_range[0]}:{self._output_ind_range[1]}, {len(self.layer_scope)} vars]"
 
     # ------------------------------------------------------------------ device state
 

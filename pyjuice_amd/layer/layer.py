@@ -24,7 +24,7 @@ class Layer:
         else:
             for i in range(1, len(self.nodes)):
                 assert self.nodes[i].block_size == self.nodes[0].block_size, \
-                    "`block_size` of nodes in the same layer must be identical."
+                    "all vectors of one layer must share a block size"
             self.block_size = self.nodes[0].block_size
         self.device = torch.device("cpu")
 

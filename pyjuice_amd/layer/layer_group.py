@@ -11,7 +11,7 @@ __all__ = ["LayerGroup"]
 class LayerGroup(nn.Module):
     def __init__(self, layers: Sequence):
         super().__init__()
-        assert len(layers) >= 1, "A `LayerGroup` must contains at least 1 layer."
+        assert len(layers) >= 1, "a layer group needs at least one layer"
         for i in range(1, len(layers)):
             assert type(layers[i]) == type(layers[0])
         if layers[0].is_input():

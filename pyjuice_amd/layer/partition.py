@@ -135,7 +135,7 @@ def partition_nodes_by_n_edges(node_n_edges, max_num_partitions: Optional[int] =
     elif max_num_partitions is None:
         max_num_partitions = 1
     else:
-        assert max_num_partitions >= 1, "Should provide at least 1 partition."
+        assert max_num_partitions >= 1, "the partitioner needs at least one partition"
 
     target = None if sparsity_tolerance is None else int(math.ceil(x.sum() * (1.0 + sparsity_tolerance)))
     if max_num_partitions == 1:
@@ -151,5 +151,5 @@ def partition_nodes_by_n_edges(node_n_edges, max_num_partitions: Optional[int] =
         vals, counts = _coalesce(x)
         sizes, _ = _weighted_dp(vals, counts, max_num_partitions, tgt)
     else:
-        raise ValueError(f"Unknown algorithm {algorithm} for `partition_nodes_by_n_edges`.")
+        raise ValueError(f"partitioning algorithm {algorithm!r} is not implemented")
     return torch.sort(torch.from_numpy(np.asarray(sizes, dtype=np.int64))).values

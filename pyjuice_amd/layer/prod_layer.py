@@ -28,7 +28,7 @@ class ProdLayer(Layer, nn.Module):
         Layer.__init__(self, nodes)
         nn.Module.__init__(self)
         assert len(self.nodes) > 0, "No input node."
-        assert len(self.nodes) == len(set(self.nodes)), "Input node list contains duplicates."
+        assert len(self.nodes) == len(set(self.nodes)), "a vector appears twice in one layer"
 
         # prod_layer.py:39-46: block-sparse only if every vector is block-sparse with one block size.
         bse = all((not ns.is_sparse()) and ns.block_size == self.nodes[0].block_size for ns in self.nodes)
@@ -72,7 +72,7 @@ class ProdLayer(Layer, nn.Module):
         return True
 
     def __repr__(self):
-        return f"ProdLayer(nid_range=({self._layer_nid_range[0]}, {self._layer_nid_range[1]}), num_nodes={self.num_nodes}, num_edges={self.num_edges})"
+        return f"ProdLayer[{self._layer_nid_range[0]}:{self._layer_nid_range[1]}, {self.num_nodes} nodes, {self.num_edges} edges]"
 
     def to(self, device):
         nn.Module.to(self, device)

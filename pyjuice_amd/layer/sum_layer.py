@@ -27,6 +27,7 @@ def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
     return torch.unique(torch.from_numpy(next_power_of_2(x)))
 
 
+ALG_NEGATE = 16   # PJAMD_ALG_NEGATE: parameter flows of MPE / GeneralLL subtracted
 PARAMS_ALIGNED = 1 << 30   # PJAMD_PARAMS_ALIGNED
 
 
@@ -96,7 +97,7 @@ class SumLayer(Layer, nn.Module):
         Layer.__init__(self, nodes)
         nn.Module.__init__(self)
         assert len(self.nodes) > 0, "No input node."
-        assert len(self.nodes) == len(set(self.nodes)), "Input node list contains duplicates."
+        assert len(self.nodes) == len(set(self.nodes)), "a vector appears twice in one layer"
 
         nid0, pid0, pfid0 = global_nid_start, global_pid_start, global_pfid_start
         nblk, nedges, n_chs = get_sum_layer_forward_stats(self.nodes, global_nid_start)
@@ -154,7 +155,7 @@ class SumLayer(Layer, nn.Module):
         return True
 
     def __repr__(self):
-        return f"SumLayer(nid_range=({self._layer_nid_range[0]}, {self._layer_nid_range[1]}), num_nodes={self.num_nodes}, num_edges={self.num_edges})"
+        return f"SumLayer[{self._layer_nid_range[0]}:{self._layer_nid_range[1]}, {self.num_nodes} nodes, {self.num_edges} edges]"
 
     def _child_run(self, p: int) -> int:
         """Largest power of two R (<= 1024) such that every R-aligned group of real entries of
@@ -342,7 +343,7 @@ class SumLayer(Layer, nn.Module):
             return 2, float(alpha)
         if propagation_alg == "LL":
             return 0, 1.0
-        raise ValueError(f"Unknown propagation algorithm {propagation_alg}.")
+        raise ValueError(f"propagation_alg {propagation_alg!r} is not one of LL, MPE, GeneralLL")
 
     def _forward_alg(self, node_mars, element_mars, params, propagation_alg, alpha):
         alg, alpha = self._alg_id(propagation_alg, alpha)
@@ -354,9 +355,9 @@ class SumLayer(Layer, nn.Module):
                    algo=(4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B))
 
     def _backward_alg(self, node_flows, element_flows, node_mars, element_mars, params, param_flows,
-                      propagation_alg, alpha, accumulate):
+                      propagation_alg, alpha, accumulate, negate=False):
         """Element flows into ``element_flows`` (then the product layer's backward) and parameter
-        flows for MPE / GeneralLL; node flows are read as stored."""
+        flows (subtracted with ``negate``) for MPE / GeneralLL; node flows are read as stored."""
         alg, alpha = self._alg_id(propagation_alg, alpha)
         B = node_flows.size(1)
         bs = self.block_size
@@ -371,7 +372,7 @@ class SumLayer(Layer, nn.Module):
             for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
                 N.call("pjamd_sum_backward_par_alg", N.ptr(param_flows), pf64, N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),
-                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, alg, s,
+                       N.ptr(nch), nids.numel(), cids.size(1), bs, B, alg | (ALG_NEGATE if negate else 0), s,
                        algo=(4 * (2 * nids.numel() * bs + tot) * B + 4 * tot * bs, 0))
 
     def backward(self, node_flows: torch.Tensor, element_flows: torch.Tensor, node_mars: torch.Tensor,
@@ -390,12 +391,13 @@ class SumLayer(Layer, nn.Module):
         backward partition, see ``pjamd_sum_backward_ele_to_children``): write the flows of the
         product layer's children directly into ``node_flows`` instead of element flows.
         """
+        # logspace_flows is handled by TensorCircuit.backward (flows propagate in linear space and
+        # the node-flow buffer is converted at the end), so a layer only sees linear flows.
+        assert not logspace_flows and not allow_neg_flows, \
+            "the layer kernels take linear, non-negative flows (TensorCircuit.backward converts log-space flows)"
         if propagation_alg != "LL":
-            assert not logspace_flows and not allow_neg_flows and not negate_pflows, \
-                "unsupported flow mode for the HIP sum layers"
             return self._backward_alg(node_flows, element_flows, node_mars, element_mars, params, param_flows,
-                                      propagation_alg, kwargs.get("alpha"), accumulate_ch_flows)
-        assert not logspace_flows and not allow_neg_flows, "unsupported flow mode for the HIP sum layers"
+                                      propagation_alg, kwargs.get("alpha"), accumulate_ch_flows, negate_pflows)
         assert not (accumulate_ch_flows and allow_modify_flows)
         prec = PRECISIONS[precision] if precision is not None else 0
         if force_use_fp32:

@@ -24,7 +24,7 @@ def _untied_sums(root_ns):
 
 
 def compile_par_update_fn(root_ns, BLOCK_SIZE: int = 32):
-    assert BLOCK_SIZE & (BLOCK_SIZE - 1) == 0, "`BLOCK_SIZE` must be a power of 2."
+    assert BLOCK_SIZE & (BLOCK_SIZE - 1) == 0, "the EM row width is not a power of two"
     cols = {k: [] for k in ("par", "pf", "bsz", "itv", "gnid", "nch")}
     blocks = []
     gnid = 0

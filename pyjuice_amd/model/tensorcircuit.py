@@ -91,7 +91,7 @@ class TensorCircuit(nn.Module):
                  device: Optional[Union[int, torch.device]] = None, verbose: bool = True,
                  precision: str = "fp32", pflow_dtype: torch.dtype = torch.float64):
         super().__init__()
-        assert isinstance(root_ns, CircuitNodes), "`root_ns` should be an instance of `CircuitNodes`."
+        assert isinstance(root_ns, CircuitNodes), "compile() takes the root CircuitNodes of a circuit"
         assert precision in ("fp32", "bf16", "fp16", "fp32_valu")
         self.root_ns = root_ns
         self.device = torch.device("cpu")
@@ -547,10 +547,10 @@ class TensorCircuit(nn.Module):
                 units.append(g[lo:hi])
         return units
 
-    def _run_par_batched(self, modify: bool, precision: str, overwrite: bool):
+    def _run_par_batched(self, modify: bool, precision: str, overwrite: bool, negate: bool = False):
         pf = self._pf
         pf64 = pf.dtype == torch.float64
-        flags = (1 if modify else 0) | (8 if pf64 else 0) | (16 if overwrite else 0)
+        flags = (1 if modify else 0) | (4 if negate else 0) | (8 if pf64 else 0) | (16 if overwrite else 0)
         prec = PRECISIONS[precision]
         B = self.node_flows.size(1)
         for tm, t in self._par_batches["tables"].items():
@@ -616,12 +616,12 @@ class TensorCircuit(nn.Module):
             self.default_propagation_alg = propagation_alg
             self.propagation_alg_kwargs.clear()
         elif propagation_alg == "GeneralLL":
-            assert "alpha" in kwargs, "Argument `alpha` should be provided for the `GeneralLL` propagation algorithm."
+            assert "alpha" in kwargs, "GeneralLL propagation needs an `alpha` keyword"
             self.default_propagation_alg = "GeneralLL"
             self.propagation_alg_kwargs.clear()
             self.propagation_alg_kwargs["alpha"] = kwargs["alpha"]
         else:
-            raise NotImplementedError(f"Unknown propagation algorithm {propagation_alg}.")
+            raise NotImplementedError(f"propagation_alg {propagation_alg!r} is not one of LL, MPE, GeneralLL")
 
     def forward_ll(self, *args, **kwargs):
         return self.forward(*args, propagation_alg="LL", **kwargs)
@@ -634,20 +634,46 @@ class TensorCircuit(nn.Module):
         ``alpha`` before calling forward; here it is passed on)."""
         return self.forward(*args, propagation_alg="GeneralLL", alpha=alpha, **kwargs)
 
-    def _run_inner_forward_alg(self, propagation_alg: str, alpha):
-        """Layer-wise forward for MPE / GeneralLL: product layers as for LL, sum layers through
-        ``pjamd_sum_forward_alg`` (no chains, fusion, split reductions or graph capture)."""
+    def _group_algs(self, propagation_alg) -> list:
+        """Propagation algorithm of every inner layer group: one name for all, or the reference's
+        per-group list indexed like ``inner_layer_groups`` (``tensorcircuit.py:270-280, 460-470``;
+        entries of product groups are ignored)."""
+        if isinstance(propagation_alg, str):
+            return [propagation_alg] * len(self.inner_layer_groups)
+        algs = list(propagation_alg)
+        assert len(algs) == len(self.inner_layer_groups), \
+            f"a per-layer propagation_alg list needs {len(self.inner_layer_groups)} entries, got {len(algs)}"
+        for a in algs:
+            assert a in ("LL", "MPE", "GeneralLL"), f"propagation_alg {a!r} is not one of LL, MPE, GeneralLL"
+        return algs
+
+    def _run_inner_forward_alg(self, propagation_alg, alpha, precision="fp32"):
+        """Layer-wise forward for MPE / GeneralLL or a per-group list of algorithms: product
+        layers as for LL, sum layers through ``pjamd_sum_forward_alg`` (their "LL" groups through
+        the LL kernels in ``precision``; no chains, fusion or graph capture)."""
+        algs = self._group_algs(propagation_alg)
         for gid, g in enumerate(self.inner_layer_groups):
             if g.is_prod():
                 g(self.node_mars, self._slab(gid))
+            elif algs[gid] == "LL":
+                g(self.node_mars, self._slab(gid), self.params, precision=precision)
             else:
                 for layer in g:
-                    layer(self.node_mars, self._slab(gid), self.params, propagation_alg=propagation_alg, alpha=alpha)
+                    layer(self.node_mars, self._slab(gid), self.params, propagation_alg=algs[gid], alpha=alpha)
 
-    def _run_inner_backward_alg(self, compute_param_flows: bool, propagation_alg: str, alpha):
-        """Layer-wise flow backward for MPE / GeneralLL: each sum layer writes its element flows
-        (``pjamd_sum_backward_ele_alg``) and parameter flows (``pjamd_sum_backward_par_alg``), the
-        product layer below spreads the element flows to its children."""
+    def _run_inner_backward_layerwise(self, compute_param_flows: bool, propagation_alg, alpha, precision: str,
+                                      allow_modify_flows: bool, negate_pflows: bool, pre_callback=None,
+                                      post_callback=None):
+        """Layer-wise flow backward (``tensorcircuit.py:437-482``): MPE / GeneralLL sum groups write
+        element flows (``pjamd_sum_backward_ele_alg``) and parameter flows
+        (``pjamd_sum_backward_par_alg``), "LL" groups run the LL kernels without the fused product
+        backward (their flows modified in place when ``allow_modify_flows``), and each product
+        group spreads the element flows to its children.  ``pre_callback`` / ``post_callback`` are
+        called as ``fn(layer, node_flows=..., element_flows=..., node_mars=..., element_mars=...,
+        params=..., param_flows=...)`` around each sum layer's backward, like the reference's
+        ``LayerGroup.callback``."""
+        algs = self._group_algs(propagation_alg)
+        pf = self._pf if compute_param_flows else None
         for gid in range(len(self.inner_layer_groups) - 1, -1, -1):
             g = self.inner_layer_groups[gid]
             if g.is_prod():
@@ -655,10 +681,22 @@ class TensorCircuit(nn.Module):
                 continue
             if self.recompute_elements:
                 self.inner_layer_groups[gid - 1](self.node_mars, self._slab(gid - 1))
+            slab = self._slab(gid)
+            cb = dict(node_flows=self.node_flows, element_flows=self.element_flows, node_mars=self.node_mars,
+                      element_mars=slab, params=self.params, param_flows=self._pf)
+            if pre_callback is not None:
+                g.callback(pre_callback, **cb)
             for layer in g:
-                layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(gid), self.params,
-                               param_flows=self._pf if compute_param_flows else None,
-                               propagation_alg=propagation_alg, alpha=alpha)
+                if algs[gid] == "LL":
+                    layer.backward(self.node_flows, self.element_flows, self.node_mars, slab, self.params,
+                                   param_flows=pf, allow_modify_flows=allow_modify_flows, precision=precision,
+                                   negate_pflows=negate_pflows)
+                else:
+                    layer.backward(self.node_flows, self.element_flows, self.node_mars, slab, self.params,
+                                   param_flows=pf, propagation_alg=algs[gid], alpha=alpha,
+                                   negate_pflows=negate_pflows)
+            if post_callback is not None:
+                g.callback(post_callback, **cb)
 
     def _slab(self, group_id: int) -> torch.Tensor:
         """``element_mars`` rows of the product group ``group_id`` (or of the sum group right above it)."""
@@ -705,7 +743,6 @@ class TensorCircuit(nn.Module):
         if propagation_alg is None:
             propagation_alg = self.default_propagation_alg
             kwargs.update(self.propagation_alg_kwargs)
-        assert isinstance(propagation_alg, str), "per-layer propagation algorithms are not supported"
         alpha = kwargs.pop("alpha", None)
         prec = precision if precision is not None else ("bf16" if force_use_bf16 else self.precision)
         if force_use_fp32:
@@ -733,8 +770,8 @@ class TensorCircuit(nn.Module):
 
             sig = (0, id(self.node_mars), id(self.element_mars), id(self.params), B, prec)
             run = lambda: self._run_inner_forward(force_use_bf16, force_use_fp32, prec)
-            if propagation_alg != "LL":
-                self._run_inner_forward_alg(propagation_alg, alpha)
+            if not isinstance(propagation_alg, str) or propagation_alg != "LL":
+                self._run_inner_forward_alg(propagation_alg, alpha, prec)
             elif record_cudagraph and sig not in self._recorded_cuda_graphs:
                 run()  # this call's result; the capture below does not execute kernels
                 self._recorded_cuda_graphs[sig] = self._capture(run)
@@ -766,7 +803,8 @@ class TensorCircuit(nn.Module):
         torch.cuda.synchronize()
         return g
 
-    def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision, overwrite=False):
+    def _run_inner_backward(self, compute_param_flows, allow_modify_flows, precision, overwrite=False,
+                            negate_pflows=False):
         # Element marginals of every layer are still resident (one slab per product group), so no
         # product layer is recomputed here (the reference recomputes, tensorcircuit.py:446-447),
         # unless ``recompute_elements`` keeps a single shared slab.
@@ -784,7 +822,7 @@ class TensorCircuit(nn.Module):
         batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu" and
                                                   not recompute) else set()
         fz = self._fused if precision != "fp32_valu" else {"tables": dict(), "prod_groups": set(), "premod": set()}
-        runs = self._chains("bwd", precision, modify, compute_param_flows)
+        runs = self._chains("bwd", precision, modify, compute_param_flows) if not negate_pflows else dict()
         lid = len(self.inner_layer_groups) - 1
         while lid >= 0:
             run = runs.get(lid)
@@ -809,9 +847,10 @@ class TensorCircuit(nn.Module):
                 layer.backward(self.node_flows, self.element_flows, self.node_mars, self._slab(gid), self.params,
                                param_flows=pf, allow_modify_flows=modify, precision=precision,
                                premodified=modify and id(layer) in fz["premod"],
-                               child_tables=fz["tables"].get(id(layer)), overwrite_pflows=overwrite)
+                               child_tables=fz["tables"].get(id(layer)), overwrite_pflows=overwrite,
+                               negate_pflows=negate_pflows)
         if batched:
-            self._run_par_batched(modify, precision, overwrite)
+            self._run_par_batched(modify, precision, overwrite, negate_pflows)
 
     def backward(self, inputs: Optional[torch.Tensor] = None, ll_weights: Optional[torch.Tensor] = None,
                  compute_param_flows: bool = True, flows_memory: float = 1.0,
@@ -827,12 +866,12 @@ class TensorCircuit(nn.Module):
         marginalized input entry's flow is spread over its categories by ``theta``."""
         missing_mask = kwargs.get("missing_mask")
         batch_first = kwargs.get("_batch_first", True)
-        assert isinstance(propagation_alg, str), "per-layer propagation algorithms are not supported"
-        assert not logspace_flows and not negate_pflows, "only flows in linear space are implemented"
+        assert not (logspace_flows and allow_modify_flows), \
+            "logspace_flows keeps node flows as log(flow): pass allow_modify_flows=False with it"
         alpha = kwargs.pop("alpha", None)
-        assert sum_layer_pre_backward_callback is None and sum_layer_post_backward_callback is None, \
-            "sum-layer callbacks are not supported"
-        assert self.node_mars is not None and self.element_mars is not None, "Should run forward path first."
+        layerwise = (not isinstance(propagation_alg, str) or propagation_alg != "LL"
+                     or sum_layer_pre_backward_callback is not None or sum_layer_post_backward_callback is not None)
+        assert self.node_mars is not None and self.element_mars is not None, "backward() needs the marginals of a preceding forward()"
         prec = precision if precision is not None else self.precision
         if force_use_fp32:
             prec = "fp32"
@@ -841,7 +880,7 @@ class TensorCircuit(nn.Module):
         with torch.no_grad():
             B = self.node_mars.size(1)
             self._ensure_buffers(B, need_flows=True)
-            if prec == "fp32_valu" or propagation_alg != "LL" or "zero_ranges" not in self._fused:
+            if prec == "fp32_valu" or layerwise or "zero_ranges" not in self._fused:
                 self._fill(self.node_flows, 0.0)
             else:
                 for a, b in self._fused["zero_ranges"]:
@@ -861,15 +900,18 @@ class TensorCircuit(nn.Module):
             # The first backward after init_param_flows(0.0) stores the sum parameter flows instead
             # of adding them when every slot has exactly one writer; otherwise the buffer is
             # cleared first.
-            overwrite = (compute_param_flows and self._pf_fresh and self._pf_overwrite_ok
-                         and propagation_alg == "LL")
+            overwrite = (compute_param_flows and self._pf_fresh and self._pf_overwrite_ok and not layerwise)
             if compute_param_flows and not overwrite:
                 _ = self.param_flows
             sig = (1, id(self.node_flows), id(self.element_flows), id(self.node_mars), id(self.element_mars),
-                   id(self.params), id(self._pf), B, allow_modify_flows, compute_param_flows, prec, overwrite)
-            run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec, overwrite)
-            if propagation_alg != "LL":
-                self._run_inner_backward_alg(compute_param_flows, propagation_alg, alpha)
+                   id(self.params), id(self._pf), B, allow_modify_flows, compute_param_flows, prec, overwrite,
+                   negate_pflows)
+            run = lambda: self._run_inner_backward(compute_param_flows, allow_modify_flows, prec, overwrite,
+                                                   negate_pflows)
+            if layerwise:
+                self._run_inner_backward_layerwise(compute_param_flows, propagation_alg, alpha, prec,
+                                                   allow_modify_flows, negate_pflows,
+                                                   sum_layer_pre_backward_callback, sum_layer_post_backward_callback)
             elif record_cudagraph and sig not in self._recorded_cuda_graphs:
                 run()  # this call's result; the capture below does not execute kernels
                 self._recorded_cuda_graphs[sig] = self._capture(run)
@@ -897,6 +939,10 @@ class TensorCircuit(nn.Module):
                     ret = input_layer_fn(layer, inputs, self.node_flows, self.node_mars, **kwargs)
                     if not ret and ret is not None:
                         layer.backward(self._to_vb(inputs), self.node_flows, self.node_mars, **kwargs)
+            if logspace_flows:
+                # the flows were propagated (and consumed by the input layers) in linear space;
+                # the buffer is handed back as log(flow), -inf where no flow arrived
+                N.call("pjamd_flows_to_log", N.ptr(self.node_flows), self.node_flows.numel(), N.stream())
 
         if return_cache:
             cache = dict() if cache is None else cache
@@ -912,8 +958,8 @@ class TensorCircuit(nn.Module):
         times the current model's expected flows (theta * top-down probability), and the update
         then takes step 1 (``tensorcircuit.py:566-578``)."""
         assert not step_size_rescaling or self._cum_flow > 0.0, \
-            "Please perform a backward pass before calling `mini_batch_em`."
-        assert 0.0 < step_size <= 1.0, "`step_size` should be between 0 and 1."
+            "step-size rescaled EM needs flows from a backward() first"
+        assert 0.0 < step_size <= 1.0, "EM step_size must lie in (0, 1]"
         if self.param_flows is None:
             self.init_param_flows(0.0)
         if step_size_rescaling:
@@ -943,7 +989,7 @@ class TensorCircuit(nn.Module):
 
     def init_param_flows(self, flows_memory: float = 1.0, batch_size: Optional[int] = None):
         """Scale (``flows_memory`` in [0, 1]) or allocate the parameter-flow accumulators."""
-        assert 0.0 <= flows_memory <= 1.0, "`flows_memory` should be in [0.0, 1.0]"
+        assert 0.0 <= flows_memory <= 1.0, "flows_memory must lie in [0, 1]"
         assert batch_size is None, "per-sample parameter flows are not supported"
         dev = self.device
         if self._pf is None or self._pf.device != dev or self._pf.dtype != self.pflow_dtype:
@@ -975,10 +1021,10 @@ class TensorCircuit(nn.Module):
                 ns.update_param_flows(pf, clone=clone, origin_ns_only=origin_ns_only)
 
     def print_statistics(self):
-        print(f"> Number of nodes: {self.num_nodes}")
-        print(f"> Number of edges: {self.num_edges}")
-        print(f"> Number of sum parameters: {self.num_sum_params}")
-        print(f"> Number of input parameters: {sum(l.params.numel() for l in self.input_layer_group)}")
+        print(f"nodes:            {self.num_nodes}")
+        print(f"edges:            {self.num_edges}")
+        print(f"sum parameters:   {self.num_sum_params}")
+        print(f"input parameters: {sum(l.params.numel() for l in self.input_layer_group)}")
 
     def get_node_mars(self, ns: CircuitNodes):
         assert self.node_mars is not None
@@ -1021,8 +1067,8 @@ class TensorCircuit(nn.Module):
                      verbose=False):
         """Depth-layering, global id assignment and per-layer compilation (``tensorcircuit.py:920-1092``)."""
         if hasattr(self, "input_layer_group") or hasattr(self, "inner_layer_groups"):
-            raise ValueError("Attempting to initialize a TensorCircuit for the second time. "
-                             "Please instead create a new TensorCircuit instance by calling `pc = TensorCircuit(root_ns)`.")
+            raise ValueError("this TensorCircuit is already compiled; "
+                             "build a new one from the root nodes instead")
         self.root_ns._clear_tensor_circuit_hooks()
         pc_num_vars = len(self.root_ns.scope)
         depth2nodes, num_layers, max_node_bs, max_ele_bs = self._create_node_layers()
@@ -1057,7 +1103,7 @@ class TensorCircuit(nn.Module):
                 continue
 
             assert len(depth2nodes[depth]["prod"]) > 0 and len(depth2nodes[depth]["sum"]) > 0, \
-                f"Depth {depth}: (# prod nodes: {len(depth2nodes[depth]['prod'])}, # sum nodes: {len(depth2nodes[depth]['sum'])})"
+                f"layer depth {depth} lacks product or sum vectors ({len(depth2nodes[depth]['prod'])} / {len(depth2nodes[depth]['sum'])})"
             g2p = dict()
             for ns in depth2nodes[depth]["prod"]:
                 g2p.setdefault(ns.block_size, []).append(ns)
@@ -1178,7 +1224,7 @@ class TensorCircuit(nn.Module):
                     if cs.is_prod():
                         if cs in pns2layer:
                             assert pns2layer[cs] == layer, \
-                                "Disallowed circumstance: a product node requested by sum nodes at different layers."
+                                "a product vector would be needed by sum layers at two different depths"
                         else:
                             depth2nodes[layer]["prod"].append(cs)
                             pns2layer[cs] = layer

@@ -21,9 +21,9 @@ __all__ = ["inputs", "multiply", "summate", "set_block_size", "structural_proper
 def inputs(var: Union[int, Sequence[int]], num_node_blocks: int = 0, dist: Distribution = None,
            params: Optional[torch.Tensor] = None, num_nodes: int = 0, block_size: int = 0, **kwargs) -> InputNodes:
     """Vector of input nodes over the same variable(s) with one distribution (construction.py:22)."""
-    assert block_size == 0 or block_size & (block_size - 1) == 0, "`block_size` must be a power of 2."
+    assert block_size == 0 or block_size & (block_size - 1) == 0, "block_size must be a power of two"
     if num_nodes > 0:
-        assert num_node_blocks == 0, "Only one of `num_nodes` and `num_node_blocks` can be set at the same time."
+        assert num_node_blocks == 0, "give num_nodes or num_node_blocks, not both"
         if block_size == 0:
             block_size = CircuitNodes.DEFAULT_BLOCK_SIZE
         assert num_nodes % block_size == 0
@@ -34,7 +34,7 @@ def inputs(var: Union[int, Sequence[int]], num_node_blocks: int = 0, dist: Distr
 
 def multiply(nodes1, *args, edge_ids: Optional[torch.Tensor] = None, sparse_edges: bool = False, **kwargs) -> ProdNodes:
     """Vector of product nodes over children with disjoint scopes (construction.py:69)."""
-    assert nodes1.is_sum() or nodes1.is_input(), "Children of product nodes must be input or sum nodes."
+    assert nodes1.is_sum() or nodes1.is_input(), "multiply() takes input or sum vectors only"
     chs = [nodes1]
     num_node_blocks = nodes1.num_node_blocks
     block_size = nodes1.block_size
@@ -47,7 +47,7 @@ def multiply(nodes1, *args, edge_ids: Optional[torch.Tensor] = None, sparse_edge
                 f"Input nodes should have the same `num_node_blocks`, but got {ns.num_node_blocks} and {num_node_blocks}."
         assert ns.block_size == block_size, "Input nodes should have the same `block_size`."
         if not RegionGraph.ALLOW_NONDECOMPOSABLE:
-            assert not scope.contains_any(ns.scope), "Children of a `ProdNodes` should have disjoint scopes."
+            assert not scope.contains_any(ns.scope), "the children of multiply() must cover disjoint variables"
         chs.append(ns)
         scope = scope | ns.scope
     if edge_ids is not None:
@@ -64,28 +64,28 @@ def multiply(nodes1, *args, edge_ids: Optional[torch.Tensor] = None, sparse_edge
 def summate(nodes1, *args, num_node_blocks: int = 0, num_nodes: int = 0, edge_ids: Optional[torch.Tensor] = None,
             block_size: int = 0, sum_edge_ids_constructor: Optional[Callable] = None, **kwargs) -> SumNodes:
     """Vector of sum nodes over children with the same scope (construction.py:122)."""
-    assert block_size == 0 or block_size & (block_size - 1) == 0, "`block_size` must be a power of 2."
+    assert block_size == 0 or block_size & (block_size - 1) == 0, "block_size must be a power of two"
     if num_nodes > 0:
-        assert num_node_blocks == 0, "Only one of `num_nodes` and `num_node_blocks` can be set at the same time."
+        assert num_node_blocks == 0, "give num_nodes or num_node_blocks, not both"
         if block_size == 0:
             block_size = CircuitNodes.DEFAULT_BLOCK_SIZE
         assert num_nodes % block_size == 0
         num_node_blocks = num_nodes // block_size
     assert nodes1.is_prod() or nodes1.is_input(), \
-        f"Children of sum nodes must be input or product nodes, but found input of type {type(nodes1)}."
+        f"summate() takes input or product vectors, not {type(nodes1).__name__}"
     if sum_edge_ids_constructor is not None:
         edge_ids = sum_edge_ids_constructor(nodes1, *args, num_node_blocks=num_node_blocks,
                                             block_size=block_size if block_size > 0 else CircuitNodes.DEFAULT_BLOCK_SIZE,
                                             **kwargs)
     if edge_ids is not None and num_node_blocks == 0:
         num_node_blocks = int(torch.as_tensor(edge_ids)[0, :].max().item()) + 1
-    assert num_node_blocks > 0, "Number of node blocks should be greater than 0."
+    assert num_node_blocks > 0, "a vector needs at least one node block"
     chs = [nodes1]
     for ns in args:
         assert ns.is_prod() or ns.is_input(), \
             f"Children of sum nodes must be input or product nodes, but found input of type {type(ns)}."
         if not RegionGraph.ALLOW_NONSMOOTH:
-            assert ns.scope == nodes1.scope, "Children of a `SumNodes` should have the same scope."
+            assert ns.scope == nodes1.scope, "the children of summate() must cover the same variables"
         chs.append(ns)
     return SumNodes(num_node_blocks, chs, edge_ids, block_size=block_size, **kwargs)
 
@@ -94,7 +94,7 @@ class set_block_size(DecoratorContextManager):
     """Sets ``CircuitNodes.DEFAULT_BLOCK_SIZE`` inside the ``with`` block (construction.py:193)."""
 
     def __init__(self, block_size: int = 1):
-        assert block_size & (block_size - 1) == 0, "`block_size` must be a power of 2."
+        assert block_size & (block_size - 1) == 0, "block_size must be a power of two"
         self.block_size = block_size
         self.original_block_size = None
 

@@ -76,7 +76,7 @@ class CircuitNodes:
             block_size = CircuitNodes.DEFAULT_BLOCK_SIZE
         assert num_node_blocks > 0
         assert block_size > 0 and (block_size & (block_size - 1)) == 0, \
-            f"`block_size` must be a power of 2, but got `block_size={block_size}`."
+            f"block_size must be a power of two (got {block_size})"
 
         self.num_node_blocks = int(num_node_blocks)
         self.block_size = int(block_size)
@@ -128,7 +128,7 @@ class CircuitNodes:
         return len(self.scope)
 
     def duplicate(self, *args, **kwargs):
-        raise ValueError(f"{type(self)} does not support `duplicate`.")
+        raise ValueError(f"{type(self).__name__} cannot be duplicated")
 
     def _draw_own_params(self, perturbation: float, **kwargs):
         """Random initialisation of this vector's own parameters (no recursion)."""

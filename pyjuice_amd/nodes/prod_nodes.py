@@ -69,18 +69,18 @@ class ProdNodes(CircuitNodes):
         if edge_ids is None:
             for c in self.chs:
                 assert self.num_node_blocks == c.num_node_blocks and self.block_size == c.block_size, \
-                    "Cannot create edges implicitly since # nodes do not match."
+                    "without edge_ids every child must have as many node blocks as the product"
             edge_ids = torch.arange(self.num_node_blocks).unsqueeze(1).repeat(1, self.num_chs)
         if isinstance(edge_ids, np.ndarray):
             edge_ids = torch.from_numpy(edge_ids)
         edge_ids = edge_ids.long()
         if edge_ids.size(0) == self.num_node_blocks:
-            assert edge_ids.size(1) == self.num_chs, f"Expect edge_ids.size() == ({self.num_node_blocks}, {self.num_chs})."
+            assert edge_ids.size(1) == self.num_chs, f"edge_ids must have shape ({self.num_node_blocks}, {self.num_chs}) for a block-sparse product"
             for i, c in enumerate(self.chs):
                 assert torch.all(edge_ids[:, i] >= 0), "Edge index underflow."
                 assert torch.all(edge_ids[:, i] < c.num_node_blocks), "Edge index overflow."
         elif edge_ids.size(0) == self.num_nodes:
-            assert edge_ids.size(1) == self.num_chs, f"Expect edge_ids.size() == ({self.num_nodes}, {self.num_chs})."
+            assert edge_ids.size(1) == self.num_chs, f"edge_ids must have shape ({self.num_nodes}, {self.num_chs}) for a sparse product"
             for i, c in enumerate(self.chs):
                 assert torch.all(edge_ids[:, i] >= 0), "Edge index underflow."
                 assert torch.all(edge_ids[:, i] < c.num_nodes), "Edge index overflow."

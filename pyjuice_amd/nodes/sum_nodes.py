@@ -48,10 +48,10 @@ class SumNodes(CircuitNodes):
     def __init__(self, num_node_blocks: int, chs: Sequence[CircuitNodes], edge_ids=None,
                  params: Optional[torch.Tensor] = None, zero_param_mask: Optional[torch.Tensor] = None,
                  block_size: int = 0, **kwargs):
-        assert len(chs) > 0, "`SumNodes` must have at least one child."
+        assert len(chs) > 0, "summate() needs at least one child vector"
         for i in range(1, len(chs)):
             assert chs[0].block_size == chs[i].block_size, \
-                "`block_size` of the children of a `SumNodes` should be the same."
+                "the children of summate() must share one block size"
         rg = InnerRegionNode([c.region_node for c in chs])
         super().__init__(num_node_blocks, rg, block_size=block_size, **kwargs)
 
@@ -84,7 +84,7 @@ class SumNodes(CircuitNodes):
             chs = self.chs
         else:
             assert self.num_chs == len(chs), \
-                f"Number of new children ({len(chs)}) must match the number of original children ({self.num_chs})."
+                f"duplicate() got {len(chs)} children for a vector with {self.num_chs}"
             for o, n in zip(self.chs, chs):
                 assert type(o) == type(n), f"Child type not match: ({type(n)} != {type(o)})."
                 assert o.num_node_blocks == n.num_node_blocks and o.block_size == n.block_size
@@ -138,7 +138,7 @@ class SumNodes(CircuitNodes):
                 and self.block_size == params.size(2) and self.ch_block_size == params.size(3)
             self._params = params[self.edge_ids[0, :], self.edge_ids[1, :], :, :].clone().contiguous()
         else:
-            raise ValueError("Unsupported parameter input.")
+            raise ValueError("params must be a tensor of edge-block weights or a dense weight matrix")
         if self.provided("_zero_param_mask"):
             self._params[self._zero_param_mask] = 0.0
         if normalize:
@@ -175,7 +175,7 @@ class SumNodes(CircuitNodes):
 
     def update_parameters(self, params: torch.Tensor, clone: bool = True):
         """Copy the compiled circuit's flat parameters back into ``_params`` (sum_nodes.py:271-296)."""
-        assert self.provided("_param_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        assert self.provided("_param_range"), "this sum vector is not part of a compiled circuit"
         if self.is_tied():
             return None
         psid, peid = self._param_range
@@ -187,7 +187,7 @@ class SumNodes(CircuitNodes):
         self._params = p[local].permute(0, 2, 1).contiguous()
 
     def update_param_flows(self, param_flows: torch.Tensor, origin_ns_only: bool = True, clone: bool = True):
-        assert self.provided("_param_flow_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        assert self.provided("_param_flow_range"), "this sum vector is not part of a compiled circuit"
         if origin_ns_only and self.is_tied():
             return None
         pfsid, pfeid = self._param_flow_range
@@ -200,7 +200,7 @@ class SumNodes(CircuitNodes):
 
     def gather_parameters(self, params: torch.Tensor):
         """Scatter ``_params`` into the compiled flat parameter vector (sum_nodes.py:333-347)."""
-        assert self.provided("_param_range"), "The `SumNodes` has not been compiled into a `TensorCircuit`."
+        assert self.provided("_param_range"), "this sum vector is not part of a compiled circuit"
         if self.is_tied() or not self.has_params():
             return None
         psid, peid = self._param_range
@@ -243,7 +243,7 @@ class SumNodes(CircuitNodes):
         if reorder:
             order = torch.argsort(edge_ids[0, :] * self.num_ch_node_blocks + edge_ids[1, :])
             edge_ids = edge_ids[:, order].contiguous()
-        assert edge_ids.size(0) == 2, "Expect `edge_ids.size(0) == 2`."
+        assert edge_ids.size(0) == 2, "sum edge_ids must have two rows (parent block, child block)"
         assert torch.all(edge_ids >= 0), "Edge index underflow."
         assert torch.all(edge_ids[0, :] < self.num_node_blocks) and \
             torch.all(edge_ids[1, :] < self.num_ch_node_blocks), "Edge index overflow."

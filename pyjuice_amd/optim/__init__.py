@@ -14,7 +14,7 @@ class CircuitOptimizer:
                  **kwargs):
         self.pc = pc
         self.base_optimizer = base_optimizer
-        assert method in self.SUPPORTED_OPTIM_METHODS, f"Unsupported optimization method {method} for PCs."
+        assert method in self.SUPPORTED_OPTIM_METHODS, f"CircuitOptimizer has no method {method!r}."
         self.method = method
         self.lr = lr
         self.pseudocount = pseudocount

@@ -27,7 +27,7 @@ def sample(pc, num_samples: Optional[int] = None, conditional: bool = False, uni
     assert not conditional, "conditional sampling is not implemented"
     assert num_samples is not None and num_samples > 0
     r0, r1 = pc.root_ns._output_ind_range
-    assert r1 - r0 == 1, "It is ambiguous to sample from multi-head PCs."
+    assert r1 - r0 == 1, "sampling needs a circuit with a single root node"
     dev = pc.device
     assert dev.type == "cuda", "sampling runs on the HIP device; call `pc.to('cuda')` first"
     S = int(num_samples)

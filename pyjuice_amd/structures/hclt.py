@@ -32,8 +32,11 @@ def mutual_information(x1: torch.Tensor, x2: torch.Tensor, num_bins: int, sigma:
     bins = torch.linspace(0, 1, num_bins, device=x1.device)
     p1 = _soft_bins(x1, bins, sigma)
     p2 = _soft_bins(x2, bins, sigma)
-    joint = torch.matmul(p1.reshape(B, K1 * num_bins).t().contiguous(),
-                         p2.reshape(B, K2 * num_bins).contiguous()).reshape(K1, num_bins, K2, num_bins) / B
+    # The joint histogram is formed with the same torch contraction (subscripts and operand shapes)
+    # as hclt.py:27, so that the same torch build rounds it identically and near-tied spanning-tree
+    # edges resolve the same way.
+    joint = torch.einsum("bia,baj->ij", p1.reshape(B, K1 * num_bins, 1),
+                         p2.reshape(B, 1, K2 * num_bins)).reshape(K1, num_bins, K2, num_bins) / B
     m1 = (p1 / p1.sum(dim=2, keepdim=True)).mean(dim=0)
     m2 = (p2 / p2.sum(dim=2, keepdim=True)).mean(dim=0)
     j = joint / joint.sum(dim=(1, 3), keepdim=True)

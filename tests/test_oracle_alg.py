@@ -103,3 +103,33 @@ def test_flows_reach_every_variable_once(name, alg, alpha):
         if n.is_input():
             per_var[n.scope.to_list()[0]] += f[n].sum(axis=0)
     np.testing.assert_allclose(per_var, 1.0, rtol=1e-9)
+
+
+def test_per_vector_algorithm_map():
+    """A map from sum vectors to algorithms reduces to the single-algorithm oracle when every
+    vector takes the same one, and mixes per vector otherwise (a vector's value depends only on
+    its own algorithm given its children)."""
+    import kat_circuits
+    from pyjuice_amd.model import TensorCircuit
+    from pyjuice_amd.nodes.nodes import node_iterator
+    ns = kat_circuits.simple_model_dag()
+    TensorCircuit(ns, layer_sparsity_tol=0.1).update_parameters()
+    X = np.random.default_rng(1).integers(0, 4, size=[16, 4])
+    sums = [n for n in node_iterator(ns) if n.is_sum()]
+    for a in ("LL", "MPE", "GeneralLL"):
+        one = dag_eval.forward(ns, X, alg=a, alpha=0.5)
+        mp = dag_eval.forward(ns, X, alg={n: a for n in sums}, alpha=0.5)
+        for n in sums:
+            assert np.array_equal(one[n], mp[n])
+        f1, p1 = dag_eval.backward(ns, X, one, alg=a, alpha=0.5)
+        f2, p2 = dag_eval.backward(ns, X, mp, alg={n: a for n in sums}, alpha=0.5)
+        for n in sums:
+            assert np.array_equal(f1[n], f2[n]) and np.array_equal(p1[n], p2[n])
+    mixed = {n: ("MPE" if i % 2 else "LL") for i, n in enumerate(sums)}
+    mv = dag_eval.forward(ns, X, alg=mixed)
+    ll, mpe = dag_eval.forward(ns, X), dag_eval.forward(ns, X, alg="MPE")
+    for i, n in enumerate(sums):
+        child = np.concatenate([mv[c] for c in n.chs], axis=0)
+        W = dag_eval.sum_dense_params(n)
+        ref = dag_eval._alg_sum(W, child, "MPE", 1.0) if i % 2 else dag_eval._lse_matmul(W, child)
+        assert np.allclose(mv[n], ref)
