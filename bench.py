@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=32)
     ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
+    ap.add_argument("--sample-count", type=int, default=512,
+                    help="samples per pass of the sampling leg (0: no sampling leg)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="profiling aid: run one device-to-device copy of this many MiB first, the known-size "
                          "reference for hardware byte counters")
@@ -220,6 +222,39 @@ def roofline_of(name, d, prec, args):
             "algorithmic_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1))}
 
 
+def sampling_leg(pc, args):
+    """Ancestral sampling throughput: ``--sample-count`` samples per pass with in-kernel
+    counter-based uniforms, the pass recorded once as a HIP graph and replayed under a new seed
+    each time (``--no-graph``: launched eagerly); plus the eager per-kernel breakdown."""
+    import torch
+
+    import pyjuice_amd as juice
+    from pyjuice_amd import _native as N
+
+    S = args.sample_count
+    graph = bool(args.graph)
+    juice.queries.sample(pc, num_samples=S, seed=1, record_cudagraph=graph)
+    for i in range(args.warmup):
+        juice.queries.sample(pc, num_samples=S, seed=2 + i, apply_cudagraph=graph)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for i in range(args.steps):
+        juice.queries.sample(pc, num_samples=S, seed=100 + i, apply_cudagraph=graph)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    with N.LaunchTimer(["pjamd_sample_sum", "pjamd_sample_prod", "pjamd_input_sample"]) as T:
+        for i in range(3):
+            juice.queries.sample(pc, num_samples=S, seed=200 + i, apply_cudagraph=False)
+    kern = T.summary()
+    log(f"[bench] sampling: {ms:.3f} ms per {S}-sample pass ({S / (ms * 1e-3):.0f} samples/s)")
+    return {"samples_per_s": round(S / (ms * 1e-3), 1), "ms_per_pass": round(ms, 4), "samples_per_pass": S,
+            "hip_graph": graph, "uniforms": "in-kernel Philox4x32-10, one per (node, sample)",
+            "kernels_ms_per_pass": {k: {"ms": round(v["ms"] / 3, 4), "launches": v["launches"] // 3}
+                                    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"])}}
+
+
 def _host_threads():
     n = os.environ.get("OMP_NUM_THREADS")
     if n and n.isdigit():
@@ -234,7 +269,8 @@ def cpu_baseline(pc, x_np, lls_by_prec, args):
       on the same compiled index tables and parameters), timed on a bounded sample of the
       batch (``--cpu-samples`` samples per step, steps repeated until ~10 s of work);
     * accuracy: the FP64 build of the same restatement on the first ``--cpu-samples``
-      samples, against which every device precision's log-likelihoods are compared.
+      samples, against which every device precision's log-likelihoods, parameter flows (sum and
+      input) and EM step are compared.
     """
     from oracle.pc_cpu import CompiledCPU
 
@@ -246,6 +282,42 @@ def cpu_baseline(pc, x_np, lls_by_prec, args):
     for prec, lls in lls_by_prec.items():
         d = np.abs(lls[:n] - ref) / V
         acc[prec] = {"mean": float(d.mean()), "max": float(d.max()), "samples": int(n)}
+
+    # parameter flows (unit root flows over the same samples) and one EM step (0.1, 0.1) of every
+    # precision against the FP64 restatement, from the same starting parameters
+    import torch
+    o64 = CompiledCPU(pc, np.float64)
+    o64.forward(xs)
+    o64.backward()
+    pf_ref = o64.param_flows.copy()
+    in_ref = [d["pflows"].copy() for d in o64.inputs]
+    o64.mini_batch_em(step_size=0.1, pseudocount=0.1)
+    p0 = pc.params.detach().clone()
+    in0 = [l.params.detach().clone() for l in pc.input_layer_group]
+    xd = torch.from_numpy(xs).to(pc.device)
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+    for prec in lls_by_prec:
+        pc.params.data.copy_(p0)
+        for l, p in zip(pc.input_layer_group, in0):
+            l.params.data.copy_(p)
+        with torch.no_grad():
+            pc(xd, precision=prec)
+            pc.backward(xd, flows_memory=0.0, precision=prec)
+        acc[prec]["param_flows_rel_err"] = rel(pc.param_flows.detach().cpu().numpy().astype(np.float64), pf_ref)
+        acc[prec]["input_param_flows_rel_err"] = max(
+            rel(l.param_flows.detach().cpu().numpy().astype(np.float64), r[:l.param_flows.numel()])
+            for l, r in zip(pc.input_layer_group, in_ref))
+        pc.mini_batch_em(step_size=0.1, pseudocount=0.1)
+        em = float(np.abs(pc.params.detach().cpu().numpy() - o64.params).max())
+        for l, d in zip(pc.input_layer_group, o64.inputs):
+            em = max(em, float(np.abs(l.params.detach().cpu().numpy() - d["params"][:l.params.numel()]).max()))
+        acc[prec]["em_params_max_abs_err"] = em
+    pc.params.data.copy_(p0)
+    for l, p in zip(pc.input_layer_group, in0):
+        l.params.data.copy_(p)
 
     cpu = CompiledCPU(pc, np.float32)
     steps, t_tot = 0, 0.0
@@ -336,6 +408,13 @@ def main():
     dom = max(kern.items(), key=lambda kv: kv[1]["ms"])
     roof = roofline_of(dom[0], dom[1], main_prec, args)
 
+    sampling = None
+    if args.sample_count > 0 and not args.only:
+        pc.params.data.copy_(state["params"])
+        for l, p in zip(pc.input_layer_group, in_state):
+            l.params.data.copy_(p)
+        sampling = sampling_leg(pc, args)
+
     cpu, acc = (None, None)
     if not args.no_cpu_baseline and not args.only:
         pc.params.data.copy_(state["params"])
@@ -365,6 +444,7 @@ def main():
         "precisions": {p: {"samples_per_s": round(v["samples_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 4)}
                        for p, v in results.items()},
         "dll_per_var_vs_fp64": acc,
+        "sampling": sampling,
         "kernels_ms_per_step": {k: {"ms": round(v["ms"], 4), "launches": v["launches"],
                                     "GB/s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
                                     "TFLOP/s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2),

@@ -156,14 +156,19 @@ int pjamd_input_em(int kind, float* params, const void* pflows, int pf64, const 
                    int keep_zero, pjamd_stream_t stream);
 
 /* For every (node n, sample s) with node_act[node_offset + n][s] != 0, write a value of
- * variable vids[n] drawn with the caller's uniform uniforms[node_offset + n][s]: Categorical
- * takes the first category whose FP32 running CDF reaches u, Bernoulli takes u < theta.
- * out is int64 (out_i64 = 1) or int32, [num_vars][S].  Replaces InputLayer.sample
+ * variable vids[n] and clear the flag: Categorical takes the first category whose FP32 running
+ * CDF (left to right) reaches u, or 0 if none does; Bernoulli takes u < theta.  u is
+ * uniforms[node_offset + n][s], or, when uniforms is NULL, the counter-based uniform of
+ * (node_offset + n, s) under the key seed[0..1] (Philox4x32-10, see pjamd_sample_sum).  out is
+ * int64 (out_i64 = 1) or uint8, [num_vars][S].  cdf (optional, Categorical with K <= 4096): the
+ * parameters' running sums from pjamd_sample_prefix, used instead of summing per workgroup.
+ * Replaces InputLayer.sample
  * (input_layer.py:573-661) with Categorical.sample_fn (categorical.py:137-162) /
- * Bernoulli.sample_fn (bernoulli.py:79-89); uniforms come from the caller instead of a seed. */
-int pjamd_input_sample(int kind, void* out, const float* params, const uint8_t* node_act, const float* uniforms,
-                       const int* vids, const int64_t* s_pids, const int* ncats, int out_i64, int64_t num_nodes,
-                       int64_t node_offset, int64_t S, pjamd_stream_t stream);
+ * Bernoulli.sample_fn (bernoulli.py:79-89), which draw tl.rand(seed, offset). */
+int pjamd_input_sample(int kind, void* out, const float* params, const float* cdf, uint8_t* node_act,
+                       const float* uniforms,
+                       const uint32_t* seed, const int* vids, const int64_t* s_pids, const int* ncats, int out_i64,
+                       int64_t num_nodes, int64_t node_offset, int64_t S, pjamd_stream_t stream);
 
 /* ----------------------------------------------------------------------------- product layers */
 
@@ -430,17 +435,36 @@ int pjamd_pflow_fuse(void* pflows, int pf64, int64_t target, int64_t n, const in
 
 /* ----------------------------------------------------------------------------- sampling */
 
-/* Top-down sampling through one sum partition: for each active node (node_act != 0) and
- * sample s, select child k = #{k : cumsum_k(theta) <= u} (clamped to the last real child) with
- * the caller's uniform u = uniforms[node][s], and mark element_act[cids[j][k] + ...][s].
+/* Top-down sampling through one sum partition: for each active node (node_act != 0, cleared as
+ * it is read) and sample s, select a child and mark element_act[cids[j][k] + ...][s].  The
+ * children are scanned in chunks of 512 (the reference's BLOCK_K, sample.py:115-131): within a
+ * chunk k is the first child whose FP32 running sum of theta (left to right, from 0) exceeds u;
+ * when u reaches the chunk's total, u -= total and the next chunk is scanned; past the last
+ * chunk the last real child is taken (where the reference would read index -1).  u is
+ * uniforms[node][s] or, when uniforms is NULL, the counter-based uniform of (node, s): word 0 of
+ * Philox4x32-10 (Salmon et al., SC 2011) with key (seed[0], seed[1]) and counter (s, node, 0, 0),
+ * as (x >> 8) * 2^-24.  seed is a device pointer, so a recorded graph takes a new key per replay.
+ * cdf (optional): the running sums of params from pjamd_sample_prefix (per node, chunks of 512,
+ * in the order of the table's children), read instead of summing per workgroup; valid when each
+ * row's parameters are the run pids[j][0] + k * bs.
  * Replaces sample_sum_layer (src/pyjuice/queries/sample.py:38-181). */
-int pjamd_sample_sum(uint8_t* elem_act, const uint8_t* node_act, const float* uniforms, const float* params,
-                     const int* nids, const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs,
-                     int64_t S, pjamd_stream_t stream);
+int pjamd_sample_sum(uint8_t* elem_act, uint8_t* node_act, const float* uniforms, const uint32_t* seed,
+                     const float* params, const float* cdf, const int* nids, const int* cids, const int* pids,
+                     const int* n_chs, int n_blocks, int K, int bs, int64_t S, pjamd_stream_t stream);
 
-/* Marks every child of an active product node active.  Replaces sample_prod_layer
+/* Running sums of strided rows: out[row_base[r] + c * row_stride[r]] = sum over c' <= c of
+ * in[row_base[r] + c' * row_stride[r]], c < row_len[r], in FP32 left to right, restarting at
+ * every multiple of chunk (0: never).  With the sum parameters' rows (one per node, stride = block
+ * size, chunk 512) it gives the cdf of pjamd_sample_sum; with the Categorical rows (stride 1, no
+ * chunks) that of pjamd_input_sample, so sampling kernels do not sum parameters per workgroup.
+ * No PyJuice counterpart (its sampling kernels take tl.cumsum per program, sample.py:125). */
+int pjamd_sample_prefix(float* out, const float* in, const int64_t* row_base, const int* row_stride,
+                        const int* row_len, int64_t n_rows, int chunk, pjamd_stream_t stream);
+
+/* Marks every child of an active product node active and clears the product's flag, so the
+ * activity masks are zero again after a pass.  Replaces sample_prod_layer
  * (src/pyjuice/queries/sample.py:270-333). */
-int pjamd_sample_prod(uint8_t* node_act, const uint8_t* elem_act, const int* nids, const int* cids, int n_blocks,
+int pjamd_sample_prod(uint8_t* node_act, uint8_t* elem_act, const int* nids, const int* cids, int n_blocks,
                       int K, int bs, int64_t S, pjamd_stream_t stream);
 
 /* ----------------------------------------------------------------------------- diagnostics */

@@ -32,7 +32,7 @@ import numpy as np
 
 from pyjuice_amd.nodes.nodes import node_iterator
 
-__all__ = ["sum_dense_params", "forward", "backward", "em_update", "sample", "input_params"]
+__all__ = ["sum_dense_params", "forward", "backward", "em_update", "sample", "input_params", "philox_uniforms"]
 
 
 def _np(t):
@@ -269,16 +269,68 @@ def em_update(root, pflows: Dict, step_size: float, pseudocount: float = 0.0,
     return out
 
 
-def sample(root, uniforms: np.ndarray, num_vars: int) -> np.ndarray:
-    """Ancestral samples ``[S, num_vars]`` with the uniform of node ``id`` (its compiled row
-    ``ns._output_ind_range[0] + i``) for sample ``s`` at ``uniforms[id, s]``.
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+    SC 2011) on arrays: ``ctr`` four uint32 words, ``key`` two; returns the four output words."""
+    M0, M1, W0, W1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57), np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+    mask = np.uint64(0xFFFFFFFF)
+    x0, x1, x2, x3 = (np.asarray(c, dtype=np.uint64) & mask for c in ctr)
+    k0, k1 = (np.asarray(k, dtype=np.uint64) & mask for k in key)
+    for _ in range(10):
+        p0, p1 = M0 * x0, M1 * x2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        x0, x1, x2, x3 = hi1 ^ x1 ^ k0, lo1, hi0 ^ x3 ^ k1, lo0
+        k0, k1 = (k0 + W0) & mask, (k1 + W1) & mask
+    return x0, x1, x2, x3
 
-    A sum node's children are ordered as its edge list (flattened per child node), the running
-    sum of their weights is accumulated in FP32 left to right and the pick is the number of
-    running sums <= u (clamped to the last child); a Categorical node picks the first category
-    whose FP32 running sum reaches u (0 if none); Bernoulli gives ``u < theta``."""
-    U = np.asarray(uniforms, dtype=np.float32)
-    S = U.shape[1]
+
+def philox_uniforms(seed: int, rows, cols) -> np.ndarray:
+    """Counter-based uniforms of (node row, sample column) pairs: word 0 of Philox4x32-10 with key
+    (seed & 0xffffffff, seed >> 32) and counter (col, row, 0, 0), as (x >> 8) * 2^-24 in
+    float32.  The device kernels draw the same numbers."""
+    rows = np.asarray(rows, dtype=np.uint64)
+    cols = np.asarray(cols, dtype=np.uint64)
+    z = np.zeros_like(cols)
+    x0 = philox4x32_10((cols, rows, z, z), (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF))[0]
+    return (x0 >> np.uint64(8)).astype(np.float32) * np.float32(2.0 ** -24)
+
+
+def _chunked_pick(w: np.ndarray, u: np.ndarray, chunk: int = 512) -> np.ndarray:
+    """Child picks of one sum node for uniforms ``u``: children in chunks of ``chunk``; within a
+    chunk the first child whose FP32 running sum (from 0, left to right) exceeds u; a u that
+    reaches the chunk's total continues in the next chunk with the total subtracted (in FP32);
+    past the last chunk the last child (``queries/sample.py:115-131``)."""
+    n = w.shape[0]
+    pick = np.full(u.shape, -1, dtype=np.int64)
+    u = u.astype(np.float32).copy()
+    for c0 in range(0, n, chunk):
+        cum = np.cumsum(w[c0:c0 + chunk], dtype=np.float32)
+        todo = pick < 0
+        k = np.searchsorted(cum, u[todo], side="right")
+        hit = k < cum.shape[0]
+        idx = np.nonzero(todo)[0]
+        pick[idx[hit]] = c0 + k[hit]
+        u[idx[~hit]] = u[idx[~hit]] - cum[-1]
+    pick[pick < 0] = n - 1
+    return pick
+
+
+def sample(root, uniforms, num_vars: int) -> np.ndarray:
+    """Ancestral samples ``[S, num_vars]``.  ``uniforms`` is either ``[num_nodes, S]`` (the uniform
+    of node ``id``, its compiled row ``ns._output_ind_range[0] + i``, for sample ``s`` at
+    ``uniforms[id, s]``) or a tuple ``(fn, S)`` with ``fn(rows, cols)`` returning the uniforms of
+    those (row, sample) pairs (e.g. ``philox_uniforms`` with a seed).
+
+    A sum node's children are ordered as its edge list (flattened per child node) and picked by
+    ``_chunked_pick`` (FP32 running sums in chunks of 512); a Categorical node picks the first
+    category whose FP32 running sum reaches u (0 if none); Bernoulli gives ``u < theta``."""
+    if isinstance(uniforms, tuple):
+        ufn, S = uniforms
+    else:
+        U = np.asarray(uniforms, dtype=np.float32)
+        S = U.shape[1]
+        ufn = lambda r, c: U[r, c]
     act = {root: np.ones([root.num_nodes, S], dtype=bool)}
     out = np.zeros([S, num_vars], dtype=np.int64)
     for ns in node_iterator(root, reverse=True):
@@ -300,10 +352,8 @@ def sample(root, uniforms: np.ndarray, num_vars: int) -> np.ndarray:
                     if cols.size == 0:
                         continue
                     w = blocks[es][:, i, :].reshape(-1)
-                    cum = np.cumsum(w, dtype=np.float32)
-                    pick = np.searchsorted(cum, U[off + n, cols], side="right")
-                    pick = np.minimum(pick, w.shape[0] - 1)
-                    chosen[n, cols] = ch_nodes[pick]
+                    u = ufn(np.full(cols.shape, off + n), cols)
+                    chosen[n, cols] = ch_nodes[_chunked_pick(w, u)]
             s0 = 0
             for c in ns.chs:
                 sel = (chosen >= s0) & (chosen < s0 + c.num_nodes)
@@ -322,7 +372,7 @@ def sample(root, uniforms: np.ndarray, num_vars: int) -> np.ndarray:
             v = ns.scope.to_list()[0]
             th = input_params(ns).astype(np.float32)
             nn_, ss = np.nonzero(a)
-            u = U[off + nn_, ss]
+            u = ufn(off + nn_, ss)
             if ns.dist.get_signature() == "Categorical":
                 cum = np.cumsum(th, axis=1, dtype=np.float32)
                 hit = cum[nn_] >= u[:, None]

@@ -44,7 +44,7 @@ _SIGNATURES = {
     "pjamd_input_backward_tied": [_i32, _c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i64, _i32, _i64, _i64, _i32,
                                   _c_p],
     "pjamd_input_em": [_i32, _c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _i64, _f32, _f32, _i32, _c_p],
-    "pjamd_input_sample": [_i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i64, _i64, _c_p],
+    "pjamd_input_sample": [_i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i64, _i64, _c_p],
     "pjamd_prod_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_prod_backward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_data_mask": [_c_p, _c_p, _i32, _i64, _i64, _c_p],
@@ -76,7 +76,8 @@ _SIGNATURES = {
     "pjamd_chain_backward": [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i64, _i32, _c_p],
     "pjamd_sum_em": [_c_p, _c_p, _i32, _c_p, _i32, _f32, _f32, _i32, _i32, _c_p],
     "pjamd_pflow_fuse": [_c_p, _i32, _i64, _i64, _c_p, _i32, _c_p],
-    "pjamd_sample_sum": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
+    "pjamd_sample_sum": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
+    "pjamd_sample_prefix": [_c_p, _c_p, _c_p, _c_p, _c_p, _i64, _i32, _c_p],
     "pjamd_sample_prod": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],
     "pjamd_fill_f32": [_c_p, _f32, _i64, _c_p],
     "pjamd_debug_mfma_probe": [_i32, _c_p, _c_p, _c_p, _c_p],

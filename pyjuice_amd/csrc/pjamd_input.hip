@@ -673,30 +673,80 @@ __global__ __launch_bounds__(256) void bern_em_kernel(float* __restrict__ params
     params[s_pids[i]] = (float)((1.0 - step) * th + step * upd);
 }
 
+// Input sampling.  A workgroup takes one node and 256 samples; if any of them is active (flags
+// cleared as they are read), a Categorical node's parameters are staged in LDS, one thread turns
+// them into FP32 running sums (left to right), and every active sample takes the first category
+// whose running sum reaches u (binary search; 0 if none).  Category counts above kCatLds scan the
+// parameters per sample instead, kCatBatch loads at a time.
+constexpr int kCatLds = 4096;
+constexpr int kCatBatch = 16;
+
 template <typename OUT>
 __global__ __launch_bounds__(256) void input_sample_kernel(int kind, OUT* __restrict__ out, const float* __restrict__ params,
-                                                           const uint8_t* __restrict__ node_act,
-                                                           const float* __restrict__ uniforms, const int* __restrict__ vids,
+                                                           const float* __restrict__ cdf, uint8_t* __restrict__ node_act,
+                                                           const float* __restrict__ uniforms,
+                                                           const uint32_t* __restrict__ seed, const int* __restrict__ vids,
                                                            const int64_t* __restrict__ s_pids, const int* __restrict__ ncats,
                                                            int64_t num_nodes, int64_t node_offset, int64_t S) {
-    const int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    __shared__ float P[kCatLds];
+    const int tid = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.y * blockDim.x + tid;
     const int64_t i = blockIdx.x;
-    if (s >= S) return;
     const int64_t row = (node_offset + i) * S + s;
-    if (!node_act[row]) return;
-    const float u = uniforms[row];
-    int pick = 0;
-    if (kind == 0) {
-        const int K = ncats[i];
-        float cum = 0.0f;
-        for (int k = 0; k < K; ++k) {
-            cum += params[s_pids[i] + k];
-            if (cum >= u) { pick = k; break; }
-        }
-    } else {
-        pick = (u < params[s_pids[i]]) ? 1 : 0;
+    const bool act = s < S && node_act[row] != 0;
+    if (!__syncthreads_or(act)) return;
+    float u = 0.0f;
+    if (act) {
+        node_act[row] = 0;
+        u = uniforms != nullptr ? uniforms[row] : philox_uniform(seed[0], seed[1], (uint32_t)s, (uint32_t)(node_offset + i));
     }
-    out[(int64_t)vids[i] * S + s] = (OUT)pick;
+    const float* th = params + s_pids[i];
+    int pick = 0;
+    if (kind == 1) {
+        pick = (u < th[0]) ? 1 : 0;
+    } else {
+        const int K = ncats[i];
+        if (K <= kCatLds) {
+            // the parameters, or their running sums when precomputed (pjamd_sample_prefix)
+            const float* src = cdf != nullptr ? cdf + s_pids[i] : th;
+            for (int k = tid; k < K; k += blockDim.x) P[k] = src[k];
+            __syncthreads();
+            if (tid == 0 && cdf == nullptr) {
+                float c = 0.0f;
+                for (int k = 0; k < K; ++k) {
+                    c += P[k];
+                    P[k] = c;
+                }
+            }
+            __syncthreads();
+            if (act && K > 0 && P[K - 1] >= u) {
+                int lo = 0, hi = K - 1;        // first k with P[k] >= u
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (P[mid] >= u) hi = mid;
+                    else lo = mid + 1;
+                }
+                pick = lo;
+            }
+        } else if (act) {
+            float cum = 0.0f;
+            int found = -1;
+            for (int k0 = 0; k0 < K && found < 0; k0 += kCatBatch) {
+                float w[kCatBatch];
+#pragma unroll
+                for (int q = 0; q < kCatBatch; ++q) w[q] = th[min(k0 + q, K - 1)];
+#pragma unroll
+                for (int q = 0; q < kCatBatch; ++q) {
+                    if (k0 + q < K && found < 0) {
+                        cum += w[q];
+                        if (cum >= u) found = k0 + q;
+                    }
+                }
+            }
+            pick = found < 0 ? 0 : found;
+        }
+    }
+    if (act) out[(int64_t)vids[i] * S + s] = (OUT)pick;
 }
 
 }  // namespace
@@ -862,18 +912,19 @@ PJAMD_EXPORT int pjamd_input_em(int kind, float* params, const void* pflows, int
     return PJAMD_OK;
 }
 
-PJAMD_EXPORT int pjamd_input_sample(int kind, void* out, const float* params, const uint8_t* node_act,
-                                    const float* uniforms, const int* vids, const int64_t* s_pids, const int* ncats,
-                                    int out_i64, int64_t num_nodes, int64_t node_offset, int64_t S, hipStream_t stream) {
-    PJAMD_REQUIRE(num_nodes >= 0 && S > 0 && (kind == 0 || kind == 1));
+PJAMD_EXPORT int pjamd_input_sample(int kind, void* out, const float* params, const float* cdf, uint8_t* node_act,
+                                    const float* uniforms, const uint32_t* seed, const int* vids,
+                                    const int64_t* s_pids, const int* ncats, int out_i64, int64_t num_nodes,
+                                    int64_t node_offset, int64_t S, hipStream_t stream) {
+    PJAMD_REQUIRE(num_nodes >= 0 && S > 0 && (kind == 0 || kind == 1) && (uniforms != nullptr || seed != nullptr));
     if (num_nodes == 0) return PJAMD_OK;
     dim3 grid((unsigned)num_nodes, cdiv(S, 256));
     if (out_i64)
-        input_sample_kernel<int64_t><<<grid, 256, 0, stream>>>(kind, (int64_t*)out, params, node_act, uniforms, vids,
-                                                              s_pids, ncats, num_nodes, node_offset, S);
+        input_sample_kernel<int64_t><<<grid, 256, 0, stream>>>(kind, (int64_t*)out, params, cdf, node_act, uniforms, seed,
+                                                              vids, s_pids, ncats, num_nodes, node_offset, S);
     else
-        input_sample_kernel<uint8_t><<<grid, 256, 0, stream>>>(kind, (uint8_t*)out, params, node_act, uniforms, vids,
-                                                              s_pids, ncats, num_nodes, node_offset, S);
+        input_sample_kernel<uint8_t><<<grid, 256, 0, stream>>>(kind, (uint8_t*)out, params, cdf, node_act, uniforms, seed,
+                                                              vids, s_pids, ncats, num_nodes, node_offset, S);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }

@@ -153,38 +153,128 @@ __global__ __launch_bounds__(256) void fuse_kernel(PF* __restrict__ pf, int64_t 
     pf[target + j] = s;
 }
 
-// Sum layer top-down step: an active node (nid + i, s) picks child #{k : u >= cumsum_k}.
-__global__ __launch_bounds__(256) void sample_sum_kernel(uint8_t* __restrict__ elem_act, const uint8_t* __restrict__ node_act,
+// Sum layer top-down step.  A workgroup takes one node and 256 samples; if any of them is active
+// (its flag is cleared as it is read), the workgroup stages the node's child parameters in LDS in
+// chunks of kSampleChunk (the reference's BLOCK_K of at most 512, sample.py:115-131), one thread
+// turns a chunk into its FP32 running sums (left to right, from 0), and every active sample
+// binary-searches them: the pick is the first child whose running sum exceeds u; a u at or past
+// the chunk's total moves on to the next chunk with the total subtracted; past the last chunk the
+// last child is taken.  The running sums are the sequential ones, so the picks equal those of a
+// per-sample left-to-right scan.
+constexpr int kSampleChunk = 512;
+
+__device__ __forceinline__ void serial_prefix(float* P, int m) {
+    float c = 0.0f;
+    int k = 0;
+    for (; k + 8 <= m; k += 8) {
+        float w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = P[k + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            c += w[q];
+            P[k + q] = c;
+        }
+    }
+    for (; k < m; ++k) {
+        c += P[k];
+        P[k] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void sample_sum_kernel(uint8_t* __restrict__ elem_act, uint8_t* __restrict__ node_act,
                                                          const float* __restrict__ uniforms,
-                                                         const float* __restrict__ params, const int* __restrict__ nids,
+                                                         const uint32_t* __restrict__ seed,
+                                                         const float* __restrict__ params, const float* __restrict__ cdf,
+                                                         const int* __restrict__ nids,
                                                          const int* __restrict__ cids, const int* __restrict__ pids,
                                                          const int* __restrict__ nch, int K, int bs, int64_t S) {
-    const int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    __shared__ float P[kSampleChunk];
+    const int tid = threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.y * blockDim.x + tid;
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
     const int64_t node = (int64_t)nids[j] + i;
-    if (!node_act[node * S + s]) return;
-    const float u = uniforms[node * S + s];
+    const int64_t o = node * S + s;
+    const bool act = s < S && node_act[o] != 0;
+    if (!__syncthreads_or(act)) return;
+    float u = 0.0f;
+    if (act) {
+        node_act[o] = 0;
+        u = uniforms != nullptr ? uniforms[o] : philox_uniform(seed[0], seed[1], (uint32_t)s, (uint32_t)node);
+    }
     const int n = nch[j];
     const int* crow = cids + (int64_t)j * K;
     const int* prow = pids + (int64_t)j * K;
-    float cum = 0.0f;
-    int pick = n - 1;
-    for (int k = 0; k < n; ++k) {
-        cum += params[prow[k] + i];
-        if (u < cum) { pick = k; break; }
+    int pick = -1;
+    for (int c0 = 0; c0 < n; c0 += kSampleChunk) {
+        const int m = min(kSampleChunk, n - c0);
+        if (cdf != nullptr) {   // running sums precomputed (pjamd_sample_prefix)
+            for (int k = tid; k < m; k += blockDim.x) P[k] = cdf[prow[c0 + k] + i];
+            __syncthreads();
+        } else {
+            for (int k = tid; k < m; k += blockDim.x) P[k] = params[prow[c0 + k] + i];
+            __syncthreads();
+            if (tid == 0) serial_prefix(P, m);
+            __syncthreads();
+        }
+        if (act && pick < 0) {
+            const float tot = P[m - 1];
+            if (u < tot) {
+                int lo = 0, hi = m - 1;        // first k with u < P[k]; P[hi] > u
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (u < P[mid]) hi = mid;
+                    else lo = mid + 1;
+                }
+                pick = c0 + lo;
+            } else {
+                u -= tot;
+            }
+        }
+        if (!__syncthreads_or(act && pick < 0)) break;   // also orders the reads before the next chunk
     }
-    elem_act[(int64_t)crow[pick] * S + s] = 1;
+    if (act) elem_act[(int64_t)crow[pick < 0 ? n - 1 : pick] * S + s] = 1;
 }
 
-// Product layer top-down step: an active element activates all its children.
-__global__ __launch_bounds__(256) void sample_prod_kernel(uint8_t* __restrict__ node_act, const uint8_t* __restrict__ elem_act,
+// Running sums of rows r < n_rows: out[base_r + c * stride_r] = sum_{c' <= c} in[base_r + c' *
+// stride_r] in FP32, left to right, restarting at every multiple of `chunk` (0: never).  One thread
+// per row, kSampleBatch loads in flight.
+constexpr int kPrefixBatch = 16;
+
+__global__ __launch_bounds__(256) void row_prefix_kernel(float* __restrict__ out, const float* __restrict__ in,
+                                                         const int64_t* __restrict__ base, const int* __restrict__ stride,
+                                                         const int* __restrict__ len, int64_t n_rows, int chunk) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const int64_t b0 = base[r];
+    const int st = stride[r], n = len[r];
+    float c = 0.0f;
+    for (int k0 = 0; k0 < n; k0 += kPrefixBatch) {
+        float w[kPrefixBatch];
+#pragma unroll
+        for (int q = 0; q < kPrefixBatch; ++q) w[q] = in[b0 + (int64_t)min(k0 + q, n - 1) * st];
+#pragma unroll
+        for (int q = 0; q < kPrefixBatch; ++q) {
+            const int k = k0 + q;
+            if (k < n) {
+                if (chunk > 0 && k % chunk == 0) c = 0.0f;
+                c += w[q];
+                out[b0 + (int64_t)k * st] = c;
+            }
+        }
+    }
+}
+
+// Product layer top-down step: an active element activates all its children (and is cleared).
+__global__ __launch_bounds__(256) void sample_prod_kernel(uint8_t* __restrict__ node_act, uint8_t* __restrict__ elem_act,
                                                           const int* __restrict__ nids, const int* __restrict__ cids,
                                                           int K, int bs, int64_t S) {
     const int64_t s = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const int j = blockIdx.x / bs, i = blockIdx.x % bs;
-    if (!elem_act[((int64_t)nids[j] + i) * S + s]) return;
+    const int64_t e = ((int64_t)nids[j] + i) * S + s;
+    if (!elem_act[e]) return;
+    elem_act[e] = 0;
     const int* crow = cids + (int64_t)j * K;
     for (int k = 0; k < K; ++k) {
         const int c = crow[k];
@@ -284,18 +374,30 @@ PJAMD_EXPORT int pjamd_pflow_fuse(void* pflows, int pf64, int64_t target, int64_
     return PJAMD_OK;
 }
 
-PJAMD_EXPORT int pjamd_sample_sum(uint8_t* elem_act, const uint8_t* node_act, const float* uniforms, const float* params,
-                                  const int* nids, const int* cids, const int* pids, const int* nch, int n_blocks, int K,
-                                  int bs, int64_t S, hipStream_t stream) {
-    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && S > 0);
+PJAMD_EXPORT int pjamd_sample_sum(uint8_t* elem_act, uint8_t* node_act, const float* uniforms, const uint32_t* seed,
+                                  const float* params, const float* cdf, const int* nids, const int* cids,
+                                  const int* pids, const int* nch, int n_blocks, int K, int bs, int64_t S,
+                                  hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && S > 0 && (uniforms != nullptr || seed != nullptr));
     if (n_blocks == 0) return PJAMD_OK;
     dim3 grid((unsigned)((int64_t)n_blocks * bs), cdiv(S, 256));
-    sample_sum_kernel<<<grid, 256, 0, stream>>>(elem_act, node_act, uniforms, params, nids, cids, pids, nch, K, bs, S);
+    sample_sum_kernel<<<grid, 256, 0, stream>>>(elem_act, node_act, uniforms, seed, params, cdf, nids, cids, pids, nch,
+                                                K, bs, S);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
 
-PJAMD_EXPORT int pjamd_sample_prod(uint8_t* node_act, const uint8_t* elem_act, const int* nids, const int* cids,
+PJAMD_EXPORT int pjamd_sample_prefix(float* out, const float* in, const int64_t* row_base, const int* row_stride,
+                                     const int* row_len, int64_t n_rows, int chunk, hipStream_t stream) {
+    PJAMD_REQUIRE(n_rows >= 0 && chunk >= 0);
+    if (n_rows == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(cdiv(n_rows, 256) < (1u << 31));
+    row_prefix_kernel<<<cdiv(n_rows, 256), 256, 0, stream>>>(out, in, row_base, row_stride, row_len, n_rows, chunk);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_sample_prod(uint8_t* node_act, uint8_t* elem_act, const int* nids, const int* cids,
                                    int n_blocks, int K, int bs, int64_t S, hipStream_t stream) {
     PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && S > 0);
     if (n_blocks == 0) return PJAMD_OK;

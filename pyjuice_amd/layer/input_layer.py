@@ -394,12 +394,19 @@ class InputLayer(Layer, nn.Module):
                    float(step_size), float(pseudocount), 1 if keep_zero_params else 0, N.stream(),
                    algo=(8 * self.num_parameters + self.param_flows.element_size() * self.num_param_flows, 0))
 
-    def sample(self, samples: torch.Tensor, node_active: torch.Tensor, uniforms: torch.Tensor, **kwargs):
-        """Draw ``samples[v_i, s]`` for every active input node ``i`` with uniform ``u[off+i, s]``:
-        Categorical picks the first category whose running sum reaches ``u``
-        (``categorical.py:137-161``), Bernoulli returns ``u < theta`` (``bernoulli.py:79-88``)."""
+    def sample(self, samples: torch.Tensor, node_active: torch.Tensor, uniforms: Optional[torch.Tensor],
+               seed: Optional[torch.Tensor] = None, cdf: Optional[torch.Tensor] = None, **kwargs):
+        """Draw ``samples[v_i, s]`` for every active input node ``i`` (clearing its flag) with
+        uniform ``uniforms[off + i, s]``, or the counter-based uniform of (off + i, s) under the
+        device key ``seed`` when ``uniforms`` is None: Categorical picks the first category whose
+        running sum reaches ``u`` (``categorical.py:137-161``), Bernoulli returns ``u < theta``
+        (``bernoulli.py:79-88``).  ``cdf``: the parameters' running sums (``pjamd_sample_prefix``)."""
         t = self._dev
         S = samples.size(1)
-        N.call("pjamd_input_sample", self._kind(), N.ptr(samples), N.ptr(self.params), N.ptr(node_active),
-               N.ptr(uniforms), N.ptr(t["vids"]), N.ptr(t["s_pids"]), N.ptr(t["ncats"]),
-               1 if samples.dtype == torch.int64 else 0, self.num_nodes, self._output_ind_range[0], S, N.stream())
+        assert uniforms is not None or seed is not None
+        N.call("pjamd_input_sample", self._kind(), N.ptr(samples), N.ptr(self.params),
+               N.ptr(cdf) if cdf is not None else None, N.ptr(node_active),
+               N.ptr(uniforms) if uniforms is not None else None, N.ptr(seed) if seed is not None else None,
+               N.ptr(t["vids"]), N.ptr(t["s_pids"]), N.ptr(t["ncats"]),
+               1 if samples.dtype == torch.int64 else 0, self.num_nodes, self._output_ind_range[0], S, N.stream(),
+               algo=(self.num_nodes * S, 0))

@@ -335,3 +335,20 @@ def test_config5_pruned_hclt512_bs32_b2048(device):
     _weighted_columns_parity(pc, x, _spread_columns(2048), "bf16", 2e-2, 2e-2)
     # and at batch 1100 (the 8-node x 1024-column form's range ends at 1024)
     _weighted_columns_parity(pc, x[:1100], _spread_columns(1100, seed=1), "bf16", 2e-2, 2e-2)
+
+
+def test_headline_sampling_bit_exact(device):
+    """Ancestral sampling at the headline size (HCLT-784, 256 latents, 512 samples), in-kernel
+    counter-based uniforms, the pass recorded as a HIP graph and replayed under a second seed:
+    every sample bit-identical to the oracle drawing the same uniforms."""
+    from oracle.dag_eval import philox_uniforms
+    U.say("[headline sampling] HCLT-784, 256 latents, 512 samples")
+    ns = _hclt784(device, 256)
+    pc = juice.compile(ns)
+    pc.to(device)
+    pc.update_parameters()
+    S = 512
+    for seed, rec in ((123, True), (456, False)):
+        got = juice.queries.sample(pc, num_samples=S, seed=seed, record_cudagraph=rec).cpu().numpy().astype(np.int64)
+        ref = dag_eval.sample(ns, (lambda r, c, sd=seed: philox_uniforms(sd, r, c), S), pc.num_vars)
+        assert np.array_equal(got, ref), (seed, int((got != ref).sum()))

@@ -156,10 +156,17 @@ def test_forward_backward_em(name, B, precision, device):
         assert np.abs(got - p).max() <= t_em, (s, np.abs(got - p).max())
 
 
-@pytest.mark.parametrize("name", ["simple_model", "hclt_bern16", "hclt_cat_bs32", "pd_small", "hmm_small"])
+@pytest.mark.parametrize("name", ["simple_model", "hclt_bern16", "hclt_cat_bs32", "pd_small", "hmm_small",
+                                  "wide_sums32", "hmm_wide"])
 def test_sampling_bit_exact(name, device):
+    """Caller uniforms: samples bit-identical to the oracle's (wide_sums32: 640 children, and
+    hmm_wide: 1024 children per sum node, scanned in two 512-child chunks)."""
     U.say(f"[sample] {name}")
-    ns, cards, kw = _circuit(name)
+    if name == "hmm_wide":
+        ns, cards = U.hmm_small(T=3, L=1024, V=8)
+        kw = dict()
+    else:
+        ns, cards, kw = _circuit(name)
     pc = TensorCircuit(ns, **kw)
     pc.to(device)
     S = 200
@@ -168,6 +175,33 @@ def test_sampling_bit_exact(name, device):
     pc.update_parameters()
     ref = dag_eval.sample(ns, u.numpy(), pc.num_vars)
     assert np.array_equal(smp.astype(np.int64), ref), np.argwhere(smp != ref)[:5]
+
+
+@pytest.mark.parametrize("name", ["simple_model", "hclt_bern16", "hclt_cat_bs64", "pd_sparse", "hmm_small", "wide_sums32"])
+def test_sampling_seeded_and_graph_replays(name, device):
+    """In-kernel Philox uniforms: bit-identical to the oracle drawing the same counter-based
+    uniforms; a recorded graph replayed under new seeds, with eager passes in between, gives each
+    seed's samples."""
+    import pyjuice_amd as juice
+    from oracle.dag_eval import philox_uniforms
+    U.say(f"[sample seeded] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, **kw)
+    pc.to(device)
+    pc.update_parameters()
+    S = 300
+    ref = lambda seed: dag_eval.sample(ns, (lambda r, c: philox_uniforms(seed, r, c), S), pc.num_vars)
+    for seed in (5, (1 << 40) + 17):
+        got = juice.queries.sample(pc, num_samples=S, seed=seed).cpu().numpy().astype(np.int64)
+        assert np.array_equal(got, ref(seed))
+    first = juice.queries.sample(pc, num_samples=S, seed=21, record_cudagraph=True).cpu().numpy().astype(np.int64)
+    assert np.array_equal(first, ref(21))
+    for seed in (22, 23):
+        got = juice.queries.sample(pc, num_samples=S, seed=seed).cpu().numpy().astype(np.int64)   # replays
+        assert np.array_equal(got, ref(seed)), seed
+        eager = juice.queries.sample(pc, num_samples=S, seed=seed + 100, apply_cudagraph=False)
+        assert np.array_equal(eager.cpu().numpy().astype(np.int64), ref(seed + 100))
+    assert int(pc._sample_buffers["node_act"].sum()) == 0 and int(pc._sample_buffers["elem_act"].sum()) == 0
 
 
 def pyjuice_sample(pc, u):

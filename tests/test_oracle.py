@@ -207,3 +207,35 @@ def test_sampling_matches_enumeration():
     freq = np.bincount(idx, minlength=16) / S
     sd = np.sqrt(p * (1 - p) / S)
     assert np.all(np.abs(freq - p) < 5 * sd + 1e-4), np.abs(freq - p) / (sd + 1e-12)
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 against the generator's published known-answer vectors (Random123)."""
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+             ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+             ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+              (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in cases:
+        got = dag_eval.philox4x32_10([np.array([c]) for c in ctr], key)
+        assert tuple(int(g[0]) for g in got) == want
+    u = dag_eval.philox_uniforms(7, np.arange(1000) % 13, np.arange(1000))
+    assert u.dtype == np.float32 and (u >= 0).all() and (u < 1).all()
+    assert np.all(u * 2 ** 24 == np.floor(u * 2 ** 24))      # 24-bit fractions, exact in FP32
+    # uniform: mean and variance of 200k draws
+    v = dag_eval.philox_uniforms(11, np.arange(200000) // 512, np.arange(200000) % 512).astype(np.float64)
+    assert abs(v.mean() - 0.5) < 4e-3 and abs(v.var() - 1 / 12) < 4e-3
+
+
+def test_chunked_pick_matches_single_scan_in_exact_arithmetic():
+    """With weights and uniforms that are multiples of 2^-12 (exact FP32 sums), subtracting a
+    chunk's total and scanning on is the same as one scan over all children, for any chunk size;
+    a u at or past the total takes the last child; chunk sizes >= n are a single scan."""
+    g = np.random.default_rng(3)
+    for n in (1, 7, 100, 700, 1300):
+        w = g.integers(0, 5, size=n).astype(np.float32) / 4096
+        w[g.integers(0, n)] += 1 / 4096
+        tot = float(w.astype(np.float64).sum())
+        u = (g.integers(0, int(tot * 4096) + 3, size=300) / 4096).astype(np.float32)
+        ref = np.minimum(np.searchsorted(np.cumsum(w.astype(np.float64)), u.astype(np.float64), side="right"), n - 1)
+        for chunk in (1, 3, 64, 512, 4096):
+            assert np.array_equal(dag_eval._chunked_pick(w, u, chunk), ref), (n, chunk)
