@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch each step's kernels from the host instead of replaying the step's HIP graph")
+    ap.add_argument("--no-tune-launches", dest="tune_launches", action="store_false",
+                    help="keep the library's default launch form of every sum layer instead of the measured "
+                         "choice (pyjuice_amd.autotune.tune_launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=32)
     ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
@@ -391,12 +394,22 @@ def main():
             # graph runs one eager step)
             with torch.no_grad():
                 lls_by_prec[prec] = pc(xb, precision=prec).detach().cpu().numpy()[:, 0]
+            forms = None
+            if args.tune_launches:
+                from pyjuice_amd import autotune
+                t0 = time.time()
+                forms = autotune.tune_launches(pc, xb, prec)
+                pc.params.data.copy_(state["params"])
+                for l, p in zip(pc.input_layer_group, in_state):
+                    l.params.data.copy_(p)
+                log(f"[bench] {prec}: launch forms {forms['fw']} / {forms['bk']} in {time.time() - t0:.1f}s")
             step = make_step(pc, xb, prec, args.graph)
             ms, wall = time_steps(step, args.steps, args.warmup)
         except Exception as e:  # an unsupported precision is reported, not fatal
             log(f"[bench] {prec}: {e}")
             continue
-        results[prec] = {"ms_per_step": ms, "wall_ms_per_step": wall, "samples_per_s": args.batch / (ms * 1e-3)}
+        results[prec] = {"ms_per_step": ms, "wall_ms_per_step": wall, "samples_per_s": args.batch / (ms * 1e-3),
+                         "launch_forms": forms}
         log(f"[bench] {prec}: {ms:.3f} ms/step ({args.batch / (ms * 1e-3):.0f} samples/s)")
         # per-launch HIP-event timing needs eager launches (a graph replay has no host calls)
         results[prec]["kernels"] = kernel_breakdown(make_step(pc, xb, prec, False) if args.graph else step,
@@ -443,6 +456,7 @@ def main():
         "cpu_baseline": cpu,
         "precisions": {p: {"samples_per_s": round(v["samples_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 4)}
                        for p, v in results.items()},
+        "launch_forms": r.get("launch_forms"),
         "dll_per_var_vs_fp64": acc,
         "sampling": sampling,
         "kernels_ms_per_step": {k: {"ms": round(v["ms"], 4), "launches": v["launches"],

@@ -203,6 +203,25 @@ int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* 
                       const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs, int64_t B,
                       int child_run, int precision, pjamd_stream_t stream);
 
+/* Launch hints (no PyJuice counterpart), or-ed into pjamd_sum_forward's child_run and the flags
+ * of pjamd_sum_backward_ele_to_children: force the preloaded form (the first two reduction stages
+ * in flight together) or the plain form of the 16-byte-load kernels instead of the default choice
+ * (preloaded below 256 workgroups).  Set per layer by the launch autotuner. */
+#define PJAMD_LAUNCH_PRE (1 << 28)
+#define PJAMD_LAUNCH_NOPRE (1 << 27)
+
+/* pjamd_sum_forward as one persistent launch: min(tiles, resident workgroups) workgroups take the
+ * (node block, row tile, batch tile) items in order from the counter queue[0] until none is left,
+ * so node blocks of uneven fan-in (pruned circuits; the merged table lists the heaviest first)
+ * need no dispatch of their own; the last workgroup to finish resets queue[0..1] to zero.  queue:
+ * two int32 in device memory, zero before the first call; calls sharing a queue must not overlap.
+ * BF16 / FP16 operands, aligned parameter rows with runs >= 16, B % 4 == 0, bs 32 or a multiple of
+ * 64, else PJAMD_EUNSUPPORTED.  Same results as pjamd_sum_forward.  Replaces SumLayer.forward's
+ * per-partition launches (src/pyjuice/layer/sum_layer.py:220-262). */
+int pjamd_sum_forward_queue(float* node_mars, const float* element_mars, const float* params, const int* nids,
+                            const int* cids, const int* pids, const int* n_chs, int n_blocks, int K, int bs,
+                            int64_t B, int child_run, int* queue, int precision, pjamd_stream_t stream);
+
 /* pjamd_sum_forward with each node block's reduction split over n_splits workgroups (BF16 / FP16
  * operands: precision 1 or 2), for layers of few tiles and long child lists: split s takes an
  * equal share of the 128-child stages and stores its partial sum and column max into workspace,

@@ -59,6 +59,8 @@ _SIGNATURES = {
     "pjamd_sum_forward_split": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _i32,
                                 _c_p, _i32, _c_p],
     "pjamd_flows_to_log": [_c_p, _i64, _c_p],
+    "pjamd_sum_forward_queue": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,
+                                _c_p],
     "pjamd_sum_forward_split_noalloc": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32,
                                         _i32, _i32, _c_p],
     "pjamd_sum_forward_prod": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p, _i32,

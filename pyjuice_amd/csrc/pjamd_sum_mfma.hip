@@ -1110,12 +1110,12 @@ __global__ __launch_bounds__(256) void sum_ele_split_combine(float* __restrict__
 // child group kg of KPG = TM / 8 consecutive children.
 constexpr int kFwdIdx = 512;
 
-template <typename T, int TM>
+template <typename T, int TM, bool QUEUE>
 __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ node_mars, const float* __restrict__ emars,
                                                           const float* __restrict__ params, const int* __restrict__ nids,
                                                           const int* __restrict__ cids, const int* __restrict__ pids,
                                                           const int* __restrict__ nch, int K, int64_t B, int mtiles,
-                                                          int btiles) {
+                                                          int btiles, int* __restrict__ queue, int n_items) {
     constexpr int TN = 4096 / TM, WM = TM / 32;
     constexpr int QN = TN / 4, GR = 256 / QN, RPG = TK / GR;
     constexpr int QM = TM / 4, KPG = TM / 8;
@@ -1125,8 +1125,21 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ no
     __shared__ __attribute__((aligned(16))) T Bs[TN][LD];
     __shared__ float red[GR][TN], mrun[TN], msc[TN];
     __shared__ int sc[kFwdIdx], sp[kFwdIdx];
+    __shared__ int s_item;
 
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    for (int it = 0;; ++it) {
+    int t;
+    if constexpr (QUEUE) {
+        // work queue: the next tile in dispatch order (heaviest node blocks first)
+        __syncthreads();                 // the previous tile is done with the shared arrays
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        t = s_item;
+        if (t >= n_items) break;
+    } else {
+        if (it > 0) break;
+        t = xcd_tile(blockIdx.x, gridDim.x);
+    }
     const int m0 = (t % mtiles) * TM;
     const int j = t / (mtiles * btiles);
     const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
@@ -1252,6 +1265,17 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ no
         for (int r = 0; r < 16; ++r) {
             const float v = acc[r];
             node_mars[(base + acc_row(r, lane)) * B + b] = (v > 0.0f && m != kNegInf) ? __logf(v) + m : kNegInf;
+        }
+    }
+    }
+    if constexpr (QUEUE) {
+        // the last workgroup to leave resets the queue for the next launch
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(queue + 1, 1) == (int)gridDim.x - 1) {
+                atomicExch(queue, 0);
+                atomicExch(queue + 1, 0);
+            }
         }
     }
 }
@@ -1717,6 +1741,8 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles));
+    const int hint = child_run & (PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
+    child_run &= ~(PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
     const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
     child_run &= ~PJAMD_PARAMS_ALIGNED;
     const bool run = child_run >= TK && child_run % TK == 0;
@@ -1725,17 +1751,19 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
         // node blocks of 32 rows, or child runs shorter than a stage: block-spanning form
         if (aligned && child_run >= 16 && B % 4 == 0 && (bs == 32 || (bs >= 64 && !run))) {
             if (bs == 32)
-                sum_fwd_vec_blocks<T, 32><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids,
-                                                                               pids, nch, K, B, mtiles, (int)btiles);
+                sum_fwd_vec_blocks<T, 32, false><<<(unsigned)tiles, 256, 0, stream>>>(
+                    node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles, (int)btiles, nullptr, 0);
             else
-                sum_fwd_vec_blocks<T, 64><<<(unsigned)tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids,
-                                                                               pids, nch, K, B, mtiles, (int)btiles);
+                sum_fwd_vec_blocks<T, 64, false><<<(unsigned)tiles, 256, 0, stream>>>(
+                    node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles, (int)btiles, nullptr, 0);
             PJAMD_LAUNCH_CHECK();
             return PJAMD_OK;
         }
         if (run && aligned && bs >= 64 && B % 4 == 0) {
             vec = true;
-            if (tiles < 256)
+            // two stages in flight for grids below the chip's 256 CUs unless the caller says otherwise
+            const bool pre = (hint & PJAMD_LAUNCH_PRE) ? true : (hint & PJAMD_LAUNCH_NOPRE) ? false : tiles < 256;
+            if (pre)
                 sum_fwd_vec<T, true, 0><<<(unsigned)tiles, 256, 0, stream>>>(
                     node_mars, emars, params, nids, cids, pids, nch, K, B, mtiles, (int)btiles, nullptr, nullptr, 0,
                     nullptr);
@@ -1870,7 +1898,7 @@ template <typename T>
 int launch_ele(float* eflows, const float* nflows, const float* nmars, const float* emars, const float* params,
                const int* chids, const int* parids, const int* parpids, const int* npar, int n, int KP, int cbs, int bs,
                int64_t B, bool mod, bool accum, float* child_flows, const int* ch_nodes, int Kc, bool mod_children,
-               bool aligned, hipStream_t stream) {
+               bool aligned, hipStream_t stream, int hint = 0) {
     const int lbs = log2_exact(bs);
     const int TM = cbs >= 64 ? 64 : 32;
     const int TN = cbs >= 64 ? 64 : 128;
@@ -1904,7 +1932,7 @@ int launch_ele(float* eflows, const float* nflows, const float* nmars, const flo
     sum_ele_vec<T, F_, P_><<<g, 256, 0, stream>>>(eflows, nflows, nmars, emars, params, chids, parids, parpids, npar, \
                                                   KP, lbs, B, mtiles, bt, mod, accum, child_flows, ch_nodes, Kc,  \
                                                   mod_children)
-            const bool pre = g < 256;
+            const bool pre = (hint & PJAMD_LAUNCH_PRE) ? true : (hint & PJAMD_LAUNCH_NOPRE) ? false : g < 256;
             if (ch_nodes != nullptr) {
                 if (pre && Kc <= kPreChildren) PJAMD_ELE_VEC(2, true);
                 else if (pre) PJAMD_ELE_VEC(1, true);
@@ -2091,15 +2119,18 @@ PJAMD_EXPORT int pjamd_sum_backward_ele_to_children(float* node_flows, const flo
     if (n_cblocks == 0) return PJAMD_OK;
     if (cbs < 32) return PJAMD_EUNSUPPORTED;
     const bool mod = flags & 1, modc = (flags & 2) != 0, aligned = (flags & 4) != 0;
+    const int hint = flags & (PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
     if (precision == 1)
         return launch_ele<__bf16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
-                                  n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream);
+                                  n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream,
+                                  hint);
     if (precision == 2)
         return launch_ele<_Float16>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids,
                                     n_pars, n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc,
-                                    aligned, stream);
+                                    aligned, stream, hint);
     return launch_ele<float>(nullptr, node_flows, node_mars, element_mars, params, chids, parids, parpids, n_pars,
-                             n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream);
+                             n_cblocks, KP, cbs, bs, B, mod, false, node_flows, ch_nodes, Kc, modc, aligned, stream,
+                             hint);
 }
 
 // FP16 operands can overflow in the parameter-flow product (exp(e + s) is unbounded above), so
@@ -2160,6 +2191,58 @@ PJAMD_EXPORT int pjamd_sum_backward_par_batched(void* param_flows, const float* 
     return launch_par_batched<__bf16, float>((float*)param_flows, node_flows, node_mars, element_mars, params, nids,
                                              cids, pids, pfids, nch, desc, tile_desc, n_tiles, tile_rows, B, mod,
                                              overwrite, sign, stream);
+}
+
+PJAMD_EXPORT int pjamd_sum_forward_queue(float* node_mars, const float* element_mars, const float* params,
+                                        const int* nids, const int* cids, const int* pids, const int* n_chs,
+                                        int n_blocks, int K, int bs, int64_t B, int child_run, int* queue,
+                                        int precision, hipStream_t stream) {
+    PJAMD_REQUIRE(n_blocks >= 0 && K >= 1 && bs >= 1 && B > 0 && queue != nullptr && child_run >= 0);
+    if (n_blocks == 0) return PJAMD_OK;
+    const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
+    const int run = child_run & ~(PJAMD_PARAMS_ALIGNED | PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
+    if (!((precision == 1 || precision == 2) && aligned && run >= 16 && run % 16 == 0 && B % 4 == 0 &&
+          (bs == 32 || (bs >= 64 && bs % 64 == 0))))
+        return PJAMD_EUNSUPPORTED;
+    const int TM = bs == 32 ? 32 : 64, TN = 4096 / TM;
+    const int mtiles = bs / TM;
+    const int64_t btiles = cdiv(B, TN);
+    const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
+    PJAMD_REQUIRE(grid_fits(tiles));
+    // as many workgroups as the chip keeps resident (occupancy of the kernel times the CUs)
+    static int resident[2][2] = {{0, 0}, {0, 0}};
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return (int)e;
+    int& occ = resident[precision - 1][TM == 32 ? 0 : 1];
+    if (occ == 0) {
+        int o = 0;
+        if (precision == 1)
+            e = TM == 32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, sum_fwd_vec_blocks<__bf16, 32, true>, 256, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, sum_fwd_vec_blocks<__bf16, 64, true>, 256, 0);
+        else
+            e = TM == 32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, sum_fwd_vec_blocks<_Float16, 32, true>, 256, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, sum_fwd_vec_blocks<_Float16, 64, true>, 256, 0);
+        if (e != hipSuccess) return (int)e;
+        occ = o > 0 ? o : 1;
+    }
+    const int64_t workers = (int64_t)occ * cus;
+    const unsigned grid = (unsigned)(tiles < workers ? tiles : workers);
+#define PJAMD_FWD_Q(T_, TM_)                                                                                         \
+    sum_fwd_vec_blocks<T_, TM_, true><<<grid, 256, 0, stream>>>(node_mars, element_mars, params, nids, cids, pids, \
+                                                                n_chs, K, B, mtiles, (int)btiles, queue, (int)tiles)
+    if (precision == 1) {
+        if (TM == 32) PJAMD_FWD_Q(__bf16, 32);
+        else PJAMD_FWD_Q(__bf16, 64);
+    } else {
+        if (TM == 32) PJAMD_FWD_Q(_Float16, 32);
+        else PJAMD_FWD_Q(_Float16, 64);
+    }
+#undef PJAMD_FWD_Q
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -28,6 +28,9 @@ def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
 
 
 ALG_NEGATE = 16   # PJAMD_ALG_NEGATE: parameter flows of MPE / GeneralLL subtracted
+LAUNCH_PRE = 1 << 28     # PJAMD_LAUNCH_PRE
+LAUNCH_NOPRE = 1 << 27   # PJAMD_LAUNCH_NOPRE
+_HINT_BITS = {"pre": LAUNCH_PRE, "nopre": LAUNCH_NOPRE}
 PARAMS_ALIGNED = 1 << 30   # PJAMD_PARAMS_ALIGNED
 
 
@@ -209,6 +212,8 @@ class SumLayer(Layer, nn.Module):
         decreasing fan-in so the longest tiles are dispatched first."""
         device = self.device
         self.__dict__.pop("_split_cache", None)
+        # work-queue counters of pjamd_sum_forward_queue (zero between launches)
+        self._queue = torch.zeros([2], dtype=torch.int32, device=device)
         fw, fw_run, fw_al = [], [], []
         for p in range(self.num_fw_partitions):
             n = self.partitioned_nids[p]
@@ -264,10 +269,15 @@ class SumLayer(Layer, nn.Module):
         B = node_mars.size(1)
         bs = self.block_size
         fused = prod_tables.get(id(self)) if prod_tables is not None else None
+        # launch choice of this layer (set by pyjuice_amd.autotune.tune_launches): "pre" / "nopre"
+        # force a form of the 16-byte-load kernels, "queue" runs the persistent work-queue kernel,
+        # "nosplit" turns the split reduction off, "split<N>" sets its split count
+        fh = (self.__dict__.get("launch_hints") or {}).get("fw")
         for e, ((nids, cids, pids, pfids, nch, tot), run, al) in enumerate(zip(self._dev["fw"], self._fw_child_run,
                                                                                self._fw_aligned)):
             run = run | (PARAMS_ALIGNED if al else 0)
-            split = self._split_plan(e, nids.numel(), nch, run & ~PARAMS_ALIGNED, al, prec, B)
+            split = None if fh == "nosplit" else self._split_plan(e, nids.numel(), nch, run & ~PARAMS_ALIGNED, al,
+                                                                     prec, B, self._forced_splits(fh))
             if split is not None and fused is None:
                 nsplit, ws = split
                 algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
@@ -286,45 +296,66 @@ class SumLayer(Layer, nn.Module):
             # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
             # per sample, parameters read once
             algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
+            if fh == "queue" and self.queue_ok(run, prec, B):
+                N.call("pjamd_sum_forward_queue", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
+                       N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run,
+                       N.ptr(self._queue), prec, N.stream(), algo=algo)
+                continue
             N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
-                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, prec,
-                   N.stream(), algo=algo)
+                   N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B,
+                   run | _HINT_BITS.get(fh, 0), prec, N.stream(), algo=algo)
 
-    def _split_plan(self, e: int, n_blocks: int, nch: torch.Tensor, run: int, aligned: bool, prec: int, B: int):
+    def queue_ok(self, run: int, prec: int, B: int) -> bool:
+        """Whether ``pjamd_sum_forward_queue`` takes a forward table with child-run flags ``run``."""
+        r = run & ~PARAMS_ALIGNED
+        bs = self.block_size
+        return (prec in (1, 2) and bool(run & PARAMS_ALIGNED) and r >= 16 and r % 16 == 0 and B % 4 == 0
+                and (bs == 32 or (bs >= 64 and bs % 64 == 0)))
+
+    @staticmethod
+    def _forced_splits(hint: Optional[str]) -> Optional[int]:
+        """The split count a launch hint "split<N>" asks for (None for other hints)."""
+        return int(hint[5:]) if isinstance(hint, str) and hint.startswith("split") else None
+
+    def _split_plan(self, e: int, n_blocks: int, nch: torch.Tensor, run: int, aligned: bool, prec: int, B: int,
+                    force: Optional[int] = None):
         """``(n_splits, workspace)`` for forward table ``e`` at batch ``B`` when the split reduction
-        applies, else None; the workspace is cached per (table, batch)."""
+        applies, else None; the workspace is cached per (table, batch).  ``force``: this split count
+        (2 .. 16, at most one per stage) instead of the default rule's."""
         bs = self.block_size
         if not (prec in (1, 2) and aligned and run >= 128 and run % 128 == 0 and bs % 64 == 0 and B % 4 == 0):
             return None
         tiles = n_blocks * (bs // 64) * -(-B // 64)
-        key = (e, B)
+        key = (e, B, force)
         cache = self.__dict__.setdefault("_split_cache", dict())
         if key not in cache:
             nst = -(-int(nch.max()) // 128)
             plan = None
             if tiles < self.split_max_tiles and nst >= self.split_min_stages and n_blocks * bs < 65536:
-                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16))
+                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16)) if force is None else min(force, nst)
                 rows = n_blocks * bs
                 ws = _split_workspace(nsplit * rows * B + nsplit * (rows // 64) * B, nch.device)
                 plan = (nsplit, ws)
             cache[key] = plan
         return cache[key]
 
-    def _split_plan_bk(self, q: int, n_cblocks: int, npar: torch.Tensor, cbs: int, aligned: bool, prec: int, B: int):
+    def _split_plan_bk(self, q: int, n_cblocks: int, npar: torch.Tensor, cbs: int, aligned: bool, prec: int, B: int,
+                       force: Optional[int] = None):
         """``(n_splits, workspace)`` for the fused element-flow pass of backward table ``q`` at
-        batch ``B`` when the split reduction applies (parent blocks of >= 128 rows), else None."""
+        batch ``B`` when the split reduction applies (parent blocks of >= 128 rows), else None;
+        ``force`` as for ``_split_plan``."""
         bs = self.block_size
         if not (prec in (1, 2) and aligned and bs >= 128 and cbs % 64 == 0 and B % 4 == 0):
             return None
         tiles = n_cblocks * (cbs // 64) * -(-B // 64)
-        key = ("bk", q, B)
+        key = ("bk", q, B, force)
         cache = self.__dict__.setdefault("_split_cache", dict())
         if key not in cache:
             nst = int(npar.max()) * bs // 128
             plan = None
             rows = n_cblocks * cbs
             if tiles < self.split_max_tiles and nst >= self.split_min_stages and rows < 65536:
-                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16))
+                nsplit = max(2, min(nst // 2, 256 // max(tiles, 1), 16)) if force is None else min(force, nst)
                 ws = _split_workspace(nsplit * rows * B + nsplit * (rows // 64) * B, npar.device)
                 plan = (nsplit, ws)
             cache[key] = plan
@@ -414,7 +445,9 @@ class SumLayer(Layer, nn.Module):
             al = 4 if self._bk_aligned[q] else 0
             if child_tables is not None:
                 tab, n_blk, n_mod = child_tables[q]
-                split = self._split_plan_bk(q, chids.numel(), npar, cbs, bool(al), prec, B)
+                bk_choice = (self.__dict__.get("launch_hints") or {}).get("bk")
+                split = None if bk_choice == "nosplit" else self._split_plan_bk(
+                    q, chids.numel(), npar, cbs, bool(al), prec, B, self._forced_splits(bk_choice))
                 # parent rows (and marginals unless modified) + element rows read, one write per
                 # child row (+ its marginal when stored as log(flow) - mar), parameters once
                 n_rd = n_mod if mod else 0
@@ -427,10 +460,11 @@ class SumLayer(Layer, nn.Module):
                            tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
                            mod | (2 if allow_modify_flows else 0) | al, nsplit, N.ptr(ws), prec, s, algo=algo)
                     continue
+                bh = _HINT_BITS.get((self.__dict__.get("launch_hints") or {}).get("bk"), 0)
                 N.call("pjamd_sum_backward_ele_to_children", N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids),
                        N.ptr(npar), N.ptr(tab), tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
-                       mod | (2 if allow_modify_flows else 0) | al, prec, s, algo=algo)
+                       mod | (2 if allow_modify_flows else 0) | al | bh, prec, s, algo=algo)
                 continue
             algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
                     2 * totp * bs * cbs * B)

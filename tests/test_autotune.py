@@ -52,3 +52,65 @@ def test_margin_keeps_first_candidate(monkeypatch):
     monkeypatch.setattr(torch.cuda, "empty_cache", lambda: None)
     best, ms = autotune.tune_block_size(lambda bs: None, torch.zeros(2, 2), [256, 128, 64])
     assert ms == {256: 10.0, 128: 9.8, 64: 9.0} and best == 64      # 9.0 < 0.97 * 10.0; 9.8 is not
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["hclt_pruned", "hclt_cat_bs128", "hmm_split"])
+def test_launch_forms_agree_and_tuner_picks_one(name):
+    """Every launch form the tuner may pick gives the default form's results: the preloaded and
+    plain 16-byte kernels and the persistent work-queue forward are the same arithmetic (bit-
+    identical), the split reduction rounds 16-bit operands per split (within the 16-bit
+    tolerance); after tune_launches the forward still matches the FP64 oracle."""
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import numpy as np
+    from oracle import dag_eval
+    from pyjuice_amd import autotune
+    from pyjuice_amd.model import TensorCircuit
+    U.say(f"[launch forms] {name}")
+    dev = torch.device("cuda:0")
+    if name == "hclt_pruned":
+        ns, _, cards = U.hclt_pruned(num_vars=12, L=128, block_size=32)
+    elif name == "hclt_cat_bs128":
+        ns, _, cards = U.hclt_categorical(num_vars=8, L=256, K=10, block_size=128)
+    else:
+        ns, cards = U.hmm_small(T=4, L=512, V=16)
+    pc = TensorCircuit(ns, precision="bf16")
+    pc.to(dev)
+    X = U.random_data(cards, 256, seed=5)
+    xt = torch.from_numpy(X).to(dev)
+    with torch.no_grad():
+        pc(xt)
+        base_m = pc.node_mars.clone()
+        pc.backward(xt, flows_memory=0.0)
+        base_f = pc.node_flows.clone()
+        base_pf = pc.param_flows.clone()
+    tried = set()
+    for g in pc.inner_layer_groups:
+        if not g.is_sum():
+            continue
+        for layer in g:
+            for pas, cands in (("fw", autotune._fw_candidates(layer, 256, "bf16")),
+                               ("bk", autotune._bk_candidates(layer, 256, "bf16", id(layer) in pc._fused["tables"]))):
+                for c in cands[1:]:
+                    tried.add((pas, c))
+                    layer.launch_hints = {"fw": None, "bk": None, pas: c}
+                    with torch.no_grad():
+                        pc(xt)
+                        pc.backward(xt, flows_memory=0.0)
+                    if c == "nosplit" or c.startswith("split"):
+                        assert (pc.node_mars - base_m).abs().max().item() <= 4e-3 * len(cards)
+                        assert U.rel_err(pc.param_flows.cpu().numpy(), base_pf.cpu().numpy()) <= 2e-2
+                    else:
+                        assert torch.equal(pc.node_mars, base_m), (pas, c)
+                        assert torch.equal(pc.node_flows, base_f), (pas, c)
+                        assert torch.equal(pc.param_flows, base_pf), (pas, c)
+                layer.launch_hints = {"fw": None, "bk": None}
+    want = {"hclt_pruned": ("fw", "queue"), "hclt_cat_bs128": ("fw", "pre"), "hmm_split": ("fw", "nosplit")}[name]
+    assert want in tried, tried
+    summary = autotune.tune_launches(pc, xt, "bf16", reps=3)
+    assert sum(summary["fw"].values()) == sum(1 for g in pc.inner_layer_groups if g.is_sum() for _ in g)
+    lls = pc(xt).detach().cpu().numpy()[:, 0]
+    pc.update_parameters()
+    ref = dag_eval.forward(ns, X)[ns][0]
+    assert np.abs(lls - ref).max() <= 1e-3 * len(cards)
