@@ -12,8 +12,8 @@ kept.  The reference fixes ``block_size = num_latents`` for HCLT unless the call
 Per layer (``tune_launches``), the launch form of each sum layer's forward and fused element-flow
 pass is measured on the circuit's own data: the 16-byte-load kernels with the first two reduction
 stages preloaded or not (the default switches at 256 workgroups), the persistent work-queue
-forward (``pjamd_sum_forward_queue``) for uneven node blocks, and the split reduction off or with
-2, 4, 8 or 16 splits.
+forward (``pjamd_sum_forward_queue``) for uneven node blocks, the split reduction off or with
+2, 4, 8 or 16 splits, and for FP32 operands the matrix-core or the vector-ALU tiles.
 The reference tunes nothing at run time; its tile sizes are fixed per launch
 (``layer/sum_layer.py:832-840``).  Choices are cached per layer shape, batch and precision.
 """
@@ -92,6 +92,8 @@ def _fw_candidates(layer, B: int, prec: str) -> List[Optional[str]]:
     from pyjuice_amd.layer.sum_layer import PARAMS_ALIGNED, PRECISIONS
     p = PRECISIONS[prec]
     out: List[Optional[str]] = [None]
+    if p == 0 and layer.block_size >= 32:
+        return out + ["valu"]          # FP32 operands: matrix cores or vector ALUs
     if p not in (1, 2) or B % 4 != 0 or layer.block_size < 32:
         return out
     bs = layer.block_size
@@ -112,6 +114,8 @@ def _bk_candidates(layer, B: int, prec: str, fused: bool) -> List[Optional[str]]
     from pyjuice_amd.layer.sum_layer import PRECISIONS
     p = PRECISIONS[prec]
     out: List[Optional[str]] = [None]
+    if p == 0 and not fused and all(t[4] >= 32 for t in layer._dev["bk"]):
+        return out + ["valu"]          # FP32 operands, unfused element flows: matrix cores or vector ALUs
     if not fused or p not in (1, 2) or B % 4 != 0 or layer.block_size < 16:
         return out
     if any(al and cbs >= 64 and layer.block_size >= 128 for (_c, _pa, _pp, _k, cbs, _t), al in
@@ -182,6 +186,11 @@ def tune_launches(pc, x: torch.Tensor, precision: Optional[str] = None, reps: in
                         continue
                     if pas == "fw":
                         run = lambda: layer(pc.node_mars, slab, pc.params, precision=prec)
+                    elif id(layer) not in fz["tables"]:
+                        # unfused: element flows of the layer's children, rewritten in place
+                        run = lambda: layer.backward(pc.node_flows, pc.element_flows, pc.node_mars, slab, pc.params,
+                                                     param_flows=None, allow_modify_flows=True, precision=prec,
+                                                     premodified=True)
                     else:
                         # flows already hold log(flow) - mar after the full backward above, so the
                         # layer's pass reads them as premodified and rewrites its children's flows

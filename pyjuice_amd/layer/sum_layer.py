@@ -271,7 +271,8 @@ class SumLayer(Layer, nn.Module):
         fused = prod_tables.get(id(self)) if prod_tables is not None else None
         # launch choice of this layer (set by pyjuice_amd.autotune.tune_launches): "pre" / "nopre"
         # force a form of the 16-byte-load kernels, "queue" runs the persistent work-queue kernel,
-        # "nosplit" turns the split reduction off, "split<N>" sets its split count
+        # "nosplit" turns the split reduction off, "split<N>" sets its split count, "valu" runs FP32
+        # operands on the vector ALUs
         fh = (self.__dict__.get("launch_hints") or {}).get("fw")
         for e, ((nids, cids, pids, pfids, nch, tot), run, al) in enumerate(zip(self._dev["fw"], self._fw_child_run,
                                                                                self._fw_aligned)):
@@ -301,9 +302,11 @@ class SumLayer(Layer, nn.Module):
                        N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run,
                        N.ptr(self._queue), prec, N.stream(), algo=algo)
                 continue
+            # "valu": FP32 operands on the vector-ALU tiles instead of the matrix cores (same arithmetic
+            # type; faster for some shapes, e.g. 32-row blocks)
             N.call("pjamd_sum_forward", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
                    N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B,
-                   run | _HINT_BITS.get(fh, 0), prec, N.stream(), algo=algo)
+                   run | _HINT_BITS.get(fh, 0), 3 if (fh == "valu" and prec == 0) else prec, N.stream(), algo=algo)
 
     def queue_ok(self, run: int, prec: int, B: int) -> bool:
         """Whether ``pjamd_sum_forward_queue`` takes a forward table with child-run flags ``run``."""
@@ -468,10 +471,11 @@ class SumLayer(Layer, nn.Module):
                 continue
             algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
                     2 * totp * bs * cbs * B)
+            bk_choice = (self.__dict__.get("launch_hints") or {}).get("bk")
             N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
                    N.ptr(element_mars), N.ptr(params), N.ptr(chids), N.ptr(parids), N.ptr(parpids), N.ptr(npar),
                    chids.numel(), parids.size(1), cbs, bs, B, mod | (2 if accumulate_ch_flows else 0) | al,
-                   prec, s, algo=algo)
+                   3 if (bk_choice == "valu" and prec == 0) else prec, s, algo=algo)
         if param_flows is not None:
             pf64 = 1 if param_flows.dtype == torch.float64 else 0
             flags = mod | (2 if self._pf_shared else 0) | (4 if negate_pflows else 0) | (8 * pf64)

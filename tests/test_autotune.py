@@ -114,3 +114,52 @@ def test_launch_forms_agree_and_tuner_picks_one(name):
     pc.update_parameters()
     ref = dag_eval.forward(ns, X)[ns][0]
     assert np.abs(lls - ref).max() <= 1e-3 * len(cards)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["pd_small", "hclt_cat_bs32"])
+def test_fp32_vector_alu_form_matches(name):
+    """FP32 operands: the vector-ALU form the tuner may pick per layer gives the matrix-core form's
+    results to FP32 rounding (node marginals, flows, parameter flows)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from pyjuice_amd import autotune
+    from pyjuice_amd.model import TensorCircuit
+    U.say(f"[fp32 valu form] {name}")
+    dev = torch.device("cuda:0")
+    if name == "pd_small":
+        import pyjuice_amd as juice
+        torch.manual_seed(0)
+        ns = juice.structures.PD(data_shape=(4, 4), num_latents=64, split_intervals=(2, 2), block_size=32,
+                                 input_dist=juice.distributions.Categorical(num_cats=6))
+        cards = [6] * 16
+    else:
+        ns, _, cards = U.hclt_categorical(L=64, block_size=32)
+    pc = TensorCircuit(ns, precision="fp32")
+    pc.to(dev)
+    X = U.random_data(cards, 128, seed=6)
+    xt = torch.from_numpy(X).to(dev)
+    with torch.no_grad():
+        pc(xt)
+        base_m = pc.node_mars.clone()
+        pc.backward(xt, flows_memory=0.0)
+        base_pf = pc.param_flows.clone()
+    n = 0
+    for g in pc.inner_layer_groups:
+        if g.is_sum():
+            for layer in g:
+                c_fw = autotune._fw_candidates(layer, 128, "fp32")
+                c_bk = autotune._bk_candidates(layer, 128, "fp32", id(layer) in pc._fused["tables"])
+                layer.launch_hints = {"fw": "valu" if "valu" in c_fw else None,
+                                      "bk": "valu" if "valu" in c_bk else None}
+                n += ("valu" in c_fw) + ("valu" in c_bk)
+    assert n > 0
+    with torch.no_grad():
+        pc(xt)
+        fin = torch.isfinite(base_m)
+        assert torch.equal(torch.isfinite(pc.node_mars), fin)
+        assert (pc.node_mars[fin] - base_m[fin]).abs().max().item() <= 1e-4 * len(cards)
+        pc.backward(xt, flows_memory=0.0)
+    assert U.rel_err(pc.param_flows.cpu().numpy(), base_pf.cpu().numpy()) <= 1e-4
+    summary = autotune.tune_launches(pc, xt, "fp32", reps=3)
+    assert "measured" in summary
