@@ -31,14 +31,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0}
-# HBM bytes per launch (MB) measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload,
-# scaled by the counters' reading of a 1 GiB copy; commands and table in
-# profiles/hclt784_L256_B512_bf16_hbm_traffic.txt.  Reported as roofline.traffic for the dominant
-# kernel (counters cannot be read from inside the timed process).
+# HBM bytes per launch (MB) measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the headline workload
+# at block sizes 256 and 64, scaled by the counters' reading of a 1 GiB copy; commands and tables in
+# profiles/hclt784_L256_B512_bf16_hbm_traffic.txt and profiles/hclt784_L256_bs64_B512_bf16_hbm_traffic.txt.
+# Reported as roofline.traffic for the dominant kernel (counters cannot be read from inside the timed
+# process).
 TRAFFIC_MB_PER_LAUNCH = {
-    "pjamd_sum_backward_ele_to_children": 34.04, "pjamd_sum_forward": 15.14,
-    "pjamd_sum_backward_par_batched": 1220.77, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 833.97,
-    "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.85, "pjamd_input_em": 847.86,
+    256: {"pjamd_sum_backward_ele_to_children": 34.01, "pjamd_sum_forward": 15.15,
+          "pjamd_sum_backward_par_batched": 1199.0, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 835.16,
+          "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.84, "pjamd_input_em": 847.86},
+    64: {"pjamd_sum_backward_ele_to_children": 35.21, "pjamd_sum_forward": 16.56,
+         "pjamd_sum_backward_par_batched": 1197.8, "pjamd_prod_forward": 16.12, "pjamd_input_backward": 835.14,
+         "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.95, "pjamd_input_em": 847.86},
 }
 HBM_KERNELS = ("pjamd_prod_forward", "pjamd_prod_backward", "pjamd_input_forward", "pjamd_input_backward",
                "pjamd_input_em", "pjamd_sum_em", "pjamd_sum_modify_flows", "pjamd_pflow_fuse")
@@ -189,10 +193,18 @@ def time_steps(step, steps, warmup):
 
 
 def kernel_breakdown(step, steps):
+    """Per-entry-point HIP-event timing of eager steps.  Before each step the stream is held by a
+    bounded spin kernel (~20 ms), so the host has enqueued the whole step before the device starts
+    it: the launches then run back to back and each event pair brackets its kernel, not the host's
+    submission gap."""
+    import torch
     from pyjuice_amd import _native as N
 
+    sleep = getattr(torch.cuda, "_sleep", None)
     with N.LaunchTimer() as T:
         for _ in range(steps):
+            if sleep is not None:
+                sleep(50_000_000)
             step()
     s = T.summary()
     for d in s.values():
@@ -208,9 +220,9 @@ def roofline_of(name, d, prec, args):
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
-    headline = (args.structure == "hclt" and args.latents == 256 and args.batch == 512 and args.block_size == 256
-                and prec == "bf16" and args.prune == 0 and not args.chains)
-    traffic = TRAFFIC_MB_PER_LAUNCH.get(name) if headline else None
+    headline = (args.structure == "hclt" and args.latents == 256 and args.batch == 512
+                and args.block_size in TRAFFIC_MB_PER_LAUNCH and prec == "bf16" and args.prune == 0 and not args.chains)
+    traffic = TRAFFIC_MB_PER_LAUNCH[args.block_size].get(name) if headline else None
     traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
     t_mfma = d["flops"] / (peak_f * 1e12)
@@ -422,7 +434,7 @@ def main():
     roof = roofline_of(dom[0], dom[1], main_prec, args)
 
     sampling = None
-    if args.sample_count > 0 and not args.only:
+    if args.sample_count > 0:
         pc.params.data.copy_(state["params"])
         for l, p in zip(pc.input_layer_group, in_state):
             l.params.data.copy_(p)

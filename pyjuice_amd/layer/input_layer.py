@@ -392,7 +392,9 @@ class InputLayer(Layer, nn.Module):
             N.call("pjamd_input_em", self._kind(), N.ptr(self.params), N.ptr(self.param_flows), pf64, N.ptr(t["src"]),
                    N.ptr(t["s_pids"]), N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), t["src"].numel(),
                    float(step_size), float(pseudocount), 1 if keep_zero_params else 0, N.stream(),
-                   algo=(8 * self.num_parameters + self.param_flows.element_size() * self.num_param_flows, 0))
+                   # the source vectors' parameters read and written, their (fused) flows read
+                   algo=(8 * self.num_parameters
+                         + self.param_flows.element_size() * self.num_parameters * (2 if self._kind() == 1 else 1), 0))
 
     def sample(self, samples: torch.Tensor, node_active: torch.Tensor, uniforms: Optional[torch.Tensor],
                seed: Optional[torch.Tensor] = None, cdf: Optional[torch.Tensor] = None, **kwargs):

@@ -77,6 +77,15 @@ _SPLIT_RING = 4
 _SPLIT_WS = dict()
 
 
+def _distinct_rows(idx: torch.Tensor, counts: torch.Tensor, rows_per_entry: int) -> int:
+    """Number of distinct rows named by the real entries (the first ``counts[j]`` of row j) of an
+    index table, each entry naming ``rows_per_entry`` consecutive rows."""
+    I = idx.detach().cpu().numpy().astype(np.int64)
+    k = counts.detach().cpu().numpy().astype(np.int64)
+    real = np.arange(I.shape[1])[None, :] < k[:, None]
+    return int(np.unique(I[real]).size) * rows_per_entry
+
+
 def _split_workspace(numel: int, device) -> torch.Tensor:
     key = (str(device), N.stream().value, int(numel))
     ring = _SPLIT_WS.setdefault(key, [[], 0])
@@ -246,6 +255,11 @@ class SumLayer(Layer, nn.Module):
                 bk.append((cbs,) + (_merge_rows(rows, count_col=3) if len(rows) > 1 else rows[0]))
                 bk_al.append(all(al for _, al in items))
         self._fw_child_run, self._fw_aligned, self._bk_aligned = fw_run, fw_al, bk_al
+        # distinct element rows read by each forward table and distinct parent node rows read by
+        # each backward table: the algorithmic bytes count each row once per launch (node blocks
+        # sharing children, or child blocks sharing parents, re-read them from cache)
+        self._fw_uniq = [_distinct_rows(c, k, 1) for _n, c, _p, _f, k in fw]
+        self._bk_uniq = [_distinct_rows(pa, k, self.block_size) for _cbs, _c, pa, _pp, k in bk]
         self._dev = {
             "fw": [(_i32(n), _i32(c), _i32(p), _i32(f), _i32(k), int(k.sum())) for n, c, p, f, k in fw],
             "bk": [(_i32(c), _i32(pa), _i32(pp), _i32(k), cbs, int(k.sum())) for cbs, c, pa, pp, k in bk],
@@ -281,7 +295,7 @@ class SumLayer(Layer, nn.Module):
                                                                      prec, B, self._forced_splits(fh))
             if split is not None and fused is None:
                 nsplit, ws = split
-                algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
+                algo = (4 * (self._fw_uniq[e] + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
                 N.call("pjamd_sum_forward_split", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
                        N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, nsplit,
                        N.ptr(ws), prec, N.stream(), algo=algo)
@@ -294,9 +308,9 @@ class SumLayer(Layer, nn.Module):
                        N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run, N.ptr(tab), kp,
                        prec, N.stream(), algo=algo)
                 continue
-            # algorithmic work: 2 flops per edge per sample; child rows read + node rows written
-            # per sample, parameters read once
-            algo = (4 * (tot + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
+            # algorithmic work: 2 flops per edge per sample; distinct child rows read + node rows
+            # written per sample, parameters read once
+            algo = (4 * (self._fw_uniq[e] + nids.numel() * bs) * B + 4 * tot * bs, 2 * tot * bs * B)
             if fh == "queue" and self.queue_ok(run, prec, B):
                 N.call("pjamd_sum_forward_queue", N.ptr(node_mars), N.ptr(element_mars), N.ptr(params), N.ptr(nids),
                        N.ptr(cids), N.ptr(pids), N.ptr(nch), nids.numel(), cids.size(1), bs, B, run,
@@ -451,10 +465,10 @@ class SumLayer(Layer, nn.Module):
                 bk_choice = (self.__dict__.get("launch_hints") or {}).get("bk")
                 split = None if bk_choice == "nosplit" else self._split_plan_bk(
                     q, chids.numel(), npar, cbs, bool(al), prec, B, self._forced_splits(bk_choice))
-                # parent rows (and marginals unless modified) + element rows read, one write per
-                # child row (+ its marginal when stored as log(flow) - mar), parameters once
+                # distinct parent rows (and marginals unless modified) + element rows read, one write
+                # per child row (+ its marginal when stored as log(flow) - mar), parameters once
                 n_rd = n_mod if mod else 0
-                algo = (4 * (totp * bs * (1 if mod else 2) + chids.numel() * cbs + (n_blk + n_rd) * cbs) * B
+                algo = (4 * (self._bk_uniq[q] * (1 if mod else 2) + chids.numel() * cbs + (n_blk + n_rd) * cbs) * B
                         + 4 * totp * bs * cbs, 2 * totp * bs * cbs * B)
                 if split is not None:
                     nsplit, ws = split
@@ -469,7 +483,7 @@ class SumLayer(Layer, nn.Module):
                        N.ptr(npar), N.ptr(tab), tab.size(1), chids.numel(), parids.size(1), cbs, bs, B,
                        mod | (2 if allow_modify_flows else 0) | al | bh, prec, s, algo=algo)
                 continue
-            algo = (4 * (totp * bs * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
+            algo = (4 * (self._bk_uniq[q] * (1 if mod else 2) + 2 * chids.numel() * cbs) * B + 4 * totp * bs * cbs,
                     2 * totp * bs * cbs * B)
             bk_choice = (self.__dict__.get("launch_hints") or {}).get("bk")
             N.call("pjamd_sum_backward_ele", N.ptr(element_flows), N.ptr(node_flows), N.ptr(node_mars),
@@ -482,8 +496,8 @@ class SumLayer(Layer, nn.Module):
             if overwrite_pflows and not self._pf_shared:
                 flags |= 16
             pfw = 8 if pf64 else 4
-            for nids, cids, pids, pfids, nch, tot in self._dev["fw"]:
-                algo = (4 * (nids.numel() * bs * (1 if mod else 2) + tot) * B + (4 + 2 * pfw) * tot * bs,
+            for e, (nids, cids, pids, pfids, nch, tot) in enumerate(self._dev["fw"]):
+                algo = (4 * (nids.numel() * bs * (1 if mod else 2) + self._fw_uniq[e]) * B + (4 + 2 * pfw) * tot * bs,
                         2 * tot * bs * B)
                 N.call("pjamd_sum_backward_par", N.ptr(param_flows), N.ptr(node_flows), N.ptr(node_mars),
                        N.ptr(element_mars), N.ptr(params), N.ptr(nids), N.ptr(cids), N.ptr(pids), N.ptr(pfids),

@@ -113,7 +113,7 @@ class TensorCircuit(nn.Module):
         self.default_propagation_alg = "LL"
         self.propagation_alg_kwargs = dict()
         self._run_params = dict()
-        self._cum_flow = 0.0
+        self._cum_flow_host, self._cum_flow_dev = 0.0, None
         self._em_dev = None
         self.modify_flows_in_place = True
         self._par_batches = {"tables": dict(), "layers": set()}
@@ -142,6 +142,20 @@ class TensorCircuit(nn.Module):
         self._data_vb, self._data_marginalized = None, False
 
     # ------------------------------------------------------------------ parameter-flow buffer
+
+    @property
+    def _cum_flow(self) -> float:
+        """Total root flow added to the parameter flows since the last rescaled EM step
+        (``tensorcircuit.py:417-421``).  Weighted backward passes add their weight sums on the
+        device; they are read back only here, so a backward never waits for the device."""
+        if self._cum_flow_dev is not None:
+            self._cum_flow_host += float(self._cum_flow_dev.item())
+            self._cum_flow_dev = None
+        return self._cum_flow_host
+
+    @_cum_flow.setter
+    def _cum_flow(self, value: float):
+        self._cum_flow_host, self._cum_flow_dev = float(value), None
 
     @property
     def param_flows(self):
@@ -474,8 +488,8 @@ class TensorCircuit(nn.Module):
             erow = self._elem_offsets[gid // 2]
             tm, tn = (64, 64) if bs >= 64 else (32, 128)
             atomic = layer._pf_shared or id(layer) in overlapping
-            for nids, cids, pids, pfids, nch, tot in layer._dev["fw"]:
-                acc[tm].append((nids, cids, pids, pfids, nch, tot, bs, tn, erow, atomic))
+            for e, (nids, cids, pids, pfids, nch, tot) in enumerate(layer._dev["fw"]):
+                acc[tm].append((nids, cids, pids, pfids, nch, tot, bs, tn, erow, atomic, layer._fw_uniq[e]))
         batches = dict()
         for tm, parts in acc.items():
             if not parts:
@@ -488,7 +502,7 @@ class TensorCircuit(nn.Module):
             d = 0
             for u in units:
                 for c in range(len(u)):
-                    nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared = parts[d]
+                    nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared, uniq = parts[d]
                     nb, K = nids.numel(), cids.size(1)
                     mt, kt = bs // tm, (K + tn - 1) // tn
                     tiles = nb * mt * kt if c == 0 else 0
@@ -499,7 +513,7 @@ class TensorCircuit(nn.Module):
                     blk_off += nb
                     idx_off += nb * K
                     tot_rows += nb * bs
-                    tot_edges += tot
+                    tot_edges += uniq        # distinct element rows of the table
                     tot_edges_bs += tot * bs
                     d += 1
             assert idx_off < 2 ** 31 and n_tiles < 2 ** 31 // 256
@@ -527,7 +541,7 @@ class TensorCircuit(nn.Module):
         HMM, tied copies of one sum node set) are tied copies; a group of them is split into just
         enough units to give ``par_tied_tiles`` tiles.  Other partitions are units of one."""
         groups, order = dict(), []
-        for i, (nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared) in enumerate(parts):
+        for i, (nids, cids, pids, pfids, nch, tot, bs, tn, erow, shared, _uniq) in enumerate(parts):
             if not shared:
                 order.append([i])
                 continue
@@ -894,7 +908,11 @@ class TensorCircuit(nn.Module):
                 assert ll_weights.size(0) == self.num_root_nodes
                 self.node_flows[r0:r1, :] = ll_weights.to(self.node_flows.dtype)
             if compute_param_flows:
-                self._cum_flow += (r1 - r0) * B if ll_weights is None else float(ll_weights.sum().item())
+                if ll_weights is None:
+                    self._cum_flow_host += (r1 - r0) * B
+                else:
+                    w = ll_weights.detach().sum().to(torch.float64)
+                    self._cum_flow_dev = w if self._cum_flow_dev is None else self._cum_flow_dev + w
                 self.init_param_flows(flows_memory=flows_memory)
 
             # The first backward after init_param_flows(0.0) stores the sum parameter flows instead
