@@ -81,7 +81,8 @@ def parse():
                     help="keep the library's default launch form of every sum layer instead of the measured "
                          "choice (pyjuice_amd.autotune.tune_launches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-samples", type=int, default=32)
+    ap.add_argument("--cpu-samples", type=int, default=128,
+                    help="samples of the bench batch in the CPU leg (baseline step and FP64 accuracy check)")
     ap.add_argument("--only", default="", help="profiling aid: run only this precision, no extra legs")
     ap.add_argument("--sample-count", type=int, default=512,
                     help="samples per pass of the sampling leg (0: no sampling leg)")
@@ -336,7 +337,7 @@ def cpu_baseline(pc, x_np, lls_by_prec, args):
 
     cpu = CompiledCPU(pc, np.float32)
     steps, t_tot = 0, 0.0
-    while steps < 1 or (t_tot < 10.0 and steps < 20):
+    while steps < 1 or (t_tot < 10.0 and steps < 200):
         t0 = time.perf_counter()
         cpu.zero_param_flows()
         cpu.forward(xs)
