@@ -396,45 +396,68 @@ int pjamd_sum_backward_par_batched(void* param_flows, const float* node_flows, c
 
 /* ----------------------------------------------------------------------------- layer chains */
 
-/* Number of int64 fields per operation of a chain, and the longest reduction (children of a
- * node block, or parent rows of a child block) a chain operation stages: PJAMD_CHAIN_MAX_K with
- * BF16 / FP16 operands, half of it with FP32 operands. */
-#define PJAMD_CHAIN_OP_FIELDS 16
-#define PJAMD_CHAIN_MAX_K 1024
+/* Sizes of the chain tables (int32 fields per operation and per component, int64 fields per
+ * pack descriptor) and the limits an operation / component must respect. */
+#define PJAMD_CHAIN_OP_INTS 12
+#define PJAMD_CHAIN_COMP_INTS 4
+#define PJAMD_CHAIN_DESC_INTS 10
+#define PJAMD_CHAIN_MAX_OPS 64
+#define PJAMD_CHAIN_TAB_INTS 2048
+#define PJAMD_CHAIN_MAX_ROWS 256
+#define PJAMD_CHAIN_MAX_KC 4
+#define PJAMD_CHAIN_COLS 16
 
-/* A run of consecutive inner layers in one launch, split over the batch: workgroup g computes
- * columns [32 g, 32 g + 32) of every operation in order (batch columns never interact), with a
- * workgroup barrier between operations.  ops is int64 [n_ops][PJAMD_CHAIN_OP_FIELDS] in device
- * memory, idx the int32 index tables the operations point into:
- *   {kind, n_blocks, K, bs, cbs, element row offset, a, b, c, d, e, Kc, flags, 0, 0, 0}
- *   kind 0, product partition: a = nids, b = cids (as pjamd_prod_forward, rows of the element
- *           slab that starts at the element row offset);
- *   kind 1, sum partition forward: a = nids, b = cids, c = pids, d = n_chs (as
- *           pjamd_sum_forward; bs a power of two >= 32; flags bit 2: every aligned group of 8
- *           entries of a cids / pids row is a run, cids consecutive and pids bs apart).
- * An operation stages n_blocks * K index entries, and the reductions of its blocks (children per
- * block, or parent rows per child block for kind 2) each rounded up to 64, in at most the staged
- * length; at most 64 blocks.
- * Results equal the per-layer entry points' up to the order of floating-point additions.
- * Replaces the forward loop over the inner layer groups of TensorCircuit.forward
- * (tensorcircuit.py:269-283) for the layers of the run. */
-int pjamd_chain_forward(float* node_mars, float* element_mars, const float* params, const int64_t* ops, int n_ops,
-                        const int* idx, int64_t B, int precision, pjamd_stream_t stream);
+/* Packed matrix-core operands of the chain groups (precision 1 = BF16, 2 = FP16), written from
+ * the FP32 parameters.  Group i (desc int64 [n][PJAMD_CHAIN_DESC_INTS] =
+ * {R, K, log2 bs, log2 cbs, parent blocks, child blocks, forward offset, backward offset,
+ * pbase offset, 0}; offsets in 16-byte units) is the dense theta[p][c] = params[pbase[p / bs]
+ * [c / cbs] + (c % cbs) bs + p % bs] over R parent rows and K child rows, packed for the
+ * 16x16x32 matrix-core instructions: forward unit (t, s, l) = theta[16 t + l % 16]
+ * [32 s + 8 (l / 16) + q], backward unit (t, s, l) = theta[32 s + 8 (l / 16) + q][16 t + l % 16],
+ * q < 8, units in row-major (t, s, l) order.
+ * Workgroup j packs 128 units: group wg[3 j], direction wg[3 j + 1], from unit wg[3 j + 2].
+ * No counterpart in the reference, which reads FP32 parameters in every layer kernel
+ * (sum_layer.py:454-568); this copy feeds pjamd_chain_forward / pjamd_chain_backward. */
+int pjamd_chain_pack(void* packed, const float* params, const int64_t* desc, const int64_t* pbase, const int* wg,
+                     int n_wg, int precision, pjamd_stream_t stream);
 
-/* The backward of a run of layers, top-down, same split and layout as pjamd_chain_forward:
- *   kind 2, backward partition of a sum layer fused with the product layer below it: a = chids,
- *           b = parids, c = parpids, d = n_pars, e = ch_nodes with row width Kc, cbs the child
- *           block size (as pjamd_sum_backward_ele_to_children; flags bit 0: the parents' flows
- *           hold log(flow) - mar; bit 1: honour the log(flow) - mar marks of ch_nodes; bit 2:
- *           every parpids entry is a multiple of 4 and bs is, so parameter rows are 16-byte
- *           aligned);
- *   kind 3, modify pass of a sum partition: a = nids, bs (as pjamd_sum_modify_flows).
- * Replaces the backward loop over the inner layer groups of TensorCircuit.backward
- * (tensorcircuit.py:436-470) for the layers of the run (their parameter flows come from
- * pjamd_sum_backward_par_batched afterwards). */
-int pjamd_chain_backward(float* node_flows, const float* node_mars, const float* element_mars, const float* params,
-                         const int64_t* ops, int n_ops, const int* idx, int64_t B, int precision,
+/* A run of narrow levels (product layer + sum layer above it) in one launch.  The run's groups
+ * (dense sum blocks, R <= 256 parent rows over K <= 256 child element rows, K a multiple of 32,
+ * block sizes powers of two, bs >= 8, cbs >= 32) are split into n_comp components that never
+ * exchange data; workgroup w computes component w % n_comp on batch columns
+ * [16 (w / n_comp), 16 (w / n_comp) + 16), its operations in order.  comp int32 [n_comp][4] = {first operation, operations (<= PJAMD_CHAIN_MAX_OPS), first
+ * table entry, table entries (<= PJAMD_CHAIN_TAB_INTS)}; ops int32 [n][PJAMD_CHAIN_OP_INTS] =
+ * {R, K, log2 bs, log2 cbs, parent blocks, child blocks, KC (<= 4), nid offset, child-block
+ * offset, packed operand offset (16-byte units), flags, 0}; table offsets are relative to the
+ * component's first entry: nid = first node row of each parent block; child-block entries of
+ * 1 + KC ints = {first element row (of the whole element_mars), KC child references: node-row
+ * start (0 = none), or -(1 + 256 slot + first position) for rows of one of the component's two
+ * previous operations, which the kernel keeps in LDS}; flags bit 3: the operation reads rows its
+ * component wrote earlier from global memory.  Forward: element rows = sums of the children's node rows (stored), node rows =
+ * log sum theta exp(element rows).  B % 4 == 0.  Replaces the forward loop over the inner layer
+ * groups of TensorCircuit.forward (tensorcircuit.py:269-283) for the levels of the run. */
+int pjamd_chain_forward(float* node_mars, float* element_mars, const void* packed, const int* comp, const int* ops,
+                        const int* tabs, int n_comp, int64_t B, int precision, pjamd_stream_t stream);
+
+/* The backward of a run, top-down, same split and tables; each parent block's node row is
+ * followed (after all of them) by its flow source: -1 = node_flows, else 256 slot + first row of
+ * the element flows one of the component's two previous operations keeps in LDS; child-block
+ * entries hold codes (node row << 2 | marked | 2 when every row of the child is read from LDS by
+ * a later operation, which then stores them as log(flow) - mar) instead of node rows.  Per group: element flows of the child rows
+ * from the parents' log(flow) - mar (flags bit 0: the parents' flows already hold it; bit 1:
+ * they are raw and it is stored back; neither: computed on the fly), written straight into the
+ * product children's node flows (every child has one parent), as log(flow) - mar for the
+ * marked children when flags bit 2 is set.  Replaces the backward loop over the inner layer
+ * groups of TensorCircuit.backward (tensorcircuit.py:436-470) for the levels of the run (their
+ * parameter flows come from pjamd_sum_backward_par_batched afterwards). */
+int pjamd_chain_backward(float* node_flows, const float* node_mars, const float* element_mars, const void* packed,
+                         const int* comp, const int* ops, const int* tabs, int n_comp, int64_t B, int precision,
                          pjamd_stream_t stream);
+
+/* Debug: C[16][16] = A[16][32] B[32][16] (row-major FP32 buffers) through one 16x16x32 BF16
+ * (precision 1) or FP16 (2) matrix-core instruction with the lane layout the chain kernels
+ * assume.  No counterpart in the reference. */
+int pjamd_debug_mfma16_probe(int precision, const float* A, const float* B, float* C, pjamd_stream_t stream);
 
 /* ----------------------------------------------------------------------------- EM */
 

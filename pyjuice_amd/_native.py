@@ -74,8 +74,10 @@ _SIGNATURES = {
                                _i32, _i32, _i32, _i64, _i32, _i32, _c_p],
     "pjamd_sum_backward_par_batched": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
                                        _i32, _i32, _i64, _i32, _i32, _c_p],
-    "pjamd_chain_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i64, _i32, _c_p],
-    "pjamd_chain_backward": [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _c_p, _i64, _i32, _c_p],
+    "pjamd_debug_mfma16_probe": [_i32, _c_p, _c_p, _c_p, _c_p],
+    "pjamd_chain_pack": [_c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _c_p],
+    "pjamd_chain_forward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i32, _c_p],
+    "pjamd_chain_backward": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i32, _c_p],
     "pjamd_sum_em": [_c_p, _c_p, _i32, _c_p, _i32, _f32, _f32, _i32, _i32, _c_p],
     "pjamd_pflow_fuse": [_c_p, _i32, _i64, _i64, _c_p, _i32, _c_p],
     "pjamd_sample_sum": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],

@@ -1,566 +1,601 @@
-// Chains of narrow layers in one launch.
+// Runs of narrow levels in one launch per pass (planner: pyjuice_amd/model/chain.py).
 //
-// Deep tree-shaped circuits (HCLT, HMM) end in long runs of layers with only a few node blocks:
-// one launch per layer keeps most of the GPU idle and pays a launch plus several dependent memory
-// round trips per layer.  Every batch column of a circuit pass is independent of every other, so
-// a run of layers can be split over the batch instead: workgroup g owns columns
-// [32 g, 32 g + 32) and walks the run's operations in order, all node rows of each layer, with a
-// workgroup barrier between operations.  No workgroup ever waits for another.
+// A run is split into components (subtrees that never exchange data inside the run) and the
+// batch into slices of 16 columns; workgroup w owns component w % C and slice w / C and walks
+// the component's group operations in order.  No workgroup ever waits for another.
 //
-// Forward operations (pjamd_chain_forward):
-//   kind 0  product layer partition: element_mars[slab + nid + i] = sum_k node_mars[cid_k + i]
-//   kind 1  sum layer partition:     node_mars[nid + i] = log sum_k theta[pid_k + i] exp(e_k)
-// Backward operations (pjamd_chain_backward):
-//   kind 2  sum layer backward partition fused with the product layer below it (children with a
-//           single parent): the flows of the product nodes' children are written to node_flows
-//   kind 3  modify pass of a sum layer partition: node_flows = log(flow) - mar
+// A group operation is a dense block of a sum layer: R <= 256 parent rows (node blocks of bs
+// rows) over K <= 256 child element rows (product blocks of cbs >= 32 rows, each the sum of at
+// most 4 children's node rows).  The matrix-core A operand (theta forward, theta^T backward)
+// comes from a packed BF16 / FP16 copy in the 16x16x32 lane layout (pjamd_chain_pack), one
+// 16-byte load per lane per instruction; the B operand (exp of child rows, or of parent
+// log(flow) - mar) is staged in LDS.  Eight wavefronts; wavefront w owns the 16-row output
+// tiles w and w + 8.  Element-wise work is spread as 2 rows x 4 columns per thread (row group
+// g = tid / 4, column quad q = tid % 4; a wavefront holds 32 consecutive rows).
 //
-// One workgroup streams every parameter of the run, so its time is set by how many bytes it
-// keeps in flight.  Eight wavefronts; an operation's index rows are staged in LDS once; every
-// phase issues all its loads before consuming any; a sum operation first stages the MFMA B
-// operand of all its blocks in LDS (exp(child - column max), or exp(log(flow / mar) - column
-// max) for the element flows), then each wavefront runs one pipeline over all its 32-row tiles of
-// all blocks, with the parameters (A operand, straight from global memory in the matrix-core
-// lane layout) of the next two 64-wide reduction chunks in flight.
+// Hand-over between operations: the outputs of the last two operations stay in an LDS ring
+// (node rows forward, element flows backward) and the planner points the next operations' reads
+// there; barriers between operations cover LDS only, so global stores stay in flight.  An
+// operation that must read rows of its component from global memory starts with a full barrier
+// (flag SYNC).  Software pipeline: the loads of operation o + 1 that do not depend on the run
+// (packed operands, rows computed before the launch) are issued at the start of operation o,
+// into a second register set.
+//
+// The index tables of a component (PJAMD_CHAIN_TAB_INTS entries at most) and its operation
+// headers are staged in LDS once, at kernel start.
 #include "pjamd_mfma.h"
 
 using namespace pjamd;
 
 namespace {
 
-constexpr int kThreads = 512;               // eight wavefronts
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
-constexpr int kCols = 32;                   // batch columns per workgroup
-constexpr int kRowGroups = kThreads / kCols;
-constexpr int kPad = 64;                    // block reductions are staged in multiples of this
-constexpr int kChunk = 32;                  // reduction values per pipelined chunk
-constexpr int kPerLane = 16;                // A values per lane per chunk (both operand types)
-constexpr int kMaxBlocks = 64;              // node / child blocks per sum operation
-constexpr int kQ = 32;                      // B-operand values per thread per pass (16 row groups x 32 = 512)
+constexpr int kCols = PJAMD_CHAIN_COLS;              // batch columns per workgroup (16)
+constexpr int kOpw = PJAMD_CHAIN_OP_INTS;
+constexpr int kMaxOps = PJAMD_CHAIN_MAX_OPS;
+constexpr int kTab = PJAMD_CHAIN_TAB_INTS;
+constexpr int kMaxRows = PJAMD_CHAIN_MAX_ROWS;       // R and K bound
+constexpr int kMaxKc = PJAMD_CHAIN_MAX_KC;
+constexpr int kSteps = kMaxRows / 32;                // 16x16x32 instructions of one tile
+constexpr int kTiles = kMaxRows / 16 / kWaves;       // 16-row tiles per wavefront
+constexpr int kLd = kMaxRows + 8;                    // LDS row stride of the B operand (16-bit entries)
+constexpr int kRing = kCols + 4;                     // LDS row stride of the ring (floats)
 
-template <typename T>
-struct ChainCap {
-    static constexpr int K = PJAMD_CHAIN_MAX_K;   // staged reduction entries of one operation
-};
-template <>
-struct ChainCap<float> {
-    static constexpr int K = PJAMD_CHAIN_MAX_K / 2;
-};
+static_assert(kCols == 16 && kThreads / 4 * 2 == kMaxRows, "thread layout: 2 rows x 4 columns per thread");
 
-struct Op {
-    int kind, nb, K, bs, cbs, Kc, flags;
-    int64_t eoff, a, b, c, d, e;
-};
-
-__device__ __forceinline__ Op load_op(const int64_t* __restrict__ o) {
-    Op op;
-    op.kind = (int)o[0];
-    op.nb = (int)o[1];
-    op.K = (int)o[2];
-    op.bs = (int)o[3];
-    op.cbs = (int)o[4];
-    op.eoff = o[5];
-    op.a = o[6];
-    op.b = o[7];
-    op.c = o[8];
-    op.d = o[9];
-    op.e = o[10];
-    op.Kc = (int)o[11];
-    op.flags = (int)o[12];
-    return op;
-}
+enum { F_R = 0, F_K, F_LBS, F_LCBS, F_NPB, F_NCB, F_KC, F_NID, F_CB, F_A, F_FLAGS };
+enum { PAR_LFM = 1, PAR_WRITE = 2, CH_MOD = 4, SYNC = 8 };
 
 template <typename T>
 struct Smem {
-    static constexpr int CAP = ChainCap<T>::K;
-    static constexpr int LDB = CAP + MF<T>::PAD;
-    T Bs[kCols][LDB];                  // MFMA B operand of all blocks: [column][block offset + k]
-    int s0[CAP];                       // staged index rows (cids / parids / nids)
-    int s1[CAP];                       // staged index rows (pids / parpids / cids)
-    int cnt[kMaxBlocks];               // reduction length of each block (children / parent rows)
-    int boff[kMaxBlocks + 1];          // block offsets into the B operand (multiples of kPad)
-    int first[kMaxBlocks];             // kind 2: first non-zero child code of each block (0: none)
-    int nid[kMaxBlocks];               // first node (kind 1) / element (kind 2) row of each block
-    float cmax[kMaxBlocks][kCols];
-    float red[kRowGroups][kCols];
+    int ops[kMaxOps * kOpw];
+    int tab[kTab];
+    __attribute__((aligned(16))) T Bs[kCols][kLd];            // B operand: [column][reduction index]
+    __attribute__((aligned(16))) float red[kWaves][kCols];    // per wavefront column maxima
+    // outputs of the last two operations (slot = operation index & 1): forward node rows by
+    // output position, backward element flows by child row
+    __attribute__((aligned(16))) float ring[2][kMaxRows][kRing];
 };
 
-// Column max over the workgroup's row groups into cmax[j].
-template <typename T>
-__device__ __forceinline__ float column_max(Smem<T>& sm, int j, float m, int rg, int col) {
-    sm.red[rg][col] = m;
-    __syncthreads();
-    if (threadIdx.x < kCols) {
-        float v = sm.red[0][threadIdx.x];
-#pragma unroll
-        for (int g = 1; g < kRowGroups; ++g) v = fmaxf(v, sm.red[g][threadIdx.x]);
-        sm.cmax[j][threadIdx.x] = v;
-    }
-    __syncthreads();
-    return sm.cmax[j][col];
-}
-
-// Stage n entries of an index table into LDS (callers add the barrier).
-__device__ __forceinline__ void stage(int* dst, const int* __restrict__ src, int n) {
-    for (int u = threadIdx.x; u < n; u += kThreads) dst[u] = src[u];
+// Barrier over the workgroup's LDS traffic only (global stores stay in flight), and the full one.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+__device__ __forceinline__ void sync_barrier() {
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Every load below is unconditional, from an address clamped into the buffer, with the value
-// masked afterwards: a load under a per-lane branch makes the compiler wait for every load in
-// flight before the next use of any of them.
-
-// kind 0: product partition (gather-add).  A thread owns rows rg + 16 u of its column; U rows x 4
-// children of loads are in flight at a time.
+// Stage the component's operation headers and tables; returns the number of operations.
 template <typename T>
-__device__ void chain_prod(const Op& op, const float* __restrict__ nm, float* __restrict__ em,
-                           const int* __restrict__ idx, int64_t B, int64_t b0, Smem<T>& sm) {
-    const int tid = threadIdx.x, col = tid % kCols, rg = tid / kCols;
-    const int64_t b = b0 + col;
-    const bool ok = b < B;
-    const int64_t bc = ok ? b : B - 1;
-    const int lbs = __builtin_ctz((unsigned)op.bs);
-    const int rows = op.nb << lbs, K = op.K;
-    stage(sm.s0, idx + op.a, op.nb);
-    stage(sm.s1, idx + op.b, op.nb * K);
+__device__ __forceinline__ int stage_component(Smem<T>& sm, const int* __restrict__ comp,
+                                               const int* __restrict__ ops, const int* __restrict__ tabs, int c) {
+    // the planner keeps a component within the LDS capacity; the clamps only bound the copies
+    const int op0 = comp[c * PJAMD_CHAIN_COMP_INTS + 0];
+    const int nops = min(comp[c * PJAMD_CHAIN_COMP_INTS + 1], kMaxOps);
+    const int tab0 = comp[c * PJAMD_CHAIN_COMP_INTS + 2];
+    const int ntab = min(comp[c * PJAMD_CHAIN_COMP_INTS + 3], kTab);
+    for (int i = threadIdx.x; i < nops * kOpw; i += kThreads) sm.ops[i] = ops[(int64_t)op0 * kOpw + i];
+    for (int i = threadIdx.x; i < ntab; i += kThreads) sm.tab[i] = tabs[tab0 + i];
     __syncthreads();
-    constexpr int U = 8, KU = 4;
-    for (int r0 = rg; r0 < rows; r0 += kRowGroups * U) {
-        float acc[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc[u] = 0.0f;
-        for (int k0 = 0; k0 < K; k0 += KU) {
-            float v[U][KU];
-            bool use[U][KU];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int r = r0 + u * kRowGroups;
-                const int rr = r < rows ? r : rows - 1;
-#pragma unroll
-                for (int q = 0; q < KU; ++q) {
-                    const int k = k0 + q < K ? k0 + q : K - 1;
-                    const int ch = sm.s1[(rr >> lbs) * K + k];
-                    v[u][q] = nm[((int64_t)ch + (rr & (op.bs - 1))) * B + bc];
-                    use[u][q] = ch != 0 && k0 + q < K;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int q = 0; q < KU; ++q) acc[u] += use[u][q] ? v[u][q] : 0.0f;
-        }
-        if (ok) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int r = r0 + u * kRowGroups;
-                if (r < rows) em[(op.eoff + sm.s0[r >> lbs] + (r & (op.bs - 1))) * B + b] = acc[u];
-            }
-        }
-    }
+    return nops;
 }
 
-// kind 3: node_flows[nid + i] = log(flow) - mar for the rows of a sum partition.
-template <typename T>
-__device__ void chain_modify(const Op& op, float* __restrict__ nf, const float* __restrict__ nm,
-                             const int* __restrict__ idx, int64_t B, int64_t b0, Smem<T>& sm) {
-    const int tid = threadIdx.x, col = tid % kCols, rg = tid / kCols;
-    const int64_t b = b0 + col;
-    const int lbs = __builtin_ctz((unsigned)op.bs);
-    const int rows = op.nb << lbs;
-    stage(sm.s0, idx + op.a, op.nb);
-    __syncthreads();
-    if (b >= B) return;
-    constexpr int U = 8;
-    for (int r0 = rg; r0 < rows; r0 += kRowGroups * U) {
-        float f[U], m[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int r = r0 + u * kRowGroups;
-            const int rr = r < rows ? r : rows - 1;
-            const int64_t o = ((int64_t)sm.s0[rr >> lbs] + (rr & (op.bs - 1))) * B + b;
-            f[u] = nf[o];
-            m[u] = nm[o];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int r = r0 + u * kRowGroups;
-            if (r < rows) {
-                const int64_t o = ((int64_t)sm.s0[r >> lbs] + (r & (op.bs - 1))) * B + b;
-                nf[o] = (m[u] == kNegInf) ? kNegInf : __logf(f[u]) - m[u];
-            }
-        }
-    }
-}
-
-// Position of a wavefront's pipeline: tile g (block j = g / tiles per block) and chunk c.
-// Wavefront-uniform (scalar registers).
-struct Cursor {
-    int g, c;
+struct Hdr {
+    int R, K, lbs, lcbs, npb, ncb, KC, nid_off, cb_off, a0, flags;
 };
 
-// The sum operations (kinds 1 and 2) share the staging and the pipeline; they differ in what the
-// B operand holds, where the A operand lives and what the epilogue writes.
-//   kind 1: block j = node block; reduction over its n_j children; rows = the block's bs nodes.
-//           flags bit 2: every aligned group of 8 children is a run (pids 8-aligned groups step
-//           by bs), so a lane's 8 parameters need one index.
-//   kind 2: block j = child (product) block; reduction over the np_j * bs parent rows; rows = the
-//           block's cbs child rows.  flags bit 2: parameter bases are multiples of 4 (16-byte
-//           aligned rows of 8 consecutive parameters).
-// Every block gets at least one chunk, so blocks without children / parents produce -inf / 0.
-template <typename T, int KIND>
-__device__ void chain_sum(const Op& op, float* __restrict__ out, const float* nflows, const float* __restrict__ nm,
-                          const float* __restrict__ em, const float* __restrict__ params,
-                          const int* __restrict__ idx, int64_t B, int64_t b0, Smem<T>& sm) {
-    constexpr int KPL = MF<T>::KPL, KS = 2 * KPL, STEPS = kChunk / KS;
-    static_assert(STEPS * KPL == kPerLane, "chunk layout");
-    const int tid = threadIdx.x, col = tid % kCols, rg = tid / kCols;
-    const int lane = tid & 63, h = lane >> 5, mcol = lane & 31;
-    const int w = uni(tid >> 6);
-    const int64_t b = b0 + col;
-    const bool ok = b < B;
-    const int64_t bc = ok ? b : B - 1;
-    const int64_t bm = b0 + mcol;
-    const int nb = op.nb, K = op.K;
-    const int bs = op.bs, lbs = __builtin_ctz((unsigned)bs);
-    const bool modified = op.flags & 1, mod_children = op.flags & 2, fast = op.flags & 4;
-    const int rows_per_block = KIND == 1 ? bs : op.cbs;
+template <typename T>
+__device__ __forceinline__ Hdr header(const Smem<T>& sm, int o) {
+    const int* op = sm.ops + o * kOpw;
+    Hdr h;
+    h.R = uni(op[F_R]);
+    h.K = uni(op[F_K]);
+    h.lbs = uni(op[F_LBS]);
+    h.lcbs = uni(op[F_LCBS]);
+    h.npb = uni(op[F_NPB]);
+    h.ncb = uni(op[F_NCB]);
+    h.KC = uni(op[F_KC]);
+    h.nid_off = uni(op[F_NID]);
+    h.cb_off = uni(op[F_CB]);
+    h.a0 = uni(op[F_A]);
+    h.flags = uni(op[F_FLAGS]);
+    return h;
+}
 
-    // ---- stage index rows and block offsets
-    stage(sm.s0, idx + op.b, nb * K);
-    stage(sm.s1, idx + op.c, nb * K);
-    if (tid < nb) {
-        sm.cnt[tid] = KIND == 1 ? idx[op.d + tid] : (idx[op.d + tid] << lbs);
-        sm.nid[tid] = idx[op.a + tid];
-        if (KIND == 2) {
-            int f = 0;
-            for (int q = op.Kc - 1; q >= 0; --q) {
-                const int code = idx[op.e + (int64_t)tid * op.Kc + q];
-                if (code != 0) f = code;
-            }
-            sm.first[tid] = f;
-        }
+__device__ __forceinline__ void add4(float4& a, const float4 b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    a.w += b.w;
+}
+
+__device__ __forceinline__ float4 ninf4() { return make_float4(kNegInf, kNegInf, kNegInf, kNegInf); }
+
+// log(f) - m, element-wise (-inf where m is -inf).
+__device__ __forceinline__ float4 lfdm4(const float4 f, const float4 m) {
+    float4 y;
+    y.x = m.x == kNegInf ? kNegInf : __logf(f.x) - m.x;
+    y.y = m.y == kNegInf ? kNegInf : __logf(f.y) - m.y;
+    y.z = m.z == kNegInf ? kNegInf : __logf(f.z) - m.z;
+    y.w = m.w == kNegInf ? kNegInf : __logf(f.w) - m.w;
+    return y;
+}
+
+// Column maxima of a [rows, 16] block held as v[2 rows] per thread: reduced across the
+// wavefront's 16 row groups with lane exchanges, then across the nw wavefronts that hold rows
+// through LDS (one barrier).  m[4]: this thread's quad's maxima; mc: the maximum of column mcol.
+template <typename T>
+__device__ __forceinline__ void column_max(Smem<T>& sm, const float4 (&v)[2], int q, int nw, int mcol,
+                                           float (&m)[4], float& mc) {
+    float a[4] = {fmaxf(v[0].x, v[1].x), fmaxf(v[0].y, v[1].y), fmaxf(v[0].z, v[1].z), fmaxf(v[0].w, v[1].w)};
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) a[c] = fmaxf(a[c], __shfl_xor(a[c], o, 64));
     }
-    __syncthreads();
-    if (tid == 0) {
-        int o = 0;
-        for (int j = 0; j < nb; ++j) {
-            sm.boff[j] = o;
-            const int len = sm.cnt[j];
-            o += len > 0 ? (len + kPad - 1) / kPad * kPad : kPad;
-        }
-        sm.boff[nb] = o;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane < 4) *reinterpret_cast<float4*>(&sm.red[w][4 * q]) = make_float4(a[0], a[1], a[2], a[3]);
+    lds_barrier();
+    float4 x = *reinterpret_cast<const float4*>(&sm.red[0][4 * q]);
+    float y = sm.red[0][mcol];
+    for (int i = 1; i < nw; ++i) {
+        const float4 z = *reinterpret_cast<const float4*>(&sm.red[i][4 * q]);
+        x.x = fmaxf(x.x, z.x);
+        x.y = fmaxf(x.y, z.y);
+        x.z = fmaxf(x.z, z.z);
+        x.w = fmaxf(x.w, z.w);
+        y = fmaxf(y, sm.red[i][mcol]);
     }
-    __syncthreads();
+    m[0] = x.x;
+    m[1] = x.y;
+    m[2] = x.z;
+    m[3] = x.w;
+    mc = y;
+}
 
-    // ---- B operand of every block: one pass of loads per block
-    // values k = p0 + rg + 16 q of block j: all loads first, then the transform and the mask
-    auto bvals = [&](int j, int p0, int len, float* v) {
-        float f[kQ], mm[kQ];
+// Bs[4q + c][2g + r] = exp(v[r].c - m[c]) (0 where m is -inf).
+template <typename T>
+__device__ __forceinline__ void stage_b(Smem<T>& sm, const float4 (&v)[2], const float (&m)[4], int g, int q) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) {
-            const int k = p0 + rg + kRowGroups * q;
-            const int kk = k < len ? k : (len > 0 ? len - 1 : 0);
-            if (KIND == 1) {
-                f[q] = em[(op.eoff + sm.s0[j * K + kk]) * B + bc];
-            } else {
-                const int64_t o = ((int64_t)sm.s0[j * K + (kk >> lbs)] + (kk & (bs - 1))) * B + bc;
-                f[q] = nflows[o];
-                if (!modified) mm[q] = nm[o];
-            }
-        }
+    for (int c = 0; c < 4; ++c) {
+        T2 x;
 #pragma unroll
-        for (int q = 0; q < kQ; ++q) {
-            const int k = p0 + rg + kRowGroups * q;
-            float x = f[q];
-            if (KIND == 2 && !modified) x = (mm[q] == kNegInf) ? kNegInf : __logf(f[q]) - mm[q];
-            v[q] = (k < len && ok) ? x : kNegInf;
+        for (int r = 0; r < 2; ++r) {
+            const float e = c == 0 ? v[r].x : c == 1 ? v[r].y : c == 2 ? v[r].z : v[r].w;
+            x[r] = (T)(m[c] == kNegInf ? 0.0f : __expf(e - m[c]));
         }
-    };
-    for (int j = 0; j < nb; ++j) {
-        const int len = sm.cnt[j];
-        const int padded = sm.boff[j + 1] - sm.boff[j];
-        float v[kQ];
-        float m = kNegInf;
-        for (int p0 = 0; p0 < len; p0 += kRowGroups * kQ) {
-            bvals(j, p0, len, v);
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) m = fmaxf(m, v[q]);
-        }
-        const float mx = column_max(sm, j, m, rg, col);
-        const bool one_pass = len <= kRowGroups * kQ;
-        for (int p0 = 0; p0 < padded; p0 += kRowGroups * kQ) {
-            if (!one_pass) bvals(j, p0, len, v);
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const int k = p0 + rg + kRowGroups * q;
-                const float e = (k < len && mx != kNegInf && v[q] != kNegInf) ? __expf(v[q] - mx) : 0.0f;
-                if (k < padded) sm.Bs[col][sm.boff[j] + k] = MF<T>::cvt(e);
-            }
-        }
+        *reinterpret_cast<T2*>(&sm.Bs[4 * q + c][2 * g]) = x;
     }
-    __syncthreads();
+}
 
-    // ---- one pipeline per wavefront over (tile, chunk) of all blocks
-    const int tpb = rows_per_block >> 5;
-    const int ntiles = nb * tpb;
-    auto nchunks = [&](int g) {
-        const int j = g / tpb;
-        return uni((sm.boff[j + 1] - sm.boff[j]) / kChunk);
-    };
-    auto advance = [&](Cursor u) {
-        if (u.g < ntiles && ++u.c >= nchunks(u.g)) {
-            u.g += kWaves;
-            u.c = 0;
-        }
-        return u;
-    };
-    auto load = [&](Cursor u, float* dst) {
-        if (u.g >= ntiles) return;
-        const int j = u.g / tpb, t = u.g - j * tpb;
-        const int row = 32 * t + mcol;
-        const int len = uni(sm.cnt[j]);
-        const int* s1 = sm.s1 + j * K;
-        if (len == 0) {   // a block without children / parents: its one chunk multiplies zeros
+// This wavefront's A operands: tiles w and w + 8 (those below ntile), n32 instructions each.
+template <typename T8>
+__device__ __forceinline__ void load_a(T8 (&a)[kTiles][kSteps], const T8* __restrict__ pa, int a0, int w, int ntile,
+                                       int n32, int lane) {
 #pragma unroll
-            for (int i = 0; i < kPerLane; ++i) dst[i] = 0.0f;
-            return;
-        }
+    for (int j = 0; j < kTiles; ++j) {
+        if (w + kWaves * j < ntile) {
+            const T8* src = pa + (int64_t)a0 + (int64_t)(w + kWaves * j) * n32 * 64 + lane;
 #pragma unroll
-        for (int s = 0; s < STEPS; ++s) {
-            const int k = u.c * kChunk + s * KS + KPL * h;
-            // Reductions beyond the block's length are loaded from clamped (valid, finite)
-            // parameters: their B operand is 0, so they add nothing.
-            if (KIND == 1) {
-                if (KPL > 1 && fast) {
-                    // one index per lane: the 8 children are a run (clamped to the last child)
-                    const int kc = k < len ? k : ((len - 1) & ~(KPL - 1));
-                    const int last = len - 1 - kc;
-                    const float* p = params + (int64_t)s1[kc > 0 ? kc : 0] + row;
-#pragma unroll
-                    for (int q = 0; q < KPL; ++q) dst[s * KPL + q] = p[(int64_t)(q < last ? q : (last > 0 ? last : 0)) * bs];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < KPL; ++q) {
-                        const int kk = k + q < len ? k + q : (len > 0 ? len - 1 : 0);
-                        dst[s * KPL + q] = params[(int64_t)s1[kk] + row];
-                    }
-                }
-            } else {
-                // KPL consecutive parent rows lie in one parent block (KPL divides bs): contiguous
-                const int kc = k < len ? k : (len >= KPL ? len - KPL : 0);
-                const float* p = params + (int64_t)s1[kc >> lbs] + (int64_t)row * bs + (kc & (bs - 1));
-                if (KPL == 8 && fast) {
-                    const float4 x0 = reinterpret_cast<const float4*>(p)[0];
-                    const float4 x1 = reinterpret_cast<const float4*>(p)[1];
-                    dst[s * KPL + 0] = x0.x;
-                    dst[s * KPL + 1] = x0.y;
-                    dst[s * KPL + 2] = x0.z;
-                    dst[s * KPL + 3] = x0.w;
-                    dst[s * KPL + 4] = x1.x;
-                    dst[s * KPL + 5] = x1.y;
-                    dst[s * KPL + 6] = x1.z;
-                    dst[s * KPL + 7] = x1.w;
-                } else {
-#pragma unroll
-                    for (int q = 0; q < KPL; ++q) dst[s * KPL + q] = p[q];
-                }
-            }
-        }
-    };
-
-    // One chunk of MFMAs from the A values in `cur`, then, at the end of a tile, its epilogue.
-    // Stores only: kind 2 parks the raw tile sums in the rows of the block's first child, and the
-    // pass after the pipeline turns them into the children's flows.
-    floatx16 acc = {0};
-    auto compute = [&](Cursor u, const float* cur) {
-        const int j = u.g / tpb, t = u.g - j * tpb;
-        const int kb = uni(sm.boff[j]) + u.c * kChunk;
-#pragma unroll
-        for (int s = 0; s < STEPS; ++s) {
-            T a[KPL];
-#pragma unroll
-            for (int q = 0; q < KPL; ++q) a[q] = MF<T>::cvt(cur[s * KPL + q]);
-            acc = MF<T>::mma(a, &sm.Bs[mcol][kb + s * KS + KPL * h], acc);
-        }
-        if (u.c + 1 == nchunks(u.g)) {
-            const int target = KIND == 1 ? uni(sm.nid[j]) : uni(sm.first[j] >> 1);
-            const bool write = KIND == 1 || uni(sm.first[j]) != 0;
-            const float mxm = sm.cmax[j][mcol];
-            if (bm < B && write) {
-                const int64_t base = (int64_t)target + 32 * t;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const float v = acc[r];
-                    float o = v;
-                    if (KIND == 1) o = (v > 0.0f && mxm != kNegInf) ? __logf(v) + mxm : kNegInf;
-                    out[(base + acc_row(r, lane)) * B + bm] = o;
-                }
-            }
-            acc = floatx16{0};
-        }
-    };
-
-    // Three rotating register buffers: a chunk's loads are issued three chunks before its MFMAs
-    // (at most 48 loads in flight per lane, within the hardware's count of 63), and no in-flight
-    // buffer is ever copied (a copy would wait for its loads).
-    Cursor ca = {w, 0};
-    Cursor cb = advance(ca);
-    Cursor cc = advance(cb);
-    float bufa[kPerLane], bufb[kPerLane], bufc[kPerLane];
-    load(ca, bufa);
-    load(cb, bufb);
-    load(cc, bufc);
-    while (ca.g < ntiles) {
-        compute(ca, bufa);
-        const Cursor cd = advance(cc);
-        load(cd, bufa);
-        if (cb.g >= ntiles) break;
-        compute(cb, bufb);
-        const Cursor ce = advance(cd);
-        load(ce, bufb);
-        if (cc.g >= ntiles) break;
-        compute(cc, bufc);
-        const Cursor cf = advance(ce);
-        load(cf, bufc);
-        ca = cd;
-        cb = ce;
-        cc = cf;
-    }
-
-    if (KIND == 2) {
-        // children's flows from the parked tile sums: v = sum * exp(e + column max), stored as
-        // log(v) - mar for the marked children
-        __syncthreads();
-        if (ok) {
-            constexpr int U = 8;
-            const int R = rows_per_block;
-            for (int j = 0; j < nb; ++j) {
-                const int first = uni(sm.first[j]);
-                if (first == 0) continue;
-                const float mx = sm.cmax[j][col];
-                const int64_t erow = op.eoff + uni(sm.nid[j]);
-                const int64_t park = first >> 1;
-                const int* ch = idx + op.e + (int64_t)j * op.Kc;
-                for (int r0 = rg; r0 < R; r0 += kRowGroups * U) {
-                    float v[U], e[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int r = r0 + u * kRowGroups;
-                        const int rr = r < R ? r : R - 1;
-                        e[u] = em[(erow + rr) * B + b];
-                        v[u] = out[(park + rr) * B + b];
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        v[u] = (mx == kNegInf || e[u] == kNegInf) ? 0.0f : v[u] * __expf(e[u] + mx);
-                    for (int q = 0; q < op.Kc; ++q) {
-                        const int code = ch[q];
-                        if (code == 0) continue;
-                        const int64_t cbase = code >> 1;
-                        if ((code & 1) && mod_children) {
-                            float mv[U];
-#pragma unroll
-                            for (int u = 0; u < U; ++u) {
-                                const int r = r0 + u * kRowGroups;
-                                mv[u] = nm[(cbase + (r < R ? r : R - 1)) * B + b];
-                            }
-#pragma unroll
-                            for (int u = 0; u < U; ++u) {
-                                const int r = r0 + u * kRowGroups;
-                                if (r < R) out[(cbase + r) * B + b] = (mv[u] == kNegInf) ? kNegInf : __logf(v[u]) - mv[u];
-                            }
-                        } else {
-#pragma unroll
-                            for (int u = 0; u < U; ++u) {
-                                const int r = r0 + u * kRowGroups;
-                                if (r < R) out[(cbase + r) * B + b] = v[u];
-                            }
-                        }
-                    }
-                }
-            }
+            for (int s = 0; s < kSteps; ++s)
+                if (s < n32) a[j][s] = src[s * 64];
         }
     }
 }
 
+// One 16 x 16 output tile: sum over n32 instructions of A x Bs, as two independent accumulation
+// chains so that consecutive matrix-core instructions do not wait for each other.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* __restrict__ nm, float* __restrict__ em,
-                                                             const float* __restrict__ params,
-                                                             const int64_t* __restrict__ ops, int n_ops,
-                                                             const int* __restrict__ idx, int64_t B) {
-    __shared__ __attribute__((aligned(16))) Smem<T> sm;
-    const int64_t b0 = (int64_t)blockIdx.x * kCols;
-    for (int o = 0; o < n_ops; ++o) {
-        const Op op = load_op(ops + (int64_t)o * PJAMD_CHAIN_OP_FIELDS);
-        if (op.kind == 0) {
-            chain_prod<T>(op, nm, em, idx, B, b0, sm);
-        } else if (op.kind == 1) {
-            chain_sum<T, 1>(op, nm, nullptr, nullptr, em, params, idx, B, b0, sm);
+__device__ __forceinline__ floatx4 tile_mma(const Smem<T>& sm, const typename MF16<T>::vec (&a)[kSteps], int n32,
+                                            int lane) {
+    typedef typename MF16<T>::vec T8;
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int col = lane & 15, kq = 8 * (lane >> 4);
+#pragma unroll
+    for (int s = 0; s < kSteps; s += 2) {
+        if (s < n32) acc0 = MF16<T>::mma(a[s], *reinterpret_cast<const T8*>(&sm.Bs[col][32 * s + kq]), acc0);
+        if (s + 1 < n32) acc1 = MF16<T>::mma(a[s + 1], *reinterpret_cast<const T8*>(&sm.Bs[col][32 * s + 32 + kq]), acc1);
+    }
+    return acc0 + acc1;
+}
+
+// The operation loop runs two operations per iteration with two register sets, so that the
+// independent loads of operation o + 1 are issued at the start of operation o (a whole
+// operation ahead of their use).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* em, const typename MF16<T>::vec* __restrict__ pa,
+                                                             const int* __restrict__ comp,
+                                                             const int* __restrict__ ops,
+                                                             const int* __restrict__ tabs, int C, int64_t B) {
+    typedef typename MF16<T>::vec T8;
+    __shared__ Smem<T> sm;
+    const int c = blockIdx.x % C;
+    const int64_t b0 = (int64_t)(blockIdx.x / C) * kCols;
+    const int nops = stage_component(sm, comp, ops, tabs, c);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = tid >> 2, q = tid & 3;
+    const int64_t cq = b0 + 4 * q;
+    const bool okq = cq < B;                       // B % 4 == 0: a quad is all in or all out
+    const int64_t cqc = okq ? cq : B - 4;
+    const int mcol = lane & 15;
+
+    // Product rows 2g, 2g + 1 of an operation: the sum of their children held in global memory
+    // (child 0 names the zero dummy rows, rows 0 .. cbs - 1 of node_mars).
+    auto global_children = [&](const Hdr& d, float4 (&gs)[2]) {
+        const int k = 2 * g < d.K ? 2 * g : 0;
+        const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
+        const int off = k & ((1 << d.lcbs) - 1);
+        gs[0] = gs[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kMaxKc; ++u) {
+            if (u < d.KC) {
+                const int ref = uni(sm.tab[ent + 1 + u]);
+                if (ref >= 0) {
+                    const int64_t ch = (int64_t)ref + off;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) add4(gs[r], *reinterpret_cast<const float4*>(nm + (ch + r) * B + cqc));
+                }
+            }
         }
-        __syncthreads();
+    };
+    auto prefetch = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2]) {
+        const Hdr d = header(sm, o);
+        load_a(a, pa, d.a0, w, d.R >> 4, d.K >> 5, lane);
+        if (!(d.flags & SYNC)) global_children(d, gs);
+    };
+    auto body = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2]) {
+        const Hdr d = header(sm, o);
+        const int n32 = d.K >> 5, ntile = d.R >> 4;
+        if (d.flags & SYNC) {
+            // children written by this component earlier in the run, read from global memory
+            sync_barrier();
+            global_children(d, gs);
+        }
+        const bool rok = 2 * g < d.K;
+        const int k = rok ? 2 * g : 0;
+        const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
+        const int off = k & ((1 << d.lcbs) - 1);
+        float4 v[2] = {gs[0], gs[1]};
+#pragma unroll
+        for (int u = 0; u < kMaxKc; ++u) {
+            if (u < d.KC) {
+                const int ref = uni(sm.tab[ent + 1 + u]);
+                if (ref < 0) {
+                    // rows of an earlier operation's output: ring slot, first position
+                    const int slot = (-ref - 1) >> 8, p0 = ((-ref - 1) & 255) + off;
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) add4(v[r], *reinterpret_cast<const float4*>(&sm.ring[slot][p0 + r][4 * q]));
+                }
+            }
+        }
+        float4 x[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) x[r] = (rok && okq) ? v[r] : ninf4();
+        float m[4], mm;
+        column_max(sm, x, q, d.K >> 5, mcol, m, mm);
+        if (rok) stage_b(sm, x, m, g, q);
+        lds_barrier();
+        float (*slot)[kRing] = sm.ring[o & 1];
+#pragma unroll
+        for (int j = 0; j < kTiles; ++j) {
+            const int t = w + kWaves * j;
+            if (t < ntile) {
+                const floatx4 acc = tile_mma(sm, a[j], n32, lane);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float y = acc[i];
+                    slot[16 * t + 4 * (lane >> 4) + i][mcol] = (y > 0.0f && mm != kNegInf) ? __logf(y) + mm : kNegInf;
+                }
+            }
+        }
+        lds_barrier();
+        // stores: the element rows, and node rows 2g, 2g + 1 (one parent block, bs >= 8) from the
+        // ring, 16 bytes per row
+        if (rok && okq) {
+            const int64_t erow = (int64_t)sm.tab[ent] + off;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) *reinterpret_cast<float4*>(em + (erow + r) * B + cq) = v[r];
+        }
+        if (2 * g < d.R && okq) {
+            const int p = 2 * g;
+            const int64_t row = (int64_t)sm.tab[d.nid_off + (p >> d.lbs)] + (p & ((1 << d.lbs) - 1));
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+                *reinterpret_cast<float4*>(nm + (row + r) * B + cq) = *reinterpret_cast<const float4*>(&slot[p + r][4 * q]);
+        }
+    };
+
+    T8 a0[kTiles][kSteps], a1[kTiles][kSteps];
+    float4 gs0[2], gs1[2];
+    if (nops > 0) prefetch(0, a0, gs0);
+    for (int o = 0; o < nops; o += 2) {
+        if (o + 1 < nops) prefetch(o + 1, a1, gs1);
+        body(o, a0, gs0);
+        if (o + 1 >= nops) break;
+        if (o + 2 < nops) prefetch(o + 2, a0, gs0);
+        body(o + 1, a1, gs1);
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void chain_bwd_kernel(float* nf, const float* __restrict__ nm,
                                                              const float* __restrict__ em,
-                                                             const float* __restrict__ params,
-                                                             const int64_t* __restrict__ ops, int n_ops,
-                                                             const int* __restrict__ idx, int64_t B) {
-    __shared__ __attribute__((aligned(16))) Smem<T> sm;
-    const int64_t b0 = (int64_t)blockIdx.x * kCols;
-    for (int o = 0; o < n_ops; ++o) {
-        const Op op = load_op(ops + (int64_t)o * PJAMD_CHAIN_OP_FIELDS);
-        if (op.kind == 2) {
-            chain_sum<T, 2>(op, nf, nf, nm, em, params, idx, B, b0, sm);
-        } else if (op.kind == 3) {
-            chain_modify<T>(op, nf, nm, idx, B, b0, sm);
+                                                             const typename MF16<T>::vec* __restrict__ pa,
+                                                             const int* __restrict__ comp,
+                                                             const int* __restrict__ ops,
+                                                             const int* __restrict__ tabs, int C, int64_t B) {
+    typedef typename MF16<T>::vec T8;
+    __shared__ Smem<T> sm;
+    const int c = blockIdx.x % C;
+    const int64_t b0 = (int64_t)(blockIdx.x / C) * kCols;
+    const int nops = stage_component(sm, comp, ops, tabs, c);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = tid >> 2, q = tid & 3;
+    const int64_t cq = b0 + 4 * q;
+    const bool okq = cq < B;
+    const int64_t cqc = okq ? cq : B - 4;
+    const int mcol = lane & 15;
+
+    // Loads of one operation that do not depend on the run: the A operands, the element rows of
+    // child rows 2g, 2g + 1, the parents' marginals and (flows from before the launch) flows.
+    struct Pre {
+        T8 a[kTiles][kSteps];
+        float4 e[2], mr[2], gf[2];
+    };
+    // parent rows 2g, 2g + 1 of an operation (inside one parent block) and their flow source:
+    // >= 0 a ring row (256 slot + first row; uniform per wavefront, bs >= 32), -1 node_flows
+    auto parent_row = [&](const Hdr& d, int64_t& prow, int& src) {
+        const int p = 2 * g < d.R ? 2 * g : 0;
+        const int pb = p >> d.lbs;
+        prow = (int64_t)sm.tab[d.nid_off + pb] + (p & ((1 << d.lbs) - 1));
+        src = sm.tab[d.nid_off + d.npb + pb];
+    };
+    auto load_flows = [&](int64_t prow, float4 (&gf)[2]) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) gf[r] = *reinterpret_cast<const float4*>(nf + (prow + r) * B + cqc);
+    };
+    auto prefetch = [&](int o, Pre& P) {
+        const Hdr d = header(sm, o);
+        load_a(P.a, pa, d.a0, w, d.K >> 4, d.R >> 5, lane);
+        const int k = 2 * g < d.K ? 2 * g : 0;
+        const int64_t erow = (int64_t)sm.tab[d.cb_off + (k >> d.lcbs) * (1 + d.KC)] + (k & ((1 << d.lcbs) - 1));
+#pragma unroll
+        for (int r = 0; r < 2; ++r) P.e[r] = *reinterpret_cast<const float4*>(em + (erow + r) * B + cqc);
+        int64_t prow;
+        int src;
+        parent_row(d, prow, src);
+        if (src >= 0 || !(d.flags & PAR_LFM)) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) P.mr[r] = *reinterpret_cast<const float4*>(nm + (prow + r) * B + cqc);
         }
-        __syncthreads();
+        if (src < 0 && !(d.flags & SYNC)) load_flows(prow, P.gf);
+    };
+    auto body = [&](int o, Pre& P) {
+        const Hdr d = header(sm, o);
+        const int n32 = d.R >> 5, ntile = d.K >> 4;
+        const int cbs = 1 << d.lcbs;
+        int64_t prow;
+        int src;
+        parent_row(d, prow, src);
+        if (d.flags & SYNC) {
+            // flows written by this component earlier in the run, read from global memory
+            sync_barrier();
+            if (src < 0) load_flows(prow, P.gf);
+        }
+        const bool rok = 2 * g < d.R;
+        // log(flow) - mar of parent rows 2g, 2g + 1, the flows from the ring (raw, a child block
+        // of an earlier operation) or from node_flows
+        float4 v[2];
+        if (src >= 0) {
+            const int rs = src >> 8, k0 = (src & 255) + ((2 * g) & ((1 << d.lbs) - 1));
+#pragma unroll
+            for (int r = 0; r < 2; ++r) v[r] = *reinterpret_cast<const float4*>(&sm.ring[rs][k0 + r][4 * q]);
+        } else {
+            v[0] = P.gf[0];
+            v[1] = P.gf[1];
+        }
+        const bool conv = src >= 0 || !(d.flags & PAR_LFM);
+        if (conv) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) v[r] = lfdm4(v[r], P.mr[r]);
+        }
+        // modified flows go back to node_flows: raw flows of a modified layer (PAR_WRITE), and
+        // ring-fed parents, whose producer stored them raw, whenever the pass modifies
+        const bool write_back = conv && (d.flags & (PAR_WRITE | (src >= 0 ? PAR_LFM : 0)));
+        float4 x[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) x[r] = (rok && okq) ? v[r] : ninf4();
+        float m[4], mm;
+        column_max(sm, x, q, d.R >> 5, mcol, m, mm);
+        if (rok) stage_b(sm, x, m, g, q);
+        lds_barrier();
+        float (*slot)[kRing] = sm.ring[o & 1];
+#pragma unroll
+        for (int j = 0; j < kTiles; ++j) {
+            const int t = w + kWaves * j;
+            if (t < ntile) {
+                const floatx4 acc = tile_mma(sm, P.a[j], n32, lane);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) slot[16 * t + 4 * (lane >> 4) + i][mcol] = acc[i];
+            }
+        }
+        lds_barrier();
+        // child rows 2g, 2g + 1: flow = sum x exp(element row + column max), kept raw in the ring
+        const bool kok = 2 * g < d.K;
+        const int k = kok ? 2 * g : 0;
+        const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
+        const int koff = k & (cbs - 1);
+        float4 f[2];
+        if (kok) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const float4 s4 = *reinterpret_cast<const float4*>(&slot[k + r][4 * q]);
+                const float4 e = P.e[r];
+                f[r].x = (m[0] == kNegInf || e.x == kNegInf) ? 0.0f : s4.x * __expf(e.x + m[0]);
+                f[r].y = (m[1] == kNegInf || e.y == kNegInf) ? 0.0f : s4.y * __expf(e.y + m[1]);
+                f[r].z = (m[2] == kNegInf || e.z == kNegInf) ? 0.0f : s4.z * __expf(e.z + m[2]);
+                f[r].w = (m[3] == kNegInf || e.w == kNegInf) ? 0.0f : s4.w * __expf(e.w + m[3]);
+                *reinterpret_cast<float4*>(&slot[k + r][4 * q]) = f[r];
+            }
+        }
+        lds_barrier();     // ring slot handed over
+        // stores
+        if (write_back && rok && okq) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) *reinterpret_cast<float4*>(nf + (prow + r) * B + cq) = v[r];
+        }
+        if (kok && okq) {
+            // product children (one parent each): code = node row << 2 | marked | ring-fed; a
+            // marked child read by no later operation of the run gets log(flow) - mar
+#pragma unroll
+            for (int u = 0; u < kMaxKc; ++u) {
+                const int code = u < d.KC ? uni(sm.tab[ent + 1 + u]) : 0;
+                if (code == 0) continue;
+                const int64_t cr = (int64_t)(code >> 2) + koff;
+                if ((code & 1) && !(code & 2) && (d.flags & CH_MOD)) {
+                    float4 mc[2];
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) mc[r] = *reinterpret_cast<const float4*>(nm + (cr + r) * B + cqc);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) *reinterpret_cast<float4*>(nf + (cr + r) * B + cq) = lfdm4(f[r], mc[r]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) *reinterpret_cast<float4*>(nf + (cr + r) * B + cq) = f[r];
+                }
+            }
+        }
+    };
+
+    Pre P0, P1;
+    if (nops > 0) prefetch(0, P0);
+    for (int o = 0; o < nops; o += 2) {
+        if (o + 1 < nops) prefetch(o + 1, P1);
+        body(o, P0);
+        if (o + 1 >= nops) break;
+        if (o + 2 < nops) prefetch(o + 2, P0);
+        body(o + 1, P1);
     }
+}
+
+// Packed A operands: workgroup i packs 128 units (16 bytes each) of group wg[3 i], direction
+// wg[3 i + 1] (0 forward, 1 backward), from unit wg[3 i + 2].
+//   forward  unit (t, s, lane) = theta[16 t + lane % 16][32 s + 8 (lane / 16) + q], q < 8
+//   backward unit (t, s, lane) = theta[32 s + 8 (lane / 16) + q][16 t + lane % 16]
+// with theta[p][c] = params[pbase[p / bs][c / cbs] + (c % cbs) bs + p % bs].
+template <typename T>
+__global__ __launch_bounds__(128) void chain_pack_kernel(typename MF16<T>::vec* __restrict__ out,
+                                                         const float* __restrict__ params,
+                                                         const int64_t* __restrict__ desc,
+                                                         const int64_t* __restrict__ pbase,
+                                                         const int* __restrict__ wg) {
+    typedef typename MF16<T>::vec T8;
+    const int gi = wg[3 * blockIdx.x], dir = wg[3 * blockIdx.x + 1];
+    const int64_t u = (int64_t)wg[3 * blockIdx.x + 2] + threadIdx.x;
+    const int64_t* d = desc + (int64_t)gi * PJAMD_CHAIN_DESC_INTS;
+    const int R = (int)d[0], K = (int)d[1], lbs = (int)d[2], lcbs = (int)d[3], ncb = (int)d[5];
+    const int bs = 1 << lbs, cbs = 1 << lcbs;
+    const int64_t* pb = pbase + d[8];
+    const int lane = (int)(u & 63);
+    const int64_t m = u >> 6;
+    T8 x;
+    if (dir == 0) {
+        const int n32 = K >> 5;
+        const int t = (int)(m / n32), s = (int)(m % n32);
+        const int pr = 16 * t + (lane & 15), c0 = 32 * s + 8 * (lane >> 4);
+        const int64_t src = pb[(int64_t)(pr >> lbs) * ncb + (c0 >> lcbs)] + (int64_t)(c0 & (cbs - 1)) * bs + (pr & (bs - 1));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = (T)params[src + (int64_t)i * bs];
+        out[d[6] + u] = x;
+    } else {
+        const int n32 = R >> 5;
+        const int t = (int)(m / n32), s = (int)(m % n32);
+        const int ch = 16 * t + (lane & 15), p0 = 32 * s + 8 * (lane >> 4);
+        const int64_t src = pb[(int64_t)(p0 >> lbs) * ncb + (ch >> lcbs)] + (int64_t)(ch & (cbs - 1)) * bs + (p0 & (bs - 1));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = (T)params[src + i];
+        out[d[7] + u] = x;
+    }
+}
+
+// Lane-layout check of the 16x16x32 instructions: C[16][16] = A[16][32] B[32][16].
+template <typename T>
+__global__ void mfma16_layout_probe(const float* A, const float* B, float* C) {
+    typedef typename MF16<T>::vec T8;
+    __shared__ __attribute__((aligned(16))) T As[16][32];
+    __shared__ __attribute__((aligned(16))) T Bt[16][32];
+    const int lane = threadIdx.x;
+    for (int i = lane; i < 16 * 32; i += 64) {
+        const int r = i / 32, k = i % 32;
+        As[r][k] = (T)A[r * 32 + k];
+        Bt[r][k] = (T)B[k * 16 + r];
+    }
+    __syncthreads();
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = MF16<T>::mma(*reinterpret_cast<const T8*>(&As[lane & 15][8 * (lane >> 4)]),
+                       *reinterpret_cast<const T8*>(&Bt[lane & 15][8 * (lane >> 4)]), acc);
+    for (int i = 0; i < 4; ++i) C[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
 }
 
 }  // namespace
 
-PJAMD_EXPORT int pjamd_chain_forward(float* node_mars, float* element_mars, const float* params,
-                                     const int64_t* ops, int n_ops, const int* idx, int64_t B, int precision,
-                                     hipStream_t stream) {
-    PJAMD_REQUIRE(n_ops >= 0 && B > 0 && precision >= 0 && precision <= 2);
-    if (n_ops == 0) return PJAMD_OK;
-    PJAMD_REQUIRE(node_mars && element_mars && params && ops && idx);
-    const int64_t g = (B + kCols - 1) / kCols;
+PJAMD_EXPORT int pjamd_chain_pack(void* packed, const float* params, const int64_t* desc, const int64_t* pbase,
+                                  const int* wg, int n_wg, int precision, pjamd_stream_t stream) {
+    PJAMD_REQUIRE(n_wg >= 0 && (precision == 1 || precision == 2));
+    if (n_wg == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(packed && params && desc && pbase && wg);
+    if (precision == 1)
+        chain_pack_kernel<__bf16><<<n_wg, 128, 0, stream>>>(reinterpret_cast<bf16x8*>(packed), params, desc, pbase, wg);
+    else
+        chain_pack_kernel<_Float16><<<n_wg, 128, 0, stream>>>(reinterpret_cast<halfx8*>(packed), params, desc, pbase,
+                                                              wg);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_chain_forward(float* node_mars, float* element_mars, const void* packed, const int* comp,
+                                     const int* ops, const int* tabs, int n_comp, int64_t B, int precision,
+                                     pjamd_stream_t stream) {
+    PJAMD_REQUIRE(n_comp >= 0 && B > 0 && B % 4 == 0 && (precision == 1 || precision == 2));
+    if (n_comp == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(node_mars && element_mars && packed && comp && ops && tabs);
+    const int64_t g = (B + kCols - 1) / kCols * n_comp;
     PJAMD_REQUIRE(g < ((int64_t)1 << 31));
     if (precision == 1)
-        chain_fwd_kernel<__bf16><<<(unsigned)g, kThreads, 0, stream>>>(node_mars, element_mars, params, ops, n_ops,
-                                                                       idx, B);
-    else if (precision == 2)
-        chain_fwd_kernel<_Float16><<<(unsigned)g, kThreads, 0, stream>>>(node_mars, element_mars, params, ops, n_ops,
-                                                                         idx, B);
+        chain_fwd_kernel<__bf16><<<(unsigned)g, kThreads, 0, stream>>>(
+            node_mars, element_mars, reinterpret_cast<const bf16x8*>(packed), comp, ops, tabs, n_comp, B);
     else
-        chain_fwd_kernel<float><<<(unsigned)g, kThreads, 0, stream>>>(node_mars, element_mars, params, ops, n_ops,
-                                                                      idx, B);
+        chain_fwd_kernel<_Float16><<<(unsigned)g, kThreads, 0, stream>>>(
+            node_mars, element_mars, reinterpret_cast<const halfx8*>(packed), comp, ops, tabs, n_comp, B);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
 
 PJAMD_EXPORT int pjamd_chain_backward(float* node_flows, const float* node_mars, const float* element_mars,
-                                      const float* params, const int64_t* ops, int n_ops, const int* idx, int64_t B,
-                                      int precision, hipStream_t stream) {
-    PJAMD_REQUIRE(n_ops >= 0 && B > 0 && precision >= 0 && precision <= 2);
-    if (n_ops == 0) return PJAMD_OK;
-    PJAMD_REQUIRE(node_flows && node_mars && element_mars && params && ops && idx);
-    const int64_t g = (B + kCols - 1) / kCols;
+                                      const void* packed, const int* comp, const int* ops, const int* tabs, int n_comp,
+                                      int64_t B, int precision, pjamd_stream_t stream) {
+    PJAMD_REQUIRE(n_comp >= 0 && B > 0 && B % 4 == 0 && (precision == 1 || precision == 2));
+    if (n_comp == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(node_flows && node_mars && element_mars && packed && comp && ops && tabs);
+    const int64_t g = (B + kCols - 1) / kCols * n_comp;
     PJAMD_REQUIRE(g < ((int64_t)1 << 31));
     if (precision == 1)
-        chain_bwd_kernel<__bf16><<<(unsigned)g, kThreads, 0, stream>>>(node_flows, node_mars, element_mars, params,
-                                                                       ops, n_ops, idx, B);
-    else if (precision == 2)
-        chain_bwd_kernel<_Float16><<<(unsigned)g, kThreads, 0, stream>>>(node_flows, node_mars, element_mars, params,
-                                                                         ops, n_ops, idx, B);
+        chain_bwd_kernel<__bf16><<<(unsigned)g, kThreads, 0, stream>>>(
+            node_flows, node_mars, element_mars, reinterpret_cast<const bf16x8*>(packed), comp, ops, tabs, n_comp, B);
     else
-        chain_bwd_kernel<float><<<(unsigned)g, kThreads, 0, stream>>>(node_flows, node_mars, element_mars, params,
-                                                                      ops, n_ops, idx, B);
+        chain_bwd_kernel<_Float16><<<(unsigned)g, kThreads, 0, stream>>>(
+            node_flows, node_mars, element_mars, reinterpret_cast<const halfx8*>(packed), comp, ops, tabs, n_comp, B);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_debug_mfma16_probe(int precision, const float* A, const float* B, float* C,
+                                          pjamd_stream_t stream) {
+    PJAMD_REQUIRE(A && B && C);
+    if (precision == 1) mfma16_layout_probe<__bf16><<<1, 64, 0, stream>>>(A, B, C);
+    else if (precision == 2) mfma16_layout_probe<_Float16><<<1, 64, 0, stream>>>(A, B, C);
+    else return PJAMD_EINVAL;
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }

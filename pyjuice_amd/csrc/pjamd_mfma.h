@@ -48,6 +48,30 @@ struct MF<float> {
     static __device__ __forceinline__ float cvt(float x) { return x; }
 };
 
+// 16 x 16 x 32 matrix-core instructions (BF16 / FP16 operands, FP32 accumulation): lane l
+// supplies A[l & 15][8 (l >> 4) + q] and B[8 (l >> 4) + q][l & 15], q < 8; accumulator register
+// i of lane l holds D[4 (l >> 4) + i][l & 15].
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MF16;
+
+template <>
+struct MF16<__bf16> {
+    typedef bf16x8 vec;
+    static __device__ __forceinline__ floatx4 mma(bf16x8 a, bf16x8 b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct MF16<_Float16> {
+    typedef halfx8 vec;
+    static __device__ __forceinline__ floatx4 mma(halfx8 a, halfx8 b, floatx4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
 // Row of accumulator register r for this lane inside a 32 x 32 tile (the column is lane & 31).
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 

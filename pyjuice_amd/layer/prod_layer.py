@@ -27,7 +27,7 @@ class ProdLayer(Layer, nn.Module):
                  disable_gpu_compilation: bool = False, force_gpu_compilation: bool = False):
         Layer.__init__(self, nodes)
         nn.Module.__init__(self)
-        assert len(self.nodes) > 0, "No input node."
+        assert len(self.nodes) > 0, "a product layer needs at least one node vector"
         assert len(self.nodes) == len(set(self.nodes)), "a vector appears twice in one layer"
 
         # prod_layer.py:39-46: block-sparse only if every vector is block-sparse with one block size.

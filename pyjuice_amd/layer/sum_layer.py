@@ -108,7 +108,7 @@ class SumLayer(Layer, nn.Module):
                  disable_gpu_compilation: bool = False, force_gpu_compilation: bool = False):
         Layer.__init__(self, nodes)
         nn.Module.__init__(self)
-        assert len(self.nodes) > 0, "No input node."
+        assert len(self.nodes) > 0, "a sum layer needs at least one node vector"
         assert len(self.nodes) == len(set(self.nodes)), "a vector appears twice in one layer"
 
         nid0, pid0, pfid0 = global_nid_start, global_pid_start, global_pfid_start

@@ -118,13 +118,12 @@ class TensorCircuit(nn.Module):
         self.modify_flows_in_place = True
         self._par_batches = {"tables": dict(), "layers": set()}
         self._fused = {"tables": dict(), "prod_groups": set(), "premod": set()}
-        # Runs of narrow layers in one launch each (pyjuice_amd.model.chain): on/off, and the
-        # largest sum-parameter footprint of a (product, sum) group pair that may join a run.
-        # Off by default: on HCLT-784 (256 latents) a run's single workgroup per 32 columns is
-        # bound by its loads in flight at HBM latency and loses to one tiled launch per layer.
+        # Runs of narrow levels in one launch per pass (pyjuice_amd.model.chain; BF16 / FP16
+        # operands, B % 4 == 0): planned per batch size, with a packed copy of the runs'
+        # parameters written at the start of every forward.
         self.use_chains = False
-        self.chain_max_param_bytes = 1 << 20
-        self._chain_cache = dict()
+        self._chain_cache, self._chain_packed = dict(), dict()
+        self._chain_packed = dict()
         # Layer-wise recomputation (the reference's own scheme, tensorcircuit.py:446-447): one
         # element buffer of the widest product group instead of one slab per group; the flow
         # backward recomputes each product group before the sum group above it, and parameter
@@ -200,7 +199,7 @@ class TensorCircuit(nn.Module):
                          for t, n, ch in self.parflow_fusing_kwargs],
             }
             self._build_launch_plans(device)
-        self._chain_cache = dict()
+        self._chain_cache, self._chain_packed = dict(), dict()
         return self
 
     def _build_launch_plans(self, device):
@@ -216,19 +215,33 @@ class TensorCircuit(nn.Module):
         self._fused = self._build_fused_backward(device)
         self._prod_fusion = self._build_prod_fusion(device)
         self._pf_overwrite_ok = self._sum_flows_overwritable()
-        self._chain_cache = dict()
+        self._chain_cache, self._chain_packed = dict(), dict()
 
-    def _chains(self, direction: str, precision: str, modify: bool = True, compute_param_flows: bool = True):
-        """Chain runs (``pyjuice_amd.model.chain.plan``) of one pass, cached per configuration."""
-        if not self.use_chains or self.recompute_elements or precision not in ("bf16", "fp16", "fp32"):
-            return dict()
-        key = (direction, "fp32" if precision == "fp32" else "16", modify, compute_param_flows,
-               self.chain_max_param_bytes)
-        runs = self._chain_cache.get(key)
-        if runs is None:
-            runs = _chain.plan(self, direction, precision, modify, compute_param_flows, self.chain_max_param_bytes)
-            self._chain_cache[key] = runs
-        return runs
+    def _chain_plan(self, precision: str, B: int):
+        """Chain runs (``pyjuice_amd.model.chain.plan``) for batches of ``B``, cached per batch
+        size; None where chains do not apply."""
+        if not self.use_chains or self.recompute_elements or precision not in ("bf16", "fp16") or B % 4 != 0:
+            return None
+        plan = self._chain_cache.get(B)
+        if plan is None:
+            plan = _chain.plan(self, self.device, B)
+            self._chain_cache[B] = plan
+        return plan if plan.runs else None
+
+    def _chain_pack(self, plan, precision: str):
+        """The packed operand copy of the runs' parameters (``pjamd_chain_pack``), refreshed from
+        ``params``; returns the buffer."""
+        buf = self._chain_packed.get((precision, id(plan)))
+        if buf is None:
+            self._chain_packed = {k: v for k, v in self._chain_packed.items() if k[1] == id(plan)}
+            dt = torch.bfloat16 if precision == "bf16" else torch.float16
+            buf = torch.empty([plan.pack_units * 8], dtype=dt, device=self.device)
+            self._chain_packed[(precision, id(plan))] = buf
+        n_wg = plan.pack_wg.size(0)
+        N.call("pjamd_chain_pack", N.ptr(buf), N.ptr(self.params), N.ptr(plan.pack_desc), N.ptr(plan.pack_pbase),
+               N.ptr(plan.pack_wg), n_wg, PRECISIONS[precision], N.stream(),
+               algo=(4 * plan.pack_units * 8 + 2 * plan.pack_units * 8, 0))
+        return buf
 
     def _mask_unreached_products(self):
         """A product block that no sum edge of the layer above points to (sparse block
@@ -722,9 +735,13 @@ class TensorCircuit(nn.Module):
                 and fz["tiles"][gid] * -(-B // 64) < self.fuse_products_max_tiles)
 
     def _run_inner_forward(self, force_use_bf16, force_use_fp32, precision):
-        runs = self._chains("fwd", precision)
         groups = self.inner_layer_groups
         B = self.node_mars.size(1)
+        plan = self._chain_plan(precision, B)
+        runs, packed = dict(), None
+        if plan is not None:
+            runs = plan.forward_runs()
+            packed = self._chain_pack(plan, precision)
         fz = self._prod_fusion
         if (not self.fuse_products or runs or self.recompute_elements or precision not in ("bf16", "fp16")
                 or B % 4 != 0):
@@ -733,9 +750,10 @@ class TensorCircuit(nn.Module):
         while gid < len(groups):
             run = runs.get(gid)
             if run is not None:
-                N.call("pjamd_chain_forward", N.ptr(self.node_mars), N.ptr(self.element_mars), N.ptr(self.params),
-                       N.ptr(run.ops), run.n_ops, N.ptr(run.idx), B, PRECISIONS[precision], N.stream(),
-                       algo=run.algo(B))
+                comp, ops, tab = run.fwd
+                N.call("pjamd_chain_forward", N.ptr(self.node_mars), N.ptr(self.element_mars), N.ptr(packed),
+                       N.ptr(comp), N.ptr(ops), N.ptr(tab), run.C, B, PRECISIONS[precision], N.stream(),
+                       algo=run.algo(B, "fwd"))
                 gid = run.hi
                 continue
             g = groups[gid]
@@ -836,15 +854,18 @@ class TensorCircuit(nn.Module):
         batched = self._par_batches["layers"] if (compute_param_flows and precision != "fp32_valu" and
                                                   not recompute) else set()
         fz = self._fused if precision != "fp32_valu" else {"tables": dict(), "prod_groups": set(), "premod": set()}
-        runs = self._chains("bwd", precision, modify, compute_param_flows) if not negate_pflows else dict()
+        B = self.node_flows.size(1)
+        plan = self._chain_plan(precision, B) if not negate_pflows else None
+        packed = self._chain_packed.get((precision, id(plan))) if plan is not None else None
+        runs = plan.backward_runs() if packed is not None else dict()
         lid = len(self.inner_layer_groups) - 1
         while lid >= 0:
             run = runs.get(lid)
             if run is not None:
-                B = self.node_flows.size(1)
+                comp, ops, tab = run.bwd[bool(modify)]
                 N.call("pjamd_chain_backward", N.ptr(self.node_flows), N.ptr(self.node_mars),
-                       N.ptr(self.element_mars), N.ptr(self.params), N.ptr(run.ops), run.n_ops, N.ptr(run.idx), B,
-                       PRECISIONS[precision], N.stream(), algo=run.algo(B))
+                       N.ptr(self.element_mars), N.ptr(packed), N.ptr(comp), N.ptr(ops), N.ptr(tab), run.C, B,
+                       PRECISIONS[precision], N.stream(), algo=run.algo(B, "bwd"))
                 lid = run.lo - 1
                 continue
             gid, lid = lid, lid - 1
@@ -1231,7 +1252,7 @@ class TensorCircuit(nn.Module):
             elif ns.is_prod():
                 stats["max_ele_bs"] = max(stats["max_ele_bs"], ns.block_size)
             else:
-                raise NotImplementedError(f"Unsupported node type {type(ns)}.")
+                raise NotImplementedError(f"cannot place a node vector of type {type(ns).__name__} in a layer")
 
         dfs_postorder(self.root_ns, finish)
 

@@ -82,4 +82,4 @@ class InputNodes(CircuitNodes):
         return self.dist.get_data_dtype()
 
     def __repr__(self):
-        return f"InputNodes(num_node_blocks={self.num_node_blocks}, block_size={self.block_size}, dist={self.dist})"
+        return f"<InputNodes {self.num_node_blocks}x{self.block_size} over {len(self.scope)} var(s), {type(self.dist).__name__}>"

@@ -41,7 +41,7 @@ class ProdNodes(CircuitNodes):
             return self.BLOCK_SPARSE
         if self.edge_ids.size(0) == self.num_nodes:
             return self.SPARSE
-        raise RuntimeError(f"Unexpected shape of `edge_ids`: {tuple(self.edge_ids.shape)}")
+        raise RuntimeError(f"product edge table must be [num_node_blocks, num_children], got {tuple(self.edge_ids.shape)}")
 
     def is_block_sparse(self):
         return self.edge_type == self.BLOCK_SPARSE
@@ -77,18 +77,18 @@ class ProdNodes(CircuitNodes):
         if edge_ids.size(0) == self.num_node_blocks:
             assert edge_ids.size(1) == self.num_chs, f"edge_ids must have shape ({self.num_node_blocks}, {self.num_chs}) for a block-sparse product"
             for i, c in enumerate(self.chs):
-                assert torch.all(edge_ids[:, i] >= 0), "Edge index underflow."
-                assert torch.all(edge_ids[:, i] < c.num_node_blocks), "Edge index overflow."
+                assert torch.all(edge_ids[:, i] >= 0), "negative child index in the edge table"
+                assert torch.all(edge_ids[:, i] < c.num_node_blocks), "child index in the edge table exceeds the child vector"
         elif edge_ids.size(0) == self.num_nodes:
             assert edge_ids.size(1) == self.num_chs, f"edge_ids must have shape ({self.num_nodes}, {self.num_chs}) for a sparse product"
             for i, c in enumerate(self.chs):
-                assert torch.all(edge_ids[:, i] >= 0), "Edge index underflow."
-                assert torch.all(edge_ids[:, i] < c.num_nodes), "Edge index overflow."
+                assert torch.all(edge_ids[:, i] >= 0), "negative child index in the edge table"
+                assert torch.all(edge_ids[:, i] < c.num_nodes), "child index in the edge table exceeds the child vector"
         else:
-            raise RuntimeError(f"Unexpected shape of `edge_ids`: {tuple(edge_ids.shape)}")
+            raise RuntimeError(f"product edge table must be [num_node_blocks, num_children], got {tuple(edge_ids.shape)}")
         self.edge_ids = edge_ids
 
     def __repr__(self):
-        t = "sparse" if self.edge_type == self.SPARSE else "block_sparse"
-        return (f"ProdNodes(num_node_blocks={self.num_node_blocks}, block_size={self.block_size}, "
-                f"num_chs={self.num_chs}, edge_type='{t}', scope_size={len(self.scope)})")
+        t = "sparse" if self.edge_type == self.SPARSE else "block-sparse"
+        return (f"<ProdNodes {self.num_node_blocks}x{self.block_size}, {t}, {self.num_chs} child vector(s), "
+                f"{len(self.scope)} var(s)>")

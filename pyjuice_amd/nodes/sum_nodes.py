@@ -86,7 +86,7 @@ class SumNodes(CircuitNodes):
             assert self.num_chs == len(chs), \
                 f"duplicate() got {len(chs)} children for a vector with {self.num_chs}"
             for o, n in zip(self.chs, chs):
-                assert type(o) == type(n), f"Child type not match: ({type(n)} != {type(o)})."
+                assert type(o) == type(n), f"replacement child is a {type(n).__name__}, the original a {type(o).__name__}"
                 assert o.num_node_blocks == n.num_node_blocks and o.block_size == n.block_size
         params = None
         if getattr(self, "_params", None) is not None and not tie_params:
@@ -244,13 +244,13 @@ class SumNodes(CircuitNodes):
             order = torch.argsort(edge_ids[0, :] * self.num_ch_node_blocks + edge_ids[1, :])
             edge_ids = edge_ids[:, order].contiguous()
         assert edge_ids.size(0) == 2, "sum edge_ids must have two rows (parent block, child block)"
-        assert torch.all(edge_ids >= 0), "Edge index underflow."
+        assert torch.all(edge_ids >= 0), "negative child index in the edge table"
         assert torch.all(edge_ids[0, :] < self.num_node_blocks) and \
-            torch.all(edge_ids[1, :] < self.num_ch_node_blocks), "Edge index overflow."
+            torch.all(edge_ids[1, :] < self.num_ch_node_blocks), "child index in the edge table exceeds the child vector"
         par = torch.unique(edge_ids[0, :])
-        assert par.size(0) == self.num_node_blocks and par.max() == self.num_node_blocks - 1, "Some node has no edge."
+        assert par.size(0) == self.num_node_blocks and par.max() == self.num_node_blocks - 1, "every parent node block needs at least one edge"
         self.edge_ids = edge_ids
 
     def __repr__(self):
-        return (f"SumNodes(num_node_blocks={self.num_node_blocks}, block_size={self.block_size}, "
-                f"num_chs={self.num_chs}, num_edges={self.num_edges}, scope_size={len(self.scope)})")
+        return (f"<SumNodes {self.num_node_blocks}x{self.block_size}, {self.num_edges} edge(s) into "
+                f"{self.num_chs} child vector(s), {len(self.scope)} var(s)>")
