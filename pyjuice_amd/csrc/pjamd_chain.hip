@@ -15,11 +15,11 @@
 //
 // Hand-over between operations: the outputs of the last two operations stay in an LDS ring
 // (node rows forward, element flows backward) and the planner points the next operations' reads
-// there; barriers between operations cover LDS only, so global stores stay in flight.  An
-// operation that must read rows of its component from global memory starts with a full barrier
-// (flag SYNC).  Software pipeline: the loads of operation o + 1 that do not depend on the run
-// (packed operands, rows computed before the launch) are issued at the start of operation o,
-// into a second register set.
+// there; every other row an operation reads was written before the launch (the planner keeps
+// no run that would read rows of its own component from global memory).  Barriers between
+// operations therefore cover LDS only and global stores stay in flight.  Software pipeline: the
+// loads of operation o + 1 (packed operands, rows computed before the launch) are issued at the
+// start of operation o, into a second register set.
 //
 // The index tables of a component (PJAMD_CHAIN_TAB_INTS entries at most) and its operation
 // headers are staged in LDS once, at kernel start.
@@ -45,7 +45,7 @@ constexpr int kRing = kCols + 4;                     // LDS row stride of the ri
 static_assert(kCols == 16 && kThreads / 4 * 2 == kMaxRows, "thread layout: 2 rows x 4 columns per thread");
 
 enum { F_R = 0, F_K, F_LBS, F_LCBS, F_NPB, F_NCB, F_KC, F_NID, F_CB, F_A, F_FLAGS };
-enum { PAR_LFM = 1, PAR_WRITE = 2, CH_MOD = 4, SYNC = 8 };
+enum { PAR_LFM = 1, PAR_WRITE = 2, CH_MOD = 4 };
 
 template <typename T>
 struct Smem {
@@ -58,11 +58,8 @@ struct Smem {
     __attribute__((aligned(16))) float ring[2][kMaxRows][kRing];
 };
 
-// Barrier over the workgroup's LDS traffic only (global stores stay in flight), and the full one.
+// Barrier over the workgroup's LDS traffic only: global stores stay in flight.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-__device__ __forceinline__ void sync_barrier() {
-    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -170,17 +167,19 @@ __device__ __forceinline__ void stage_b(Smem<T>& sm, const float4 (&v)[2], const
     }
 }
 
-// This wavefront's A operands: tiles w and w + 8 (those below ntile), n32 instructions each.
+// This wavefront's A operands: tiles w and w + 8, n32 instructions each.  Every load is issued
+// (branch-free: a tile at or beyond ntile, or a step at or beyond n32, reloads the operation's
+// first unit and is never used), so that the waits for them can be counted exactly.
 template <typename T8>
 __device__ __forceinline__ void load_a(T8 (&a)[kTiles][kSteps], const T8* __restrict__ pa, int a0, int w, int ntile,
                                        int n32, int lane) {
 #pragma unroll
     for (int j = 0; j < kTiles; ++j) {
-        if (w + kWaves * j < ntile) {
-            const T8* src = pa + (int64_t)a0 + (int64_t)(w + kWaves * j) * n32 * 64 + lane;
+        const int t = w + kWaves * j;
 #pragma unroll
-            for (int s = 0; s < kSteps; ++s)
-                if (s < n32) a[j][s] = src[s * 64];
+        for (int s = 0; s < kSteps; ++s) {
+            const int unit = (t < ntile && s < n32) ? (t * n32 + s) * 64 : 0;
+            a[j][s] = pa[(int64_t)a0 + unit + lane];
         }
     }
 }
@@ -221,43 +220,36 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
     const int64_t cqc = okq ? cq : B - 4;
     const int mcol = lane & 15;
 
-    // Product rows 2g, 2g + 1 of an operation: the sum of their children held in global memory
-    // (child 0 names the zero dummy rows, rows 0 .. cbs - 1 of node_mars).
-    auto global_children = [&](const Hdr& d, float4 (&gs)[2]) {
+    // Product rows 2g, 2g + 1 of an operation: the rows of their first two children (the planner
+    // lists children in global memory first), loaded unconditionally and kept raw until use so
+    // that nothing waits for them at issue; a ring child or no child reads the zero dummy rows
+    // (rows 0 .. cbs - 1 of node_mars).
+    auto global_children = [&](const Hdr& d, float4 (&gs)[2][2]) {
         const int k = 2 * g < d.K ? 2 * g : 0;
         const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
         const int off = k & ((1 << d.lcbs) - 1);
-        gs[0] = gs[1] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int u = 0; u < kMaxKc; ++u) {
-            if (u < d.KC) {
-                const int ref = uni(sm.tab[ent + 1 + u]);
-                if (ref >= 0) {
-                    const int64_t ch = (int64_t)ref + off;
+        for (int u = 0; u < 2; ++u) {
+            const int ref = u < d.KC ? max(uni(sm.tab[ent + 1 + u]), 0) : 0;
 #pragma unroll
-                    for (int r = 0; r < 2; ++r) add4(gs[r], *reinterpret_cast<const float4*>(nm + (ch + r) * B + cqc));
-                }
-            }
+            for (int r = 0; r < 2; ++r) gs[u][r] = *reinterpret_cast<const float4*>(nm + ((int64_t)ref + off + r) * B + cqc);
         }
     };
-    auto prefetch = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2]) {
+    auto prefetch = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2][2]) {
         const Hdr d = header(sm, o);
         load_a(a, pa, d.a0, w, d.R >> 4, d.K >> 5, lane);
-        if (!(d.flags & SYNC)) global_children(d, gs);
+        global_children(d, gs);
     };
-    auto body = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2]) {
+    auto body = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2][2]) {
         const Hdr d = header(sm, o);
         const int n32 = d.K >> 5, ntile = d.R >> 4;
-        if (d.flags & SYNC) {
-            // children written by this component earlier in the run, read from global memory
-            sync_barrier();
-            global_children(d, gs);
-        }
         const bool rok = 2 * g < d.K;
         const int k = rok ? 2 * g : 0;
         const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
         const int off = k & ((1 << d.lcbs) - 1);
-        float4 v[2] = {gs[0], gs[1]};
+        float4 v[2] = {gs[0][0], gs[0][1]};
+        add4(v[0], gs[1][0]);
+        add4(v[1], gs[1][1]);
 #pragma unroll
         for (int u = 0; u < kMaxKc; ++u) {
             if (u < d.KC) {
@@ -267,6 +259,11 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
                     const int slot = (-ref - 1) >> 8, p0 = ((-ref - 1) & 255) + off;
 #pragma unroll
                     for (int r = 0; r < 2; ++r) add4(v[r], *reinterpret_cast<const float4*>(&sm.ring[slot][p0 + r][4 * q]));
+                } else if (u >= 2 && ref > 0) {
+                    // a third or fourth child in global memory (not prefetched)
+#pragma unroll
+                    for (int r = 0; r < 2; ++r)
+                        add4(v[r], *reinterpret_cast<const float4*>(nm + ((int64_t)ref + off + r) * B + cqc));
                 }
             }
         }
@@ -308,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
     };
 
     T8 a0[kTiles][kSteps], a1[kTiles][kSteps];
-    float4 gs0[2], gs1[2];
+    float4 gs0[2][2], gs1[2][2];
     if (nops > 0) prefetch(0, a0, gs0);
     for (int o = 0; o < nops; o += 2) {
         if (o + 1 < nops) prefetch(o + 1, a1, gs1);
@@ -366,11 +363,11 @@ __global__ __launch_bounds__(kThreads) void chain_bwd_kernel(float* nf, const fl
         int64_t prow;
         int src;
         parent_row(d, prow, src);
-        if (src >= 0 || !(d.flags & PAR_LFM)) {
+        // the marginals unconditionally (nearly every operation uses them); the flows only where
+        // they come from global memory (src is uniform per wavefront)
 #pragma unroll
-            for (int r = 0; r < 2; ++r) P.mr[r] = *reinterpret_cast<const float4*>(nm + (prow + r) * B + cqc);
-        }
-        if (src < 0 && !(d.flags & SYNC)) load_flows(prow, P.gf);
+        for (int r = 0; r < 2; ++r) P.mr[r] = *reinterpret_cast<const float4*>(nm + (prow + r) * B + cqc);
+        if (src < 0) load_flows(prow, P.gf);
     };
     auto body = [&](int o, Pre& P) {
         const Hdr d = header(sm, o);
@@ -379,11 +376,6 @@ __global__ __launch_bounds__(kThreads) void chain_bwd_kernel(float* nf, const fl
         int64_t prow;
         int src;
         parent_row(d, prow, src);
-        if (d.flags & SYNC) {
-            // flows written by this component earlier in the run, read from global memory
-            sync_barrier();
-            if (src < 0) load_flows(prow, P.gf);
-        }
         const bool rok = 2 * g < d.R;
         // log(flow) - mar of parent rows 2g, 2g + 1, the flows from the ring (raw, a child block
         // of an earlier operation) or from node_flows

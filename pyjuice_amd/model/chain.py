@@ -59,7 +59,7 @@ PACK_WG = 128            # units (16 bytes each) packed per workgroup of pjamd_c
 PAR_LFM = 1      # the parents' node flows already hold log(flow) - mar
 PAR_WRITE = 2    # the parents' node flows are raw: store log(flow) - mar back (modify in place)
 CH_MOD = 4       # children marked in their code (bit 0) receive log(flow) - mar
-SYNC = 8         # (both passes) the operation reads rows of this component from global memory
+SYNC = 8         # planner only: the operation would read rows of its component from global memory
 
 # cost model (microseconds per training step, forward + backward) used to choose the runs,
 # calibrated on HCLT-784 (256 latents, batch 512) on one MI355X
@@ -369,6 +369,8 @@ def plan(pc, device, B: int, min_levels: int = 2) -> Plan:
         chosen.extend(reversed(picked))
         i = j
 
+    # runs whose components would read their own rows back from global memory stay layer-wise
+    chosen = [(a, b, comps) for a, b, comps in chosen if not _needs_sync(comps)]
     out = Plan()
     unit = 0
     for a, b, comps in chosen:
@@ -389,6 +391,14 @@ def plan(pc, device, B: int, min_levels: int = 2) -> Plan:
     return out
 
 
+def _needs_sync(comps: List[List[Group]]) -> bool:
+    for direction in ("fwd", "bwd"):
+        _c, ops, _t = _tables(comps, direction, lambda g: 0, lambda x: 4 * x, torch.device("cpu"))
+        if bool(((ops[:, F_FLAGS] & SYNC) != 0).any()):
+            return True
+    return False
+
+
 def _i32(rows, width: int, device) -> torch.Tensor:
     a = np.asarray(rows, dtype=np.int64).reshape(-1, width) if rows else np.zeros([0, width], dtype=np.int64)
     assert a.size == 0 or (a.min() >= -2 ** 31 and a.max() < 2 ** 31)
@@ -407,8 +417,8 @@ def _tables(comps: List[List[Group]], direction: str, flags_of, code_of, device)
     first row`` in its source entry (after the parent blocks' node rows; -1 = node flows in
     global memory); a child code is ``node row << 2 | marked | 2 if every row of the child is
     read through the ring`` (its producer then stores the flows raw and the consumer stores them
-    back as log(flow) - mar).  An operation that reads in-run rows from global memory instead is
-    flagged SYNC (its loads wait for the component's earlier stores)."""
+    back as log(flow) - mar).  An operation that would have to read in-run rows from global
+    memory gets the flag SYNC, which the kernels do not implement: ``plan`` drops such runs."""
     comp_rows, op_rows, tab = [], [], []
     for c in comps:
         seq = sorted(c, key=lambda g: g.level, reverse=(direction == "bwd"))
@@ -449,16 +459,20 @@ def _tables(comps: List[List[Group]], direction: str, flags_of, code_of, device)
             cb_off = len(tab) - tab0
             for b, (e, ch) in enumerate(zip(g.erow, g.ch)):
                 tab.append(e)
-                for x in ch:
-                    if direction == "fwd":
+                if direction == "fwd":
+                    # global-memory children first (the kernel prefetches the first two)
+                    refs = []
+                    for x in ch:
                         src = source(x, g.cbs)
                         if src is not None and src[1] is not None and o - src[0][2] in (1, 2):
-                            tab.append(-(1 + ((src[0][2] & 1) << 8) + src[1]))
+                            refs.append(-(1 + ((src[0][2] & 1) << 8) + src[1]))
                             continue
                         if src is not None:
                             flags |= SYNC
-                        tab.append(code_of(x))
-                    else:
+                        refs.append(code_of(x))
+                    tab.extend(sorted(refs, key=lambda r: r < 0))
+                else:
+                    for x in ch:
                         made.append([x, g.cbs, o, b * g.cbs, len(tab), 0])
                         tab.append(code_of(x))
                 tab.extend([0] * (KC - len(ch)))
