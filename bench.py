@@ -71,8 +71,9 @@ def parse():
     ap.add_argument("--prune", type=float, default=0.0,
                     help="fraction of sum-edge blocks removed at seeded random (seed 4) after the structure is "
                          "built, e.g. 0.5 with --latents 512 --block-size 32 --batch 2048")
-    ap.add_argument("--chains", action="store_true",
-                    help="run eligible stretches of layers as chain kernels (pjamd_chain_forward/_backward)")
+    ap.add_argument("--chains", default="auto", choices=["auto", "on", "off"],
+                    help="runs of narrow levels in one launch per pass (pjamd_chain_forward / _backward): "
+                         "auto = measured choice (with the block size for HCLT), on, off")
     ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
     ap.add_argument("--no-graph", dest="graph", action="store_false",
@@ -120,9 +121,9 @@ def build_circuit(args, dev):
         args.block_size, args.block_size_ms = 0, None
         t0 = time.time()
         pc = juice.compile(ns, verbose=False)
-        pc.use_chains = bool(args.chains)
         pc.to(dev)
         log(f"[bench] compiled {args.structure}: {pc.num_nodes} nodes, {pc.num_edges} edges in {time.time() - t0:.1f}s")
+        choose_chains(pc, xb, args)
         return ns, pc, xb, data[args.fit:args.fit + args.batch]
 
     def make_nodes(bs):
@@ -134,25 +135,51 @@ def build_circuit(args, dev):
         return ns
 
     args.block_size_ms = None
+    tuned_chains = None
     if args.block_size < 0:
         from pyjuice_amd import autotune
         t0 = time.time()
         cands = autotune.block_size_candidates(args.latents)
         prec = "bf16" if args.precision == "auto" else args.precision
-        best, ms = autotune.tune_block_size(make_nodes, xb, cands, steps=3, warmup=1,
-                                            precision=None if prec == "auto" else prec)
-        args.block_size, args.block_size_ms = best, {str(k): round(v, 4) for k, v in ms.items()}
-        log(f"[bench] block size {best} chosen from {args.block_size_ms} ms/step in {time.time() - t0:.1f}s")
+        opts = {"auto": (False, True), "on": (True,), "off": (False,)}[args.chains]
+        (best, tuned_chains), ms = autotune.tune_build(make_nodes, xb, cands, opts, steps=3, warmup=1,
+                                                       precision=None if prec == "auto" else prec)
+        args.block_size = best
+        args.block_size_ms = {f"{k[0]}{'+chains' if k[1] else ''}": round(v, 4) for k, v in ms.items()}
+        log(f"[bench] block size {best}, chains {'on' if tuned_chains else 'off'}, chosen from "
+            f"{args.block_size_ms} ms/step in {time.time() - t0:.1f}s")
     t0 = time.time()
     ns = make_nodes(args.block_size or None)
     log(f"[bench] HCLT structure in {time.time() - t0:.1f}s")
     t0 = time.time()
     pc = juice.compile(ns, verbose=False)
-    pc.use_chains = bool(args.chains)
     pc.to(dev)
     log(f"[bench] compiled: {pc.num_nodes} nodes, {pc.num_edges} edges, {pc.num_sum_params} sum params, "
         f"{len(pc.inner_layer_groups)} layer groups in {time.time() - t0:.1f}s")
+    if tuned_chains is not None:
+        pc.use_chains = tuned_chains
+        args.chains_ms = None
+    else:
+        choose_chains(pc, xb, args)
     return ns, pc, xb, data[args.fit:args.fit + args.batch]
+
+
+def choose_chains(pc, xb, args):
+    """``pc.use_chains`` from ``--chains`` (auto: measured on recorded steps, kept when faster)."""
+    args.chains_ms = None
+    if args.chains != "auto":
+        pc.use_chains = args.chains == "on"
+        return
+    from pyjuice_amd import autotune
+    prec = "bf16" if args.precision == "auto" else args.precision
+    if prec not in ("bf16", "fp16"):
+        pc.use_chains = False
+        return
+    t0 = time.time()
+    ms = autotune.tune_chains(pc, xb, prec, steps=3, warmup=1)
+    args.chains_ms = {("on" if k else "off"): round(v, 4) for k, v in ms.items()}
+    log(f"[bench] chains {'on' if pc.use_chains else 'off'} chosen from {args.chains_ms} ms/step in "
+        f"{time.time() - t0:.1f}s")
 
 
 def make_step(pc, xb, prec, graph):
@@ -216,13 +243,23 @@ def kernel_breakdown(step, steps):
     return s
 
 
+def chain_report(pc, args):
+    """Whether chain runs were on, their measured alternative and, when on, the runs planned at
+    the bench batch (levels, components)."""
+    out = {"on": bool(pc.use_chains), "chains_ms_per_step": getattr(args, "chains_ms", None)}
+    plan = pc._chain_plan("bf16", args.batch) if pc.use_chains else None
+    if plan is not None:
+        out["runs"] = [{"levels": (r.hi - r.lo) // 2, "components": r.C} for r in plan.runs]
+    return out
+
+
 def roofline_of(name, d, prec, args):
     """Roofline of one kernel from its algorithmic bytes and flops per step: the bound is the
     resource whose time at peak is larger (the sum-layer contractions at B = 512 and block size
     256 carry ~50 flop/byte, below the BF16 ridge point of ~310, so they are HBM-bound too)."""
     peak_f = PEAK_TFLOPS[prec]
     headline = (args.structure == "hclt" and args.latents == 256 and args.batch == 512
-                and args.block_size in TRAFFIC_MB_PER_LAUNCH and prec == "bf16" and args.prune == 0 and not args.chains)
+                and args.block_size in TRAFFIC_MB_PER_LAUNCH and prec == "bf16" and args.prune == 0)
     traffic = TRAFFIC_MB_PER_LAUNCH[args.block_size].get(name) if headline else None
     traffic = None if traffic is None else round(traffic * 1e6)   # HBM bytes per launch
     t_hbm = d["bytes"] / (HBM_PEAK_GBS * 1e9)
@@ -463,7 +500,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name(args),
                    "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
-                   "precision": main_prec, "hip_graph": bool(args.graph), "chain_kernels": bool(args.chains),
+                   "precision": main_prec, "hip_graph": bool(args.graph), "chain_runs": chain_report(pc, args),
                    "block_size_ms_per_step": args.block_size_ms},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -1,12 +1,13 @@
-"""Measured choices of a circuit's block size and of each sum layer's kernel launch.
+"""Measured choices of a circuit's block size, of its chain runs and of each sum layer's kernel
+launch.
 
 A structure builder's ``block_size`` only regroups the same nodes (an HCLT with L latents is the
 same L x L sum layers at any block size dividing L), but it changes every kernel's tiling: the
 number of node blocks per layer, which tile shapes and fast paths apply, and how many
 workgroups a narrow layer launches.  The best value depends on the circuit, the batch and the
 precision, so it is measured: each candidate is built, compiled and run for a few recorded
-training steps (``TrainStepGraph``: clear flows, forward, flow backward, EM), and the fastest is
-kept.  The reference fixes ``block_size = num_latents`` for HCLT unless the caller passes one
+training steps (``TrainStepGraph``: clear flows, forward, flow backward, EM), with the chain runs
+of ``pyjuice_amd.model.chain`` off and on, and the fastest pair is kept.  The reference fixes ``block_size = num_latents`` for HCLT unless the caller passes one
 (``structures/compilation.py:39-40``).
 
 Per layer (``tune_launches``), the launch form of each sum layer's forward and fused element-flow
@@ -24,7 +25,8 @@ from typing import Callable, Dict, Iterable, List, Optional, Tuple
 
 import torch
 
-__all__ = ["block_size_candidates", "tune_block_size", "tune_launches", "launch_summary"]
+__all__ = ["block_size_candidates", "tune_build", "tune_block_size", "tune_chains", "tune_launches",
+           "launch_summary"]
 
 
 def block_size_candidates(num_latents: int, smallest: int = 64) -> List[int]:
@@ -50,31 +52,64 @@ def _time_steps(step: Callable[[], object], steps: int, warmup: int) -> float:
     return a.elapsed_time(b) / steps
 
 
-def tune_block_size(make_nodes: Callable[[int], object], x: torch.Tensor, candidates: Iterable[int],
-                    steps: int = 5, warmup: int = 2, precision: Optional[str] = None, lr: float = 0.1,
-                    pseudocount: float = 0.1, margin: float = 0.03) -> Tuple[int, Dict[int, float]]:
-    """``make_nodes(block_size)`` builds the root nodes of one candidate; ``x [B, V]`` (on the
-    device) is the batch the steps run on.  Returns the chosen block size and the measured
-    milliseconds per training step of every candidate.  A candidate replaces an earlier one only
-    when it is faster by more than ``margin`` (relative), so timing noise does not flip the
-    choice away from the first (the builder's default when listed first)."""
+def tune_build(make_nodes: Callable[[int], object], x: torch.Tensor, candidates: Iterable[int],
+               chain_options: Iterable[bool] = (False,), steps: int = 5, warmup: int = 2,
+               precision: Optional[str] = None, lr: float = 0.1, pseudocount: float = 0.1,
+               margin: float = 0.03) -> Tuple[Tuple[int, bool], Dict[Tuple[int, bool], float]]:
+    """Measured choice of the block size and of the chain runs (``TensorCircuit.use_chains``):
+    ``make_nodes(block_size)`` builds the root nodes of one candidate; ``x [B, V]`` (on the
+    device) is the batch the recorded training steps run on.  Returns the chosen
+    ``(block_size, use_chains)`` and the milliseconds per step of every pair.  A pair replaces
+    an earlier one only when faster by more than ``margin`` (relative), so timing noise does not
+    flip the choice away from the first (the builder's default block size, chains off)."""
     import pyjuice_amd as juice
     from pyjuice_amd.optim import TrainStepGraph
 
-    times: Dict[int, float] = dict()
+    times: Dict[Tuple[int, bool], float] = dict()
+    chain_options = list(chain_options)
     for bs in candidates:
         pc = juice.compile(make_nodes(bs), verbose=False)
         pc.to(x.device)
-        g = TrainStepGraph(pc, x, lr=lr, pseudocount=pseudocount, precision=precision)
-        times[bs] = _time_steps(g, steps, warmup)
-        del g, pc
+        for chains in chain_options:
+            pc.use_chains = bool(chains)
+            g = TrainStepGraph(pc, x, lr=lr, pseudocount=pseudocount, precision=precision)
+            times[(bs, bool(chains))] = _time_steps(g, steps, warmup)
+            del g
+        del pc
         gc.collect()
         torch.cuda.empty_cache()
     best = None
-    for bs, t in times.items():
+    for k, t in times.items():
         if best is None or t < times[best] * (1.0 - margin):
-            best = bs
+            best = k
     return best, times
+
+
+def tune_block_size(make_nodes: Callable[[int], object], x: torch.Tensor, candidates: Iterable[int],
+                    steps: int = 5, warmup: int = 2, precision: Optional[str] = None, lr: float = 0.1,
+                    pseudocount: float = 0.1, margin: float = 0.03) -> Tuple[int, Dict[int, float]]:
+    """``tune_build`` over block sizes alone (layer by layer, no chain runs): the chosen block
+    size and the milliseconds per step of every candidate."""
+    best, times = tune_build(make_nodes, x, candidates, (False,), steps=steps, warmup=warmup, precision=precision,
+                             lr=lr, pseudocount=pseudocount, margin=margin)
+    return best[0], {bs: t for (bs, _c), t in times.items()}
+
+
+def tune_chains(pc, x: torch.Tensor, precision: Optional[str] = None, steps: int = 5, warmup: int = 2,
+                lr: float = 0.1, pseudocount: float = 0.1, margin: float = 0.03) -> Dict[bool, float]:
+    """Measured choice of ``pc.use_chains`` (recorded training steps on ``x``, chains off first,
+    kept unless chains are faster by more than ``margin``); returns the milliseconds per step of
+    both settings."""
+    from pyjuice_amd.optim import TrainStepGraph
+
+    times: Dict[bool, float] = dict()
+    for chains in (False, True):
+        pc.use_chains = chains
+        g = TrainStepGraph(pc, x, lr=lr, pseudocount=pseudocount, precision=precision)
+        times[chains] = _time_steps(g, steps, warmup)
+        del g
+    pc.use_chains = times[True] < times[False] * (1.0 - margin)
+    return times
 
 
 # (pass, layer shape, batch, precision) -> chosen launch form (None = the library's default rule)
