@@ -75,6 +75,9 @@ def parse():
                     help="runs of narrow levels in one launch per pass (pjamd_chain_forward / _backward): "
                          "auto = measured choice (with the block size for HCLT), on, off")
     ap.add_argument("--fit", type=int, default=2048, help="samples used to learn the Chow-Liu tree")
+    ap.add_argument("--pflow-dtype", default="fp64", choices=["fp32", "fp64"],
+                    help="parameter-flow accumulators (sum and input layers): fp64 (the library's default, for "
+                         "accumulating over many mini-batches) or fp32 as PyJuice keeps them")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "fp32_valu", "bf16", "fp16"])
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch each step's kernels from the host instead of replaying the step's HIP graph")
@@ -120,7 +123,7 @@ def build_circuit(args, dev):
                                       homogeneous=True)
         args.block_size, args.block_size_ms = 0, None
         t0 = time.time()
-        pc = juice.compile(ns, verbose=False)
+        pc = juice.compile(ns, verbose=False, pflow_dtype=pflow_dtype(args))
         pc.to(dev)
         log(f"[bench] compiled {args.structure}: {pc.num_nodes} nodes, {pc.num_edges} edges in {time.time() - t0:.1f}s")
         choose_chains(pc, xb, args)
@@ -143,7 +146,8 @@ def build_circuit(args, dev):
         prec = "bf16" if args.precision == "auto" else args.precision
         opts = {"auto": (False, True), "on": (True,), "off": (False,)}[args.chains]
         (best, tuned_chains), ms = autotune.tune_build(make_nodes, xb, cands, opts, steps=3, warmup=1,
-                                                       precision=None if prec == "auto" else prec)
+                                                       precision=None if prec == "auto" else prec,
+                                                       compile_kwargs={"pflow_dtype": pflow_dtype(args)})
         args.block_size = best
         args.block_size_ms = {f"{k[0]}{'+chains' if k[1] else ''}": round(v, 4) for k, v in ms.items()}
         log(f"[bench] block size {best}, chains {'on' if tuned_chains else 'off'}, chosen from "
@@ -152,7 +156,7 @@ def build_circuit(args, dev):
     ns = make_nodes(args.block_size or None)
     log(f"[bench] HCLT structure in {time.time() - t0:.1f}s")
     t0 = time.time()
-    pc = juice.compile(ns, verbose=False)
+    pc = juice.compile(ns, verbose=False, pflow_dtype=pflow_dtype(args))
     pc.to(dev)
     log(f"[bench] compiled: {pc.num_nodes} nodes, {pc.num_edges} edges, {pc.num_sum_params} sum params, "
         f"{len(pc.inner_layer_groups)} layer groups in {time.time() - t0:.1f}s")
@@ -162,6 +166,11 @@ def build_circuit(args, dev):
     else:
         choose_chains(pc, xb, args)
     return ns, pc, xb, data[args.fit:args.fit + args.batch]
+
+
+def pflow_dtype(args):
+    import torch
+    return {"fp32": torch.float32, "fp64": torch.float64}[args.pflow_dtype]
 
 
 def choose_chains(pc, xb, args):
@@ -500,7 +509,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name(args),
                    "num_nodes": pc.num_nodes, "num_edges": pc.num_edges, "num_sum_params": pc.num_sum_params,
-                   "precision": main_prec, "hip_graph": bool(args.graph), "chain_runs": chain_report(pc, args),
+                   "precision": main_prec, "param_flow_accumulator": args.pflow_dtype, "hip_graph": bool(args.graph),
+                   "chain_runs": chain_report(pc, args),
                    "block_size_ms_per_step": args.block_size_ms},
         "roofline": roof,
         "cpu_baseline": cpu,

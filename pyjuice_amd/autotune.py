@@ -55,20 +55,22 @@ def _time_steps(step: Callable[[], object], steps: int, warmup: int) -> float:
 def tune_build(make_nodes: Callable[[int], object], x: torch.Tensor, candidates: Iterable[int],
                chain_options: Iterable[bool] = (False,), steps: int = 5, warmup: int = 2,
                precision: Optional[str] = None, lr: float = 0.1, pseudocount: float = 0.1,
-               margin: float = 0.03) -> Tuple[Tuple[int, bool], Dict[Tuple[int, bool], float]]:
+               margin: float = 0.03, compile_kwargs: Optional[dict] = None
+               ) -> Tuple[Tuple[int, bool], Dict[Tuple[int, bool], float]]:
     """Measured choice of the block size and of the chain runs (``TensorCircuit.use_chains``):
     ``make_nodes(block_size)`` builds the root nodes of one candidate; ``x [B, V]`` (on the
     device) is the batch the recorded training steps run on.  Returns the chosen
     ``(block_size, use_chains)`` and the milliseconds per step of every pair.  A pair replaces
     an earlier one only when faster by more than ``margin`` (relative), so timing noise does not
-    flip the choice away from the first (the builder's default block size, chains off)."""
+    flip the choice away from the first (the builder's default block size, chains off).
+    ``compile_kwargs`` go to ``juice.compile`` (e.g. ``pflow_dtype``)."""
     import pyjuice_amd as juice
     from pyjuice_amd.optim import TrainStepGraph
 
     times: Dict[Tuple[int, bool], float] = dict()
     chain_options = list(chain_options)
     for bs in candidates:
-        pc = juice.compile(make_nodes(bs), verbose=False)
+        pc = juice.compile(make_nodes(bs), verbose=False, **(compile_kwargs or {}))
         pc.to(x.device)
         for chains in chain_options:
             pc.use_chains = bool(chains)
