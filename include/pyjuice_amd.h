@@ -381,8 +381,9 @@ int pjamd_sum_backward_par_alg(void* param_flows, int pf64, const float* node_fl
  * tile_desc[t] = p for each of the n_tiles tiles.  copies = c > 1 makes rows p+1 .. p+c-1 tied
  * copies of row p (same K, block count, n_chs, pids and pfids; their own node blocks, child
  * rows and element rows): row p's tiles sum the products of all c copies before writing each
- * slot once, and the copy rows get no tiles of their own.  tile_rows is 64 (block sizes >= 64, child
- * tiles of 64) or 32 (block size 32, child tiles of 128).  element_mars is the buffer that
+ * slot once, and the copy rows get no tiles of their own.  tile_rows is 128 (block sizes and child
+ * counts >= 128, child tiles of 128), 64 (block sizes >= 64, child tiles of 64) or 32 (block size
+ * 32, child tiles of 128).  element_mars is the buffer that
  * holds every layer's element rows.  flags as pjamd_sum_backward_par (bit 1 unused; bit 4 applies
  * to the partitions without the shared-slot flag).
  * Same per-partition results as pjamd_sum_backward_par; the reference runs one launch per

@@ -158,10 +158,11 @@ __device__ __forceinline__ void stage_b(Smem<T>& sm, const float4 (&v)[2], const
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         T2 x;
+        const float ns = neg_shift2(m[c]);
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const float e = c == 0 ? v[r].x : c == 1 ? v[r].y : c == 2 ? v[r].z : v[r].w;
-            x[r] = (T)(m[c] == kNegInf ? 0.0f : __expf(e - m[c]));
+            x[r] = (T)op_exp2(fmaf(e, kLog2e, ns));
         }
         *reinterpret_cast<T2*>(&sm.Bs[4 * q + c][2 * g]) = x;
     }

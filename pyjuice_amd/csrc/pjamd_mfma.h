@@ -72,6 +72,20 @@ struct MF16<_Float16> {
     }
 };
 
+// Matrix-core operands exp(x - s) are formed as exp2(fma(x, log2 e, -s log2 e)) with the bare
+// v_exp_f32 (two VALU instructions instead of expf's denormal-range rescaling, which costs five
+// more).  Results below 2^-126 flush to zero; an operand tile is scaled so that its largest entry
+// is at most about 1, and BF16 / FP16 rounding of the tile's sum is far coarser than 2^-126.
+// A shift s is floored at kShiftFloor, so that a column whose entries are all -inf gives
+// exp2(-inf) = 0 instead of exp(-inf - (-inf)) = NaN, with no compare per element.
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kShiftFloor = -1.0e30f;
+
+__device__ __forceinline__ float op_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// -max(s, kShiftFloor) log2 e: the addend of exp2(fma(x, log2 e, addend)) = exp(x - s).
+__device__ __forceinline__ float neg_shift2(float s) { return -fmaxf(s, kShiftFloor) * kLog2e; }
+
 // Row of accumulator register r for this lane inside a 32 x 32 tile (the column is lane & 31).
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 

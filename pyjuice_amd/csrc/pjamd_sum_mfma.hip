@@ -115,8 +115,9 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
         }
         __syncthreads();
         const float mn = mrun[bcol];
+        const float ns = neg_shift2(mn);   // exp(ev - mn); 0 for -inf entries
 #pragma unroll
-        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(ev[q] - mn));
+        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(op_exp2(fmaf(ev[q], kLog2e, ns)));
 #pragma unroll
         for (int q = 0; q < PA; ++q) As[arow][ak0 + q] = MF<T>::cvt(av[q]);
         const float s = msc[mycol];
@@ -297,8 +298,9 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
         }
         __syncthreads();
         const float mn = mrun[bcol];
+        const float ns = neg_shift2(mn);   // exp(lv - mn); 0 for -inf entries
 #pragma unroll
-        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(mn == kNegInf ? 0.0f : __expf(lv[q] - mn));
+        for (int q = 0; q < PB; ++q) Bs[bcol][bk0 + q] = MF<T>::cvt(op_exp2(fmaf(lv[q], kLog2e, ns)));
 #pragma unroll
         for (int q = 0; q < PA; ++q) As[ar0 + q * RA][ak] = MF<T>::cvt(av[q]);
         const float s = msc[mycol];
@@ -497,10 +499,11 @@ __device__ __forceinline__ void vec_stage_b(const float4 (&v)[kVecRows], const f
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         float e[8];
+        const float ns = neg_shift2(mn[c]);   // exp(x - max); 0 for -inf entries
 #pragma unroll
         for (int r = 0; r < kVecRows; ++r) {
             const float x = c == 0 ? v[r].x : c == 1 ? v[r].y : c == 2 ? v[r].z : v[r].w;
-            e[r] = mn[c] == kNegInf ? 0.0f : __expf(x - mn[c]);
+            e[r] = op_exp2(fmaf(x, kLog2e, ns));
         }
         Pack8<T>::store(&Bs[4 * q + c][8 * g], e);
     }
@@ -1222,7 +1225,7 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ no
             // B tile: Bs[4 q + c][RPG g + r] = exp(ev[r][c] - max[c])
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const float mn = mrun[4 * q + c];
+                const float ns = neg_shift2(mrun[4 * q + c]);   // exp(x - max); 0 for -inf
 #pragma unroll
                 for (int h = 0; h < RPG / 8; ++h) {
                     float e[8];
@@ -1230,7 +1233,7 @@ __global__ __launch_bounds__(256) void sum_fwd_vec_blocks(float* __restrict__ no
                     for (int r = 0; r < 8; ++r) {
                         const float4 v = ev[8 * h + r];
                         const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
-                        e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                        e[r] = op_exp2(fmaf(x, kLog2e, ns));
                     }
                     Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
                 }
@@ -1390,7 +1393,7 @@ __global__ __launch_bounds__(256) void sum_ele_vec_blocks(float* __restrict__ ef
         // B tile: Bs[4 q + c][RPG g + r] = exp(lv[r][c] - max[c])
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float mn = mrun[4 * q + c];
+            const float ns = neg_shift2(mrun[4 * q + c]);   // exp(x - max); 0 for -inf
 #pragma unroll
             for (int h = 0; h < RPG / 8; ++h) {
                 float e[8];
@@ -1398,7 +1401,7 @@ __global__ __launch_bounds__(256) void sum_ele_vec_blocks(float* __restrict__ ef
                 for (int r = 0; r < 8; ++r) {
                     const float4 v = lv[8 * h + r];
                     const float x = c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
-                    e[r] = mn == kNegInf ? 0.0f : __expf(x - mn);
+                    e[r] = op_exp2(fmaf(x, kLog2e, ns));
                 }
                 Pack8<T>::store(&Bs[4 * q + c][RPG * g + 8 * h], e);
             }
@@ -1553,20 +1556,20 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
                 float v = xred[0][tid];
 #pragma unroll
                 for (int g = 1; g < RGV; ++g) v = fmaxf(v, xred[g][tid]);
-                sh[tid] = v;
+                sh[tid] = neg_shift2(v);
             }
             __syncthreads();
             float s4[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) s4[c] = sh[4 * xq + c];
             typedef T T4 __attribute__((ext_vector_type(4)));
+            // -inf entries (padding, zero flows, impossible children) give exp2(-inf) = 0.
 #pragma unroll
             for (int q = 0; q < PXV; ++q) {
                 const float x[4] = {xv[q].x, xv[q].y, xv[q].z, xv[q].w};
                 T4 o;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    o[c] = MF<T>::cvt((s4[c] == kNegInf || x[c] == kNegInf) ? 0.0f : __expf(x[c] - s4[c]));
+                for (int c = 0; c < 4; ++c) o[c] = MF<T>::cvt(op_exp2(fmaf(x[c], kLog2e, s4[c])));
                 *reinterpret_cast<T4*>(&As[rg + q * RGV][4 * xq]) = o;
             }
 #pragma unroll
@@ -1574,8 +1577,7 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
                 const float y[4] = {yv[q].x, yv[q].y, yv[q].z, yv[q].w};
                 T4 o;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    o[c] = MF<T>::cvt((s4[c] == kNegInf || y[c] == kNegInf) ? 0.0f : __expf(y[c] + s4[c]));
+                for (int c = 0; c < 4; ++c) o[c] = MF<T>::cvt(op_exp2(fmaf(y[c], kLog2e, -s4[c])));
                 *reinterpret_cast<T4*>(&Bs[rg + q * RGV][4 * xq]) = o;
             }
             __syncthreads();
@@ -1614,13 +1616,11 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
             sh[tid] = v;
         }
         __syncthreads();
-        const float s = sh[xb];
+        const float s = neg_shift2(sh[xb]);   // exp(x - max), exp(y + max); 0 for -inf entries
 #pragma unroll
-        for (int q = 0; q < PX; ++q)
-            As[r0 + q * RG][xb] = MF<T>::cvt((s == kNegInf || xv[q] == kNegInf) ? 0.0f : __expf(xv[q] - s));
+        for (int q = 0; q < PX; ++q) As[r0 + q * RG][xb] = MF<T>::cvt(op_exp2(fmaf(xv[q], kLog2e, s)));
 #pragma unroll
-        for (int q = 0; q < PY; ++q)
-            Bs[r0 + q * RG][xb] = MF<T>::cvt((s == kNegInf || yv[q] == kNegInf) ? 0.0f : __expf(yv[q] + s));
+        for (int q = 0; q < PY; ++q) Bs[r0 + q * RG][xb] = MF<T>::cvt(op_exp2(fmaf(yv[q], kLog2e, -s)));
         __syncthreads();
         if (b0 + TK < bend) load(b0 + TK);
         acc = stage_mma<T, LD>(As, Bs, wm, wn, lane, acc);
@@ -1638,6 +1638,244 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
             const float v = sign * params[p0 + i] * acc[r];
             pf_add<PF>(&pflows[f0 + i], (PF)v, atomic_out);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 128 x 128 parameter-flow tiles (BF16 / FP16 operands, B % 4 == 0, block sizes >= 128).  A tile
+// moves (TM + TN) B floats from L2 for TM TN B products, so 128 x 128 tiles halve both the
+// L2 -> CU traffic of 64 x 64 tiles (the limit of that form: a CU keeps a bounded number of L2
+// requests in flight) and the exp work per parameter.  (256 x 128 tiles, one workgroup of eight
+// wavefronts per CU, measured slower: nothing overlaps a workgroup's barriers.)  Four
+// wavefronts, each a 64 x 64 quadrant of four 32 x 32 accumulators, stages of TKB = 64 batch
+// columns; thread (rg = tid / 16, xq = tid % 16) loads rows rg + 16 q (q < 8) of both operand
+// tiles, columns 4 xq .. 4 xq + 3.  The column max is reduced inside each wavefront by lane
+// shuffles and across the four through LDS.  The epilogue stages each quadrant through LDS so
+// that the slots of one child (consecutive rows) are read and written as contiguous runs.
+constexpr int TKB = 64;
+
+struct BigTile {
+    static constexpr int TM = 128, TN = 128, NW = 4, NT = 64 * NW, RG = NT / 16;
+    static constexpr int NQX = TM / RG, NQY = TN / RG;
+    static constexpr int LD = TKB + 8;    // operand row: 72 BF16 / FP16 values
+    static constexpr int SLD = 64 + 4;    // epilogue staging: 64 rows (+ pad) per child, floats
+    static constexpr int OPS_BYTES = (TM + TN) * LD * 2;
+    static_assert(NW * 32 * SLD * 4 <= OPS_BYTES, "epilogue staging fits the operand tiles");
+};
+
+template <typename T, typename PF>
+__device__ __forceinline__ void par_tile_big(PF* __restrict__ pflows, const float* __restrict__ nflows,
+                                             const float* __restrict__ nmars, const float* __restrict__ emars,
+                                             const float* __restrict__ params, const int* __restrict__ nids,
+                                             const int* __restrict__ cids, const int* __restrict__ pids,
+                                             const int* __restrict__ pfids, const int* __restrict__ nch, int K,
+                                             int64_t B, int j, int m0, int kt0, bool modified, int mode, float sign,
+                                             const int* __restrict__ copies, int n_copies) {
+    typedef BigTile G;
+    constexpr int TM = G::TM, TN = G::TN, LD = G::LD, NW = G::NW, RG = G::RG, NQX = G::NQX, NQY = G::NQY;
+    constexpr int SLD = G::SLD;
+    static_assert(sizeof(T) == 2 && MF<T>::PAD == 8, "16-bit operands");
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    __shared__ __attribute__((aligned(16))) char ops_raw[G::OPS_BYTES];
+    __shared__ __attribute__((aligned(16))) float red[NW][TKB];
+    __shared__ int64_t sy[TN];
+    __shared__ int spid[TN], spf[TN];
+    T* ops = reinterpret_cast<T*>(ops_raw);
+    T (*As)[LD] = reinterpret_cast<T (*)[LD]>(ops);
+    T (*Bs)[LD] = reinterpret_cast<T (*)[LD]>(ops + TM * LD);
+
+    const int n = nch[j];
+    if (kt0 >= n) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const int rg = tid >> 4, xq = tid & 15;
+    if (tid < TN) {
+        const int k = kt0 + tid < n ? kt0 + tid : kt0;
+        spid[tid] = pids[(int64_t)j * K + k];
+        spf[tid] = pfids[(int64_t)j * K + k];
+    }
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = floatx16{0};
+    const float4 ninf = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+
+    // Tied copies (rows copies[r * PJAMD_PAR_DESC_FIELDS]) add into the same accumulators.
+    for (int r = 0; r < n_copies; ++r) {
+        int64_t nrow0 = (int64_t)nids[j] + m0;
+        const int* crow = cids + (int64_t)j * K;
+        const float* em = emars;
+        if (r > 0) {
+            const int* R = copies + (int64_t)r * PJAMD_PAR_DESC_FIELDS;
+            nrow0 = (int64_t)nids[(int64_t)R[4] - copies[4] + j] + m0;
+            crow = cids + ((int64_t)R[5] - copies[5]) + (int64_t)j * K;
+            em = emars + ((int64_t)R[6] - copies[6]) * B;
+        }
+        __syncthreads();   // the previous copy's child rows are no longer read
+        // Padding children (k >= n) read child kt0's row: their columns are never written.
+        if (tid < TN) sy[tid] = (int64_t)crow[kt0 + tid < n ? kt0 + tid : kt0] * B;
+        __syncthreads();
+        const float* xrow = nflows + nrow0 * B;
+        const float* mrow = nmars + nrow0 * B;
+        float4 xv[NQX], yv[NQY];
+        auto load = [&](int64_t b0) {
+            const int64_t b = b0 + 4 * xq;
+            const int64_t bc = b < B ? b : B - 4;   // clamped: the stage's tail columns are masked
+#pragma unroll
+            for (int q = 0; q < NQX; ++q) {
+                const int64_t o = (int64_t)(rg + RG * q) * B + bc;
+                float4 f = *reinterpret_cast<const float4*>(xrow + o);
+                if (!modified) {
+                    const float4 m = *reinterpret_cast<const float4*>(mrow + o);
+                    f.x = m.x == kNegInf ? kNegInf : __logf(f.x) - m.x;
+                    f.y = m.y == kNegInf ? kNegInf : __logf(f.y) - m.y;
+                    f.z = m.z == kNegInf ? kNegInf : __logf(f.z) - m.z;
+                    f.w = m.w == kNegInf ? kNegInf : __logf(f.w) - m.w;
+                }
+                xv[q] = f;
+            }
+#pragma unroll
+            for (int q = 0; q < NQY; ++q) yv[q] = *reinterpret_cast<const float4*>(em + sy[rg + RG * q] + bc);
+        };
+        load(0);
+        for (int64_t b0 = 0; b0 < B; b0 += TKB) {
+            // Columns past B: -inf node flows give a column max of -inf, hence zero operands on
+            // both sides (the child side through the floored shift).
+            if (b0 + 4 * xq >= B) {
+#pragma unroll
+                for (int q = 0; q < NQX; ++q) xv[q] = ninf;
+            }
+            float m4[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+            for (int q = 0; q < NQX; ++q) {
+                m4[0] = fmaxf(m4[0], xv[q].x);
+                m4[1] = fmaxf(m4[1], xv[q].y);
+                m4[2] = fmaxf(m4[2], xv[q].z);
+                m4[3] = fmaxf(m4[3], xv[q].w);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                m4[c] = fmaxf(m4[c], __shfl_xor(m4[c], 16, 64));
+                m4[c] = fmaxf(m4[c], __shfl_xor(m4[c], 32, 64));
+            }
+            if (lane < 16) *reinterpret_cast<float4*>(&red[w][4 * xq]) = make_float4(m4[0], m4[1], m4[2], m4[3]);
+            __syncthreads();
+            float s4[4];
+            {
+                float4 mx = *reinterpret_cast<const float4*>(&red[0][4 * xq]);
+#pragma unroll
+                for (int v = 1; v < NW; ++v) {
+                    const float4 r = *reinterpret_cast<const float4*>(&red[v][4 * xq]);
+                    mx = make_float4(fmaxf(mx.x, r.x), fmaxf(mx.y, r.y), fmaxf(mx.z, r.z), fmaxf(mx.w, r.w));
+                }
+                s4[0] = neg_shift2(mx.x);
+                s4[1] = neg_shift2(mx.y);
+                s4[2] = neg_shift2(mx.z);
+                s4[3] = neg_shift2(mx.w);
+            }
+#pragma unroll
+            for (int q = 0; q < NQX; ++q) {
+                const float x[4] = {xv[q].x, xv[q].y, xv[q].z, xv[q].w};
+                T4 ox;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) ox[c] = MF<T>::cvt(op_exp2(fmaf(x[c], kLog2e, s4[c])));
+                *reinterpret_cast<T4*>(&As[rg + RG * q][4 * xq]) = ox;
+            }
+#pragma unroll
+            for (int q = 0; q < NQY; ++q) {
+                const float y[4] = {yv[q].x, yv[q].y, yv[q].z, yv[q].w};
+                T4 oy;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) oy[c] = MF<T>::cvt(op_exp2(fmaf(y[c], kLog2e, -s4[c])));
+                *reinterpret_cast<T4*>(&Bs[rg + RG * q][4 * xq]) = oy;
+            }
+            __syncthreads();
+            if (b0 + TKB < B) load(b0 + TKB);
+#pragma unroll
+            for (int kk = 0; kk < TKB; kk += 2 * MF<T>::KPL) {
+                const int kc = kk + MF<T>::KPL * (lane >> 5);
+                const T* a0 = &As[wm * 64 + (lane & 31)][kc];
+                const T* a1 = &As[wm * 64 + 32 + (lane & 31)][kc];
+                const T* c0 = &Bs[wn * 64 + (lane & 31)][kc];
+                const T* c1 = &Bs[wn * 64 + 32 + (lane & 31)][kc];
+                acc[0][0] = MF<T>::mma(a0, c0, acc[0][0]);
+                acc[0][1] = MF<T>::mma(a0, c1, acc[0][1]);
+                acc[1][0] = MF<T>::mma(a1, c0, acc[1][0]);
+                acc[1][1] = MF<T>::mma(a1, c1, acc[1][1]);
+            }
+        }
+    }
+
+    // Epilogue: wavefront w stages its 64 rows x 32 children at a time as [child][row] floats;
+    // lane (it, lane) then owns child c = 2 it + (lane >> 5) of the half, rows 2 (lane & 31) + {0, 1}.
+    // The quadrant's parameters are loaded first, all at once (padding children read child
+    // kt0's, which are not used), so that the epilogue waits for memory once.
+    float2 pv[2][16];
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj)
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int kl = wn * 64 + sj * 32 + 2 * it + (lane >> 5);
+            const int64_t p = (int64_t)spid[kl] + m0 + wm * 64 + 2 * (lane & 31);
+            pv[sj][it] = make_float2(params[p], params[p + 1]);
+        }
+    __syncthreads();   // every wavefront is done reading the operand tiles
+    float* stg = reinterpret_cast<float*>(ops) + w * 32 * SLD;
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj) {
+#pragma unroll
+        for (int si = 0; si < 2; ++si)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                *reinterpret_cast<float4*>(&stg[(lane & 31) * SLD + si * 32 + 8 * g + 4 * (lane >> 5)]) =
+                    make_float4(acc[si][sj][4 * g], acc[si][sj][4 * g + 1], acc[si][sj][4 * g + 2],
+                                acc[si][sj][4 * g + 3]);
+        __syncthreads();
+        // slot offset and product pair of iteration it
+        auto item = [&](int it, int64_t& f, float2& v) {
+            const int c = 2 * it + (lane >> 5);
+            const int kl = wn * 64 + sj * 32 + c;
+            const float2 a = *reinterpret_cast<const float2*>(&stg[c * SLD + 2 * (lane & 31)]);
+            v = make_float2(sign * pv[sj][it].x * a.x, sign * pv[sj][it].y * a.y);
+            f = (int64_t)spf[kl] + m0 + wm * 64 + 2 * (lane & 31);
+            return kt0 + kl < n;
+        };
+        if (mode == kPfOverwrite || mode == kPfAtomic) {
+#pragma unroll
+            for (int it = 0; it < 16; ++it) {
+                int64_t f;
+                float2 v;
+                if (item(it, f, v)) {
+                    pf_add<PF>(&pflows[f], (PF)v.x, mode);
+                    pf_add<PF>(&pflows[f + 1], (PF)v.y, mode);
+                }
+            }
+        } else {
+            // read-modify-write in two batches of eight iterations (all loads, then all stores)
+#pragma unroll
+            for (int h = 0; h < 16; h += 8) {
+                PF o[8][2];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    int64_t f;
+                    float2 v;
+                    const bool ok = item(h + u, f, v);
+                    o[u][0] = ok ? pflows[f] : (PF)0;
+                    o[u][1] = ok ? pflows[f + 1] : (PF)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    int64_t f;
+                    float2 v;
+                    if (item(h + u, f, v)) {
+                        pflows[f] = o[u][0] + (PF)v.x;
+                        pflows[f + 1] = o[u][1] + (PF)v.y;
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1664,8 +1902,10 @@ __global__ __launch_bounds__(256) void sum_par_mfma(PF* __restrict__ pflows, con
 
 // Batched launch over the partitions of many sum layers (one descriptor row per partition, see
 // pjamd_sum_backward_par_batched).  The tile order inside a partition is m-tile fastest, then
-// child tile, then node block; each tile reduces over the whole batch.
-template <typename T, int TM, int TN, typename PF, bool VEC>
+// child tile, then node block; each tile reduces over the whole batch.  QUAD: the descriptor
+// tiles are 128 x 128 and workgroup 4 t + s computes the 64 x 64 quarter s of tile t (the form
+// for FP32 operands or B % 4 != 0 under a 128-row table).
+template <typename T, int TM, int TN, typename PF, bool VEC, bool QUAD = false>
 __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, const float* __restrict__ nflows,
                                                        const float* __restrict__ nmars,
                                                        const float* __restrict__ emars,
@@ -1675,18 +1915,42 @@ __global__ __launch_bounds__(256) void sum_par_batched(PF* __restrict__ pflows, 
                                                        const int* __restrict__ desc,
                                                        const int* __restrict__ tile_desc, int64_t B, bool modified,
                                                        bool overwrite, float sign) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    static_assert(!QUAD || (TM == 64 && TN == 64), "quarters of 128 x 128 tiles are 64 x 64");
+    const int tq = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = QUAD ? tq >> 2 : tq;
     const int* D = desc + (int64_t)tile_desc[t] * PJAMD_PAR_DESC_FIELDS;
     const int local = t - D[0];
     const int K = D[1], mtiles = D[2], ktiles = D[3];
     const int64_t bo = D[4], io = D[5];
-    const int m0 = (local % mtiles) * TM;
-    const int kt0 = ((local / mtiles) % ktiles) * TN;
+    constexpr int SM = QUAD ? 2 * TM : TM, SN = QUAD ? 2 * TN : TN;
+    const int m0 = (local % mtiles) * SM + (QUAD ? (tq & 1) * TM : 0);
+    const int kt0 = ((local / mtiles) % ktiles) * SN + (QUAD ? ((tq >> 1) & 1) * TN : 0);
     const int j = local / (mtiles * ktiles);
     const int mode = D[7] ? kPfAtomic : (overwrite ? kPfOverwrite : kPfAdd);
     par_tile<T, TM, TN, PF, VEC>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io,
                                  pids + io, pfids + io, nch + bo, K, B, j, m0, kt0, 0, B, modified, mode, sign, D,
                                  D[8]);
+}
+
+// Batched launch with 128 x 128 tiles (par_tile_big); same descriptor tables as sum_par_batched.
+template <typename T, typename PF>
+__global__ __launch_bounds__(BigTile::NT) void sum_par_batched_big(
+    PF* __restrict__ pflows, const float* __restrict__ nflows, const float* __restrict__ nmars,
+    const float* __restrict__ emars, const float* __restrict__ params, const int* __restrict__ nids,
+    const int* __restrict__ cids, const int* __restrict__ pids, const int* __restrict__ pfids,
+    const int* __restrict__ nch, const int* __restrict__ desc, const int* __restrict__ tile_desc, int64_t B,
+    bool modified, bool overwrite, float sign) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int* D = desc + (int64_t)tile_desc[t] * PJAMD_PAR_DESC_FIELDS;
+    const int local = t - D[0];
+    const int K = D[1], mtiles = D[2], ktiles = D[3];
+    const int64_t bo = D[4], io = D[5];
+    const int m0 = (local % mtiles) * BigTile::TM;
+    const int kt0 = ((local / mtiles) % ktiles) * BigTile::TN;
+    const int j = local / (mtiles * ktiles);
+    const int mode = D[7] ? kPfAtomic : (overwrite ? kPfOverwrite : kPfAdd);
+    par_tile_big<T, PF>(pflows, nflows, nmars, emars + (int64_t)D[6] * B, params, nids + bo, cids + io, pids + io,
+                        pfids + io, nch + bo, K, B, j, m0, kt0, modified, mode, sign, D, D[8]);
 }
 
 // One 32 x 32 x KI product with the operand / accumulator lane layouts the kernels above assume:
@@ -2015,7 +2279,21 @@ int launch_par_batched(PF* pflows, const float* nflows, const float* nmars, cons
     sum_par_batched<T, TM_, TN_, PF, V_><<<(unsigned)n_tiles, 256, 0, stream>>>(                                  \
         pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite, sign)
     const bool vec = sizeof(T) == 2 && B % 4 == 0;
-    if (tile_rows == 64) {
+    if (tile_rows == 128) {
+        if constexpr (sizeof(T) == 2) {
+            if (vec) {
+                sum_par_batched_big<T, PF><<<(unsigned)n_tiles, BigTile::NT, 0, stream>>>(
+                    pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod,
+                    overwrite, sign);
+                PJAMD_LAUNCH_CHECK();
+                return PJAMD_OK;
+            }
+        }
+        PJAMD_REQUIRE(grid_fits(4 * (int64_t)n_tiles));
+        sum_par_batched<T, 64, 64, PF, false, true><<<(unsigned)(4 * n_tiles), 256, 0, stream>>>(
+            pflows, nflows, nmars, emars, params, nids, cids, pids, pfids, nch, desc, tile_desc, B, mod, overwrite,
+            sign);
+    } else if (tile_rows == 64) {
         if (vec) PJAMD_PARB(64, 64, sizeof(T) == 2);
         else PJAMD_PARB(64, 64, false);
     } else {
@@ -2169,7 +2447,9 @@ PJAMD_EXPORT int pjamd_sum_backward_par_batched(void* param_flows, const float* 
                                                 const int* cids, const int* pids, const int* pfids, const int* nch,
                                                 const int* desc, const int* tile_desc, int n_tiles, int tile_rows,
                                                 int64_t B, int flags, int precision, hipStream_t stream) {
-    PJAMD_REQUIRE(n_tiles >= 0 && B > 0 && (tile_rows == 32 || tile_rows == 64) && precision >= 0 && precision <= 2);
+    PJAMD_REQUIRE(n_tiles >= 0 && B > 0 && (tile_rows == 32 || tile_rows == 64 || tile_rows == 128) &&
+                  precision >= 0 &&
+                  precision <= 2);
     if (n_tiles == 0) return PJAMD_OK;
     const bool mod = flags & 1;
     const float sign = (flags & 4) ? -1.0f : 1.0f;

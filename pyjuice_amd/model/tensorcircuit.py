@@ -493,15 +493,19 @@ class TensorCircuit(nn.Module):
                 overlapping.update((i, hid))
             if r[1] > hi:
                 hi, hid = r[1], i
-        acc = {64: [], 32: []}
+        # Tile shapes (rows x children): 128 x 128 where a partition has >= 128 rows and children
+        # per block (half the L2 traffic and exp work per parameter of 64 x 64; off with
+        # ``par_tile_rows = 64``), else 64 x 64, or 32 x 128 for block size 32.
+        acc = {128: [], 64: [], 32: []}
         for gid, layer in sum_layers:
             bs = layer.block_size
             if bs < 32:
                 continue
             erow = self._elem_offsets[gid // 2]
-            tm, tn = (64, 64) if bs >= 64 else (32, 128)
             atomic = layer._pf_shared or id(layer) in overlapping
             for e, (nids, cids, pids, pfids, nch, tot) in enumerate(layer._dev["fw"]):
+                wide = self.par_tile_rows >= 128 and bs >= 128 and cids.size(1) >= 128
+                tm, tn = (128, 128) if wide else (64, 64) if bs >= 64 else (32, 128)
                 acc[tm].append((nids, cids, pids, pfids, nch, tot, bs, tn, erow, atomic, layer._fw_uniq[e]))
         batches = dict()
         for tm, parts in acc.items():
@@ -546,6 +550,8 @@ class TensorCircuit(nn.Module):
     # Tiles a group of tied partitions should keep in one batched launch: fewer, longer tiles
     # mean fewer atomic adds into the shared slots, more keep the chip busy.
     par_tied_tiles = 1024
+    # Largest parameter-flow tile height (128 or 64 rows).
+    par_tile_rows = 128
 
     def _tied_par_units(self, parts, tm):
         """Order the batched parameter-flow partitions into units: a unit is a run of partitions
@@ -583,9 +589,10 @@ class TensorCircuit(nn.Module):
         for tm, t in self._par_batches["tables"].items():
             rows, edges, edges_bs = t["work"]
             # algorithmic work: node-flow rows (and marginals unless modified) and child rows once
-            # per sample, parameters read and parameter flows read + written once
-            algo = (4 * (rows * (1 if modify else 2) + edges) * B + (4 + 2 * (8 if pf64 else 4)) * edges_bs,
-                    2 * edges_bs * B)
+            # per sample, parameters read and parameter flows written once (and read, unless
+            # overwritten)
+            pfb = (1 if overwrite else 2) * (8 if pf64 else 4)
+            algo = (4 * (rows * (1 if modify else 2) + edges) * B + (4 + pfb) * edges_bs, 2 * edges_bs * B)
             N.call("pjamd_sum_backward_par_batched", N.ptr(pf), N.ptr(self.node_flows), N.ptr(self.node_mars),
                    N.ptr(self.element_mars), N.ptr(self.params), N.ptr(t["nids"]), N.ptr(t["cids"]),
                    N.ptr(t["pids"]), N.ptr(t["pfids"]), N.ptr(t["nch"]), N.ptr(t["desc"]), N.ptr(t["tile_desc"]),
