@@ -205,10 +205,13 @@ int pjamd_sum_forward(float* node_mars, const float* element_mars, const float* 
 
 /* Launch hints (no PyJuice counterpart), or-ed into pjamd_sum_forward's child_run and the flags
  * of pjamd_sum_backward_ele_to_children: force the preloaded form (the first two reduction stages
- * in flight together) or the plain form of the 16-byte-load kernels instead of the default choice
- * (preloaded below 256 workgroups).  Set per layer by the launch autotuner. */
+ * in flight together) or the plain form of the 64 x 64 16-byte-load kernels, or (forward, block
+ * sizes that are multiples of 128, child runs >= 128) the 128 x 128 tile form, instead of the
+ * default choice (64 x 64 tiles, preloaded below 256 workgroups).  Set per layer by the launch
+ * autotuner. */
 #define PJAMD_LAUNCH_PRE (1 << 28)
 #define PJAMD_LAUNCH_NOPRE (1 << 27)
+#define PJAMD_LAUNCH_WIDE (1 << 26)
 
 /* pjamd_sum_forward as one persistent launch: min(tiles, resident workgroups) workgroups take the
  * (node block, row tile, batch tile) items in order from the counter queue[0] until none is left,

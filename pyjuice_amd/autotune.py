@@ -138,6 +138,8 @@ def _fw_candidates(layer, B: int, prec: str) -> List[Optional[str]]:
         flags = run | (PARAMS_ALIGNED if al else 0)
         if al and bs >= 64 and run >= 128 and run % 128 == 0:
             out += ["pre", "nopre"]
+            if bs % 128 == 0:
+                out.append("wide")
         if layer.queue_ok(flags, p, B):
             out.append("queue")
         plan = layer._split_plan(e, t[0].numel(), t[4], run, al, p, B)

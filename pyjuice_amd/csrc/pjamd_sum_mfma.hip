@@ -24,6 +24,7 @@ using namespace pjamd;
 namespace {
 
 constexpr int TK = 128;
+constexpr int TKB = 64;     // reduction steps per stage of the 128 x 128 tile kernels
 constexpr int kXcds = 8;
 
 template <typename T, int LD>
@@ -641,6 +642,162 @@ __global__ __launch_bounds__(256) void sum_fwd_vec(float* __restrict__ node_mars
             const float v = acc[r];
             node_mars[(base + acc_row(r, lane)) * B + b] = (v > 0.0f && m != kNegInf) ? __logf(v) + m : kNegInf;
         }
+    }
+}
+
+// Forward with 128 x 128 tiles (16-bit operands, child runs >= TK, aligned parameter rows,
+// B % 4 == 0, block sizes >= 128), for wide layers.  A 64 x 64 tile moves (64 + 64) K floats
+// from L2 for 4096 outputs; 128 x 128 tiles halve that per output (the limit of the 64 x 64 form
+// on wide layers: a CU keeps a bounded number of L2 requests in flight).  Four wavefronts, each a
+// 64 x 64 quadrant of four 32 x 32 accumulators; stages of TKB = 64 children.  Element rows:
+// thread (g = tid / 32, q = tid % 32) loads children 8 g .. 8 g + 7 at columns 4 q .. 4 q + 3;
+// parameters: thread (iq = tid % 32, kg = tid / 32) loads node rows 4 iq .. 4 iq + 3 of children
+// 8 kg .. 8 kg + 7.  The stage's column max goes through LDS once; every thread then folds it
+// into the running maxima it needs (its four load columns and its two accumulator columns), so a
+// stage has two barriers.
+template <typename T>
+__global__ __launch_bounds__(256) void sum_fwd_big(float* __restrict__ node_mars, const float* __restrict__ emars,
+                                                   const float* __restrict__ params, const int* __restrict__ nids,
+                                                   const int* __restrict__ cids, const int* __restrict__ pids,
+                                                   const int* __restrict__ nch, int K, int64_t B, int mtiles,
+                                                   int btiles) {
+    constexpr int TM = 128, TN = 128, LD = TKB + MF<T>::PAD;
+    __shared__ __attribute__((aligned(16))) T As[TM][LD];     // parameters [node row][child]
+    __shared__ __attribute__((aligned(16))) T Bs[TN][LD];     // exp(e - max) [column][child]
+    __shared__ __attribute__((aligned(16))) float red[4][TN];  // per-wavefront stage column max
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int m0 = (t % mtiles) * TM;
+    const int j = t / (mtiles * btiles);
+    const int64_t b0 = (int64_t)((t / mtiles) % btiles) * TN;
+    const int bs = mtiles * TM;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w & 1, wn = w >> 1;
+    const int n = nch[j];
+    const int* crow = cids + (int64_t)j * K;
+    const int* prow = pids + (int64_t)j * K;
+    const int g = tid >> 5, q = tid & 31;
+    const int64_t bq = b0 + 4 * q;
+    const bool bok = bq < B;                        // B % 4 == 0: a quad is all in or all out
+    const int64_t bqc = bok ? bq : B - 4;
+    const int iq = tid & 31, kg = tid >> 5;
+    const float4 ninf = make_float4(kNegInf, kNegInf, kNegInf, kNegInf);
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = floatx16{0};
+    float mq[4] = {kNegInf, kNegInf, kNegInf, kNegInf};   // running max of columns 4 q + c
+    float ma[2] = {kNegInf, kNegInf};                     // running max of accumulator columns
+
+    float4 ev[8], av[8];
+    auto load = [&](int k0) {
+        // child run layout: the stage's children are consecutive element rows with parameters bs apart
+        const int64_t c0 = crow[k0], p0 = prow[k0];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = 8 * g + r;
+            const int kc = (k0 + k < n) ? k : 0;
+            ev[r] = *reinterpret_cast<const float4*>(emars + (c0 + kc) * B + bqc);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int k = 8 * kg + r;
+            const int kc = (k0 + k < n) ? k : 0;
+            av[r] = *reinterpret_cast<const float4*>(params + p0 + (int64_t)kc * bs + m0 + 4 * iq);
+        }
+    };
+    if (n > 0) load(0);
+    for (int k0 = 0; k0 < n; k0 += TKB) {
+        // padding children and columns past B are -inf (element rows) and 0 (parameters)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (!(k0 + 8 * g + r < n && bok)) ev[r] = ninf;
+            if (!(k0 + 8 * kg + r < n)) av[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        float m4[4] = {kNegInf, kNegInf, kNegInf, kNegInf};
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            m4[0] = fmaxf(m4[0], ev[r].x);
+            m4[1] = fmaxf(m4[1], ev[r].y);
+            m4[2] = fmaxf(m4[2], ev[r].z);
+            m4[3] = fmaxf(m4[3], ev[r].w);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m4[c] = fmaxf(m4[c], __shfl_xor(m4[c], 32, 64));
+        if (lane < 32) *reinterpret_cast<float4*>(&red[w][4 * q]) = make_float4(m4[0], m4[1], m4[2], m4[3]);
+        __syncthreads();
+        {
+            float4 x = *reinterpret_cast<const float4*>(&red[0][4 * q]);
+#pragma unroll
+            for (int v = 1; v < 4; ++v) {
+                const float4 y = *reinterpret_cast<const float4*>(&red[v][4 * q]);
+                x = make_float4(fmaxf(x.x, y.x), fmaxf(x.y, y.y), fmaxf(x.z, y.z), fmaxf(x.w, y.w));
+            }
+            mq[0] = fmaxf(mq[0], x.x);
+            mq[1] = fmaxf(mq[1], x.y);
+            mq[2] = fmaxf(mq[2], x.z);
+            mq[3] = fmaxf(mq[3], x.w);
+        }
+#pragma unroll
+        for (int sj = 0; sj < 2; ++sj) {
+            const int col = wn * 64 + sj * 32 + (lane & 31);
+            const float x = fmaxf(fmaxf(red[0][col], red[1][col]), fmaxf(red[2][col], red[3][col]));
+            const float mo = ma[sj], mn = fmaxf(mo, x);
+            const float sc = (mn == kNegInf) ? 1.0f : __expf(mo - mn);
+            ma[sj] = mn;
+#pragma unroll
+            for (int si = 0; si < 2; ++si)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[si][sj][r] *= sc;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float e[8];
+            const float ns = neg_shift2(mq[c]);   // exp(e - max); 0 for -inf entries
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const float x = c == 0 ? ev[r].x : c == 1 ? ev[r].y : c == 2 ? ev[r].z : ev[r].w;
+                e[r] = op_exp2(fmaf(x, kLog2e, ns));
+            }
+            Pack8<T>::store(&Bs[4 * q + c][8 * g], e);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) x[r] = c == 0 ? av[r].x : c == 1 ? av[r].y : c == 2 ? av[r].z : av[r].w;
+            Pack8<T>::store(&As[4 * iq + c][8 * kg], x);
+        }
+        __syncthreads();
+        if (k0 + TKB < n) load(k0 + TKB);
+#pragma unroll
+        for (int kk = 0; kk < TKB; kk += 2 * MF<T>::KPL) {
+            const int kc = kk + MF<T>::KPL * (lane >> 5);
+            const T* a0 = &As[wm * 64 + (lane & 31)][kc];
+            const T* a1 = &As[wm * 64 + 32 + (lane & 31)][kc];
+            const T* c0 = &Bs[wn * 64 + (lane & 31)][kc];
+            const T* c1 = &Bs[wn * 64 + 32 + (lane & 31)][kc];
+            acc[0][0] = MF<T>::mma(a0, c0, acc[0][0]);
+            acc[0][1] = MF<T>::mma(a0, c1, acc[0][1]);
+            acc[1][0] = MF<T>::mma(a1, c0, acc[1][0]);
+            acc[1][1] = MF<T>::mma(a1, c1, acc[1][1]);
+        }
+    }
+    const int64_t base = (int64_t)nids[j] + m0 + wm * 64;
+#pragma unroll
+    for (int sj = 0; sj < 2; ++sj) {
+        const int64_t b = b0 + wn * 64 + sj * 32 + (lane & 31);
+        if (b >= B) continue;
+        const float m = ma[sj];
+#pragma unroll
+        for (int si = 0; si < 2; ++si)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float v = acc[si][sj][r];
+                node_mars[(base + si * 32 + acc_row(r, lane)) * B + b] =
+                    (v > 0.0f && m != kNegInf) ? __logf(v) + m : kNegInf;
+            }
     }
 }
 
@@ -1652,8 +1809,6 @@ __device__ __forceinline__ void par_tile(PF* __restrict__ pflows, const float* _
 // tiles, columns 4 xq .. 4 xq + 3.  The column max is reduced inside each wavefront by lane
 // shuffles and across the four through LDS.  The epilogue stages each quadrant through LDS so
 // that the slots of one child (consecutive rows) are read and written as contiguous runs.
-constexpr int TKB = 64;
-
 struct BigTile {
     static constexpr int TM = 128, TN = 128, NW = 4, NT = 64 * NW, RG = NT / 16;
     static constexpr int NQX = TM / RG, NQY = TN / RG;
@@ -2005,13 +2160,23 @@ int launch_fwd(float* node_mars, const float* emars, const float* params, const 
     const int64_t btiles = cdiv(B, TN);
     const int64_t tiles = (int64_t)n_blocks * mtiles * btiles;
     PJAMD_REQUIRE(bs % TM == 0 && grid_fits(tiles));
-    const int hint = child_run & (PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
-    child_run &= ~(PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE);
+    const int hint = child_run & (PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE | PJAMD_LAUNCH_WIDE);
+    child_run &= ~(PJAMD_LAUNCH_PRE | PJAMD_LAUNCH_NOPRE | PJAMD_LAUNCH_WIDE);
     const bool aligned = (child_run & PJAMD_PARAMS_ALIGNED) != 0;
     child_run &= ~PJAMD_PARAMS_ALIGNED;
     const bool run = child_run >= TK && child_run % TK == 0;
     bool vec = false;
     if constexpr (sizeof(T) == 2) {
+        // 128 x 128 tiles when asked for (measured per layer: fewer bytes from L2 per output, but
+        // half the tiles, so fewer loads in flight per CU; faster on some wide layers only)
+        const int64_t wide_tiles = (int64_t)n_blocks * (bs / 128) * cdiv(B, 128);
+        const bool wide_ok = run && aligned && bs >= 128 && bs % 128 == 0 && B % 4 == 0 && grid_fits(wide_tiles);
+        if ((hint & PJAMD_LAUNCH_WIDE) && wide_ok) {
+            sum_fwd_big<T><<<(unsigned)wide_tiles, 256, 0, stream>>>(node_mars, emars, params, nids, cids, pids, nch,
+                                                                      K, B, bs / 128, (int)cdiv(B, 128));
+            PJAMD_LAUNCH_CHECK();
+            return PJAMD_OK;
+        }
         // node blocks of 32 rows, or child runs shorter than a stage: block-spanning form
         if (aligned && child_run >= 16 && B % 4 == 0 && (bs == 32 || (bs >= 64 && !run))) {
             if (bs == 32)

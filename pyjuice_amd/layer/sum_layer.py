@@ -30,7 +30,8 @@ def _unique_pow2(x: torch.Tensor) -> torch.Tensor:
 ALG_NEGATE = 16   # PJAMD_ALG_NEGATE: parameter flows of MPE / GeneralLL subtracted
 LAUNCH_PRE = 1 << 28     # PJAMD_LAUNCH_PRE
 LAUNCH_NOPRE = 1 << 27   # PJAMD_LAUNCH_NOPRE
-_HINT_BITS = {"pre": LAUNCH_PRE, "nopre": LAUNCH_NOPRE}
+LAUNCH_WIDE = 1 << 26    # PJAMD_LAUNCH_WIDE
+_HINT_BITS = {"pre": LAUNCH_PRE, "nopre": LAUNCH_NOPRE, "wide": LAUNCH_WIDE}
 PARAMS_ALIGNED = 1 << 30   # PJAMD_PARAMS_ALIGNED
 
 
@@ -284,7 +285,8 @@ class SumLayer(Layer, nn.Module):
         bs = self.block_size
         fused = prod_tables.get(id(self)) if prod_tables is not None else None
         # launch choice of this layer (set by pyjuice_amd.autotune.tune_launches): "pre" / "nopre"
-        # force a form of the 16-byte-load kernels, "queue" runs the persistent work-queue kernel,
+        # force a form of the 64 x 64 16-byte-load kernels, "wide" the 128 x 128 one, "queue" runs
+        # the persistent work-queue kernel,
         # "nosplit" turns the split reduction off, "split<N>" sets its split count, "valu" runs FP32
         # operands on the vector ALUs
         fh = (self.__dict__.get("launch_hints") or {}).get("fw")
