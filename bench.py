@@ -32,14 +32,16 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "fp32_valu": 157.3, "bf16": 2500.0, "fp16": 2500.0}
 # HBM bytes per launch (MB) measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the headline workload
-# at block sizes 256 and 64, scaled by the counters' reading of a 1 GiB copy; commands and tables in
+# at block sizes 256 (chain runs on) and 64, scaled by the counters' reading of a 1 GiB copy; commands
+# and tables in
 # profiles/hclt784_L256_B512_bf16_hbm_traffic.txt and profiles/hclt784_L256_bs64_B512_bf16_hbm_traffic.txt.
 # Reported as roofline.traffic for the dominant kernel (counters cannot be read from inside the timed
 # process).
 TRAFFIC_MB_PER_LAUNCH = {
-    256: {"pjamd_sum_backward_ele_to_children": 34.01, "pjamd_sum_forward": 15.15,
-          "pjamd_sum_backward_par_batched": 1199.0, "pjamd_prod_forward": 16.38, "pjamd_input_backward": 835.16,
-          "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.84, "pjamd_input_em": 847.86},
+    256: {"pjamd_sum_backward_ele_to_children": 133.36, "pjamd_sum_forward": 59.32,
+          "pjamd_sum_backward_par_batched": 1008.62, "pjamd_prod_forward": 53.75, "pjamd_input_backward": 835.17,
+          "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.85, "pjamd_input_em": 847.86,
+          "pjamd_chain_forward": 190.0, "pjamd_chain_backward": 261.34, "pjamd_chain_pack": 145.43},
     64: {"pjamd_sum_backward_ele_to_children": 35.21, "pjamd_sum_forward": 16.56,
          "pjamd_sum_backward_par_batched": 1197.8, "pjamd_prod_forward": 16.12, "pjamd_input_backward": 835.14,
          "pjamd_input_forward": 636.21, "pjamd_sum_em": 569.95, "pjamd_input_em": 847.86},
