@@ -719,6 +719,45 @@ def test_tied_copies_share_parameter_flow_tiles(shape, precision, device):
     assert checked > 0
 
 
+@pytest.mark.parametrize("name", ["hclt_cat_bs128", "hclt_cat_L256"])
+@pytest.mark.parametrize("precision,B", [("bf16", 300), ("bf16", 13), ("fp16", 64), ("fp32", 300)])
+def test_parameter_flow_tile_forms_agree(name, precision, B, device):
+    """The batched parameter flows from 128 x 128 tiles (16-bit operands with B % 4 == 0: one
+    workgroup per tile, 64-column stages, LDS-staged epilogue; otherwise four 64 x 64 workgroups
+    per tile) against the 64 x 64 tile table, and both against the FP64 oracle."""
+    U.say(f"[par tile forms] {name} {precision} B={B}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision=precision, **kw)
+    pc.to(device)
+    X = U.random_data(cards, B, seed=21)
+    xt = torch.from_numpy(X).to(device)
+    out = dict()
+    for rows in (128, 64):
+        pc.par_tile_rows = rows
+        pc._par_batches = pc._build_par_batches(xt.device)
+        if rows == 128:
+            assert 128 in pc._par_batches["tables"], sorted(pc._par_batches["tables"])
+        pc(xt)
+        pc.backward(xt, flows_memory=0.0)
+        out[rows] = pc.param_flows.detach().double().cpu().numpy()
+    tol = TOL[precision][3]
+    assert U.rel_err(out[128], out[64]) <= tol
+    vals = dag_eval.forward(ns, X)
+    flows, pflows = dag_eval.backward(ns, X, vals)
+    pc.par_tile_rows = 128
+    pc._par_batches = pc._build_par_batches(xt.device)
+    pc(xt)
+    pc.backward(xt, flows_memory=0.0)
+    pc.update_param_flows(origin_ns_only=False)
+    checked = 0
+    for n in node_iterator(ns):
+        if n.is_sum() and n.provided("_param_flows") and not n.is_tied():
+            got = U.dense_pf_from_blocks(n, n._param_flows.numpy())
+            assert U.rel_err(got, pflows[n]) <= tol, (n, U.rel_err(got, pflows[n]))
+            checked += 1
+    assert checked > 0
+
+
 @pytest.mark.parametrize("name", ["simple_model", "tying_block1", "hclt_cat_bs32", "hclt_cat_bs64", "hmm_small",
                                   "pd_small", "pd_sparse"])
 @pytest.mark.parametrize("alg,alpha", [("MPE", None), ("GeneralLL", 0.5), ("GeneralLL", 2.0)])
