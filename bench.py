@@ -420,6 +420,7 @@ def main():
         sys.exit(2)
     import torch
 
+    import pyjuice_amd as juice
     from pyjuice_amd.layer.sum_layer import PRECISIONS
 
     assert torch.cuda.is_available(), "bench.py needs the HIP device"
@@ -473,7 +474,9 @@ def main():
                          "launch_forms": forms}
         log(f"[bench] {prec}: {ms:.3f} ms/step ({args.batch / (ms * 1e-3):.0f} samples/s)")
         # per-launch HIP-event timing needs eager launches (a graph replay has no host calls)
-        results[prec]["kernels"] = kernel_breakdown(make_step(pc, xb, prec, False) if args.graph else step,
+        # eager steps of the graph's own sequence (TrainStepGraph.run_step), else the timed step
+        eager = (lambda: juice.optim.TrainStepGraph.run_step(pc, xb, 0.1, 0.1, prec)) if args.graph else step
+        results[prec]["kernels"] = kernel_breakdown(eager,
                                                     min(args.steps, 5))
     if main_prec not in results:
         main_prec = "fp32"

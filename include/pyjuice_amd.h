@@ -155,6 +155,23 @@ int pjamd_input_em(int kind, float* params, const void* pflows, int pf64, const 
                    const int64_t* s_pfids, const int* ncats, int64_t num_src, float step, float pseudocount,
                    int keep_zero, pjamd_stream_t stream);
 
+/* pjamd_input_backward of a Categorical layer with the slots stored (the accumulator counts as
+ * zero: every node owns its slots) followed by pjamd_input_em of the same nodes, in one pass:
+ * each workgroup holds a node's whole flow histogram, so it stores the slots and updates the
+ * node's parameters in place from them (same formula, double accumulation).  Saves the EM
+ * launch's re-read of the slots and parameters.  Every node must be its own EM source with its
+ * own parameter and slot ranges, every tile of 8 consecutive nodes (from node 0) must share one
+ * variable and one category count, and no entry may be marginalized; B % 4 == 0, B <= 2048,
+ * kmax <= 1024 and 16-byte aligned rows, else PJAMD_EUNSUPPORTED (the caller then runs the two
+ * passes).  No single counterpart in the reference: replaces InputLayer.backward
+ * (input_layer.py:406-571) followed by InputLayer.mini_batch_em (input_layer.py:663-700) with
+ * Categorical.bk_flow_fn / small_ncats_em_fn (categorical.py:94-101, 164-205) when a training step
+ * runs them back to back. */
+int pjamd_input_backward_em(void* pflows, const float* node_flows, int pf64, const int* data, const int* vids,
+                            const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset,
+                            int64_t B, int kmax, float* params, const int64_t* s_pids, float step,
+                            float pseudocount, int keep_zero, pjamd_stream_t stream);
+
 /* For every (node n, sample s) with node_act[node_offset + n][s] != 0, write a value of
  * variable vids[n] and clear the flag: Categorical takes the first category whose FP32 running
  * CDF (left to right) reaches u, or 0 if none does; Bernoulli takes u < theta.  u is

@@ -44,6 +44,8 @@ _SIGNATURES = {
     "pjamd_input_backward_tied": [_i32, _c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i64, _i32, _i64, _i64, _i32,
                                   _c_p],
     "pjamd_input_em": [_i32, _c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _i64, _f32, _f32, _i32, _c_p],
+    "pjamd_input_backward_em": [_c_p, _c_p, _i32, _c_p, _c_p, _c_p, _c_p, _i64, _i64, _i64, _i32, _c_p, _c_p, _f32,
+                                _f32, _i32, _c_p],
     "pjamd_input_sample": [_i32, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _i32, _i64, _i64, _i64, _c_p],
     "pjamd_prod_forward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _i32, _c_p],
     "pjamd_prod_backward": [_c_p, _c_p, _c_p, _c_p, _i32, _i32, _i32, _i64, _c_p],

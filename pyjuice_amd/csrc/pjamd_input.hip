@@ -323,24 +323,81 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
 // rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
 // staging takes TILE * SB * 4 bytes of LDS: 16 KiB (8 x 512 columns, 2 x 2048 columns) or 32 KiB
 // (8 x 1024 columns); the smaller forms keep more workgroups resident.
+//
+// EM (pjamd_input_backward_em): the slots are stored, and each node's parameters are updated in
+// place from them as pjamd_input_em would (the node's whole histogram is in the workgroup): the
+// tile's parameters are loaded beside its flow rows, the flow sums reduced per node over the
+// workgroup, and the new parameters stored.  Needs every node to own its slots and parameters.
 constexpr int kGroupNodes = 64;
 
-template <typename PF, int TILE, int SB>
+// EM step of the fused backward (pjamd_input_backward_em); unused by the plain backward.
+struct CatEm {
+    float* params;
+    const int64_t* s_pids;
+    double step, pc;
+    bool keep_zero;
+};
+
+// One tile's loads of cat_bwd_group_kernel, issued a tile ahead: its flow rows (thread share of
+// NF float4) and, with EM, its parameters.  Branch-free: a tile past the group's last one
+// reloads the group's first (never used); entries past the tile are clamped to real ones.
+typedef float f32x4 __attribute__((ext_vector_type(4)));   // a register vector (float4 copies lower to memcpy)
+
+template <int TILE, int NF, bool EM, int KPT>
+__device__ __forceinline__ void cat_group_prefetch(f32x4 (&fr)[NF], float (&thn)[EM ? KPT : 1][EM ? TILE : 1],
+                                                   const float* __restrict__ node_flows, const int* __restrict__ ncats,
+                                                   int64_t g0, int tq, int64_t num_nodes, int64_t node_offset,
+                                                   int64_t B, int nq, int tid, const CatEm& em) {
+    const int64_t pt = g0 + (int64_t)tq * TILE;
+    const int64_t p0 = (tq < kGroupNodes / TILE && pt < num_nodes) ? pt : g0;
+    const int nh = (int)((num_nodes - p0) < TILE ? (num_nodes - p0) : TILE);
+#pragma unroll
+    for (int u = 0; u < NF; ++u) {
+        const int t = tid + 256 * u;
+        const int tc = t < nh * nq ? t : 0;
+        const int r = tc / nq, c = tc - r * nq;
+        fr[u] = *reinterpret_cast<const f32x4*>(node_flows + (node_offset + p0 + r) * B + 4 * c);
+    }
+    if constexpr (EM) {
+        const int kt = ncats[p0];
+#pragma unroll
+        for (int r = 0; r < TILE; ++r) {
+            const int64_t pr = em.s_pids[p0 + (r < nh ? r : 0)];
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) {
+                const int k = tid + q * 256;
+                thn[q][r] = em.params[pr + (k < kt ? k : 0)];
+            }
+        }
+    }
+}
+
+template <typename PF, int TILE, int SB, bool EM = false, int KPT = kSortK / 256>
 __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pflows,
                                                             const float* __restrict__ node_flows,
                                                             const int* __restrict__ data, const int* __restrict__ vids,
                                                             const int64_t* __restrict__ s_pfids,
                                                             const int* __restrict__ ncats, int64_t num_nodes,
-                                                            int64_t node_offset, int64_t B, int pf_mode) {
+                                                            int64_t node_offset, int64_t B, int pf_mode,
+                                                            CatEm em = CatEm{}) {
     __shared__ int cnt[kSortK + 1];
     __shared__ int cur[kSortK];
     __shared__ short perm[SB];
     __shared__ __attribute__((aligned(16))) float fl[TILE][SB];
+    __shared__ double emred[EM ? 4 : 1][TILE];
+    __shared__ double eminv[EM ? TILE : 1];
     static_assert(kSortTile % TILE == 0 && kGroupNodes % TILE == 0 && SB <= 32767, "tile and batch limits");
-    constexpr int KPT = kSortK / 256;
+    static_assert(KPT >= 1 && KPT * 256 <= kSortK, "categories per thread");   // the caller keeps K <= 256 KPT
     const int tid = threadIdx.x;
     const int nb = (int)B;                                  // B <= SB
+    const int nq = nb / 4;
     const int64_t g0 = (int64_t)blockIdx.x * kGroupNodes;
+    // Software pipeline: a tile's flow rows (and, with EM, its parameters) are loaded into
+    // registers while the previous tile is summed and updated, then copied into LDS.
+    constexpr int NF = TILE * SB / 1024;                    // float4 of a tile's flow rows per thread
+    f32x4 fr[NF];
+    float thn[EM ? KPT : 1][EM ? TILE : 1];
+    cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, g0, 0, num_nodes, node_offset, B, nq, tid, em);
     int cur_v = -1, K = 0;
     for (int tt = 0; tt < kGroupNodes / TILE; ++tt) {
         const int64_t i0 = g0 + (int64_t)tt * TILE;
@@ -385,17 +442,31 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
                 if (x >= 0 && x < K) perm[atomicAdd(&cur[x], 1)] = (short)t;
             }
         }
-        // the tile's flow rows (B % 4 == 0: 16-byte loads)
-        const int nq = nb / 4;
-        for (int t = tid; t < n_here * nq; t += 256) {
-            const int r = t / nq, c = t - r * nq;
-            *reinterpret_cast<float4*>(&fl[r][4 * c]) =
-                *reinterpret_cast<const float4*>(node_flows + (node_offset + i0 + r) * B + 4 * c);
+        // the tile's flow rows (B % 4 == 0: 16-byte loads, issued one tile ahead)
+#pragma unroll
+        for (int u = 0; u < NF; ++u) {
+            const int t = tid + 256 * u;
+            if (t < n_here * nq) {
+                const int r = t / nq, c = t - r * nq;
+                *reinterpret_cast<f32x4*>(&fl[r][4 * c]) = fr[u];
+            }
+        }
+        float th[EM ? KPT : 1][EM ? TILE : 1];
+        if constexpr (EM) {
+#pragma unroll
+            for (int q = 0; q < KPT; ++q)
+#pragma unroll
+                for (int r = 0; r < TILE; ++r) th[q][r] = thn[q][r];
         }
         __syncthreads();
+        float sq[EM ? KPT : 1][EM ? TILE : 1];   // EM: the slot values, kept for the update
 #pragma unroll
         for (int q = 0; q < KPT; ++q) {
             const int k = tid + q * 256;
+            if constexpr (EM) {
+#pragma unroll
+                for (int r = 0; r < TILE; ++r) sq[q][r] = 0.0f;
+            }
             if (k >= K) continue;
             float sums[TILE];
 #pragma unroll
@@ -405,6 +476,10 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
                 const int c = perm[p];
 #pragma unroll
                 for (int r = 0; r < TILE; ++r) sums[r] += fl[r][c];
+            }
+            if constexpr (EM) {
+#pragma unroll
+                for (int r = 0; r < TILE; ++r) sq[q][r] = sums[r];
             }
             if (pf_mode == kPfOverwrite) {
 #pragma unroll
@@ -421,6 +496,48 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
 #pragma unroll
                 for (int r = 0; r < TILE; ++r)
                     if (r < n_here) pflows[s_pfids[i0 + r] + k] = old[r] + (PF)sums[r];
+            }
+        }
+        cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, g0, tt + 1, num_nodes, node_offset, B, nq,
+                                              tid, em);
+        if constexpr (EM) {
+            // per node: the flow total (or, keeping zeros, the pseudocounted flows of the nonzero
+            // parameters) over its K categories, as pjamd_input_em sums them, from the stored values
+            const double npc = em.pc / (double)K;
+            const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+            for (int r = 0; r < TILE; ++r) {
+                double part = 0.0;
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) {
+                    if (tid + q * 256 >= K) continue;
+                    const double f = (double)(PF)sq[q][r];
+                    part += em.keep_zero ? ((th[q][r] < 1e-12f) ? 0.0 : f + npc) : f;
+                }
+                part = warp_sum(part);
+                if (lane == 0) emred[wv][r] = part;
+            }
+            __syncthreads();
+            // one division per node: step / denominator, shared through LDS
+            if (tid < TILE) {
+                const double tot = emred[0][tid] + emred[1][tid] + emred[2][tid] + emred[3][tid];
+                eminv[tid] = em.step / (em.keep_zero ? tot : tot + em.pc);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < TILE; ++r) {
+                if (r >= n_here) continue;
+                const double sc = eminv[r];
+                const int64_t pr = em.s_pids[i0 + r];
+#pragma unroll
+                for (int q = 0; q < KPT; ++q) {
+                    const int k = tid + q * 256;
+                    if (k >= K) continue;
+                    const double t = (double)th[q][r], f = (double)(PF)sq[q][r];
+                    double nv = (1.0 - em.step) * t + (f + npc) * sc;
+                    if (em.keep_zero && t < 1e-12) nv = 0.0;
+                    em.params[pr + k] = (float)nv;
+                }
             }
         }
         __syncthreads();
@@ -886,6 +1003,51 @@ PJAMD_EXPORT int pjamd_input_backward(int kind, void* pflows, const float* node_
                                            node_offset, B, pf_mode, sorted_tiles, kmax, stream);
     return input_backward_impl<float>(kind, (float*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
                                       node_offset, B, pf_mode, sorted_tiles, kmax, stream);
+}
+
+template <typename PF, int KPT>
+static int input_backward_em_impl(PF* pflows, const float* node_flows, const int* data, const int* vids,
+                                  const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset,
+                                  int64_t B, const CatEm& em, hipStream_t stream) {
+    const unsigned grid = (unsigned)cdiv(num_nodes, kGroupNodes);
+    if (B <= 512)
+        cat_bwd_group_kernel<PF, 8, 512, true, KPT><<<grid, 256, 0, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
+    else if (B <= 1024)
+        cat_bwd_group_kernel<PF, 8, 1024, true, KPT><<<grid, 256, 0, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
+    else
+        cat_bwd_group_kernel<PF, 2, 2048, true, KPT><<<grid, 256, 0, stream>>>(
+            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
+PJAMD_EXPORT int pjamd_input_backward_em(void* pflows, const float* node_flows, int pf64, const int* data,
+                                         const int* vids, const int64_t* s_pfids, const int* ncats,
+                                         int64_t num_nodes, int64_t node_offset, int64_t B, int kmax, float* params,
+                                         const int64_t* s_pids, float step, float pseudocount, int keep_zero,
+                                         hipStream_t stream) {
+    PJAMD_REQUIRE(num_nodes >= 0 && B > 0 && kmax > 0 && step > 0.0f && step <= 1.0f);
+    if (num_nodes == 0) return PJAMD_OK;
+    PJAMD_REQUIRE(num_nodes < ((int64_t)1 << 31) && params && s_pids);
+    const bool vec = B % 4 == 0 && (node_offset * B) % 4 == 0 &&
+                     ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0);
+    if (!(vec && B <= 2048 && kmax <= kSortK)) return PJAMD_EUNSUPPORTED;
+    const CatEm em{params, s_pids, (double)step, (double)pseudocount, keep_zero != 0};
+    // categories per thread: one up to 256 (K <= 256 KPT)
+    if (pf64) {
+        if (kmax <= 256)
+            return input_backward_em_impl<double, 1>((double*)pflows, node_flows, data, vids, s_pfids, ncats,
+                                                     num_nodes, node_offset, B, em, stream);
+        return input_backward_em_impl<double, 4>((double*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
+                                                 node_offset, B, em, stream);
+    }
+    if (kmax <= 256)
+        return input_backward_em_impl<float, 1>((float*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
+                                                node_offset, B, em, stream);
+    return input_backward_em_impl<float, 4>((float*)pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
+                                            node_offset, B, em, stream);
 }
 
 PJAMD_EXPORT int pjamd_input_em(int kind, float* params, const void* pflows, int pf64, const int* src,

@@ -341,10 +341,14 @@ class InputLayer(Layer, nn.Module):
 
     def backward(self, data: torch.Tensor, node_flows: torch.Tensor, node_mars: torch.Tensor,
                  params: Optional[Dict] = None, logspace_flows: bool = False,
-                 missing_mask: Optional[torch.Tensor] = None, marginalized: bool = False, **kwargs):
+                 missing_mask: Optional[torch.Tensor] = None, marginalized: bool = False,
+                 fused_em: Optional[tuple] = None, **kwargs):
         """``param_flows[s_pfid_i + x] += node_flows[off + i, b]`` (``input_layer.py:406-571``).
         Marginalized entries (``missing_mask``, or -1 in ``data`` with ``marginalized``) add their
-        flows to every category in proportion to ``theta`` (``categorical.py:103-134``)."""
+        flows to every category in proportion to ``theta`` (``categorical.py:103-134``).
+        ``fused_em = (step_size, pseudocount, keep_zero_params)``: a training step's EM follows;
+        where the layer allows (``fused_em_ok``) the flows are stored and the parameters updated in
+        the same pass (``pjamd_input_backward_em``), and the next ``mini_batch_em`` skips the layer."""
         assert not logspace_flows, "logspace flows are not supported by the HIP input layers"
         if missing_mask is not None:
             data = data.clone()
@@ -359,6 +363,19 @@ class InputLayer(Layer, nn.Module):
         overwrite = self._pf_fresh and (not t["shared_pf"] or (tg is not None and tg[3]))
         pf = self._pf if overwrite else self.param_flows
         pf64 = 1 if pf.dtype == torch.float64 else 0
+        if (fused_em is not None and overwrite and not marginalized and params is None
+                and not self._used_external_params and self.fused_em_ok(B, self._output_ind_range[0])):
+            step, pcount, keep_zero = fused_em
+            self._pf_fresh = False
+            # node flows read, slots stored, parameters read and written
+            algo = (4 * (self.num_nodes + len(self.layer_scope)) * B
+                    + pf.element_size() * self.num_param_flows + 8 * self.num_parameters, 0)
+            N.call("pjamd_input_backward_em", N.ptr(pf), N.ptr(node_flows), pf64, N.ptr(data), N.ptr(t["vids"]),
+                   N.ptr(t["s_pfids"]), N.ptr(t["ncats"]), self.num_nodes, self._output_ind_range[0], B, t["kmax"],
+                   N.ptr(self.params), N.ptr(t["s_pids"]), float(step), float(pcount), 1 if keep_zero else 0,
+                   N.stream(), algo=algo)
+            self._em_done = True
+            return
         self._pf_fresh = False
         algo = (4 * (self.num_nodes + len(self.layer_scope)) * B
                 + (1 if overwrite else 2) * pf.element_size() * self.num_param_flows, 0)
@@ -378,8 +395,20 @@ class InputLayer(Layer, nn.Module):
                    self.num_nodes, self._output_ind_range[0], B, N.stream(),
                    algo=(8 * self.num_nodes * B + 2 * pf.element_size() * self.num_param_flows, 0))
 
+    def fused_em_ok(self, B: int, node_offset: int) -> bool:
+        """Whether ``pjamd_input_backward_em`` takes this layer at batch ``B``: Categorical, every
+        node its own EM source with its own slots, tiles of 8 nodes on one variable and category
+        count, B % 4 == 0 and <= 2048, at most 1024 categories."""
+        t = self._dev
+        return (self.num_nodes > 0 and self._kind() == 0 and not t["shared_pf"] and t["tied_groups"] is None
+                and not t["tied"] and t["src"].numel() == self.num_nodes and t["sorted_tiles"]
+                and B % 4 == 0 and B <= 2048 and (node_offset * B) % 4 == 0 and t["kmax"] <= 1024)
+
     def mini_batch_em(self, step_size: float, pseudocount: float = 0.0, keep_zero_params: bool = False):
-        """Tied-flow fusing then the closed-form EM update (``input_layer.py:663-724``)."""
+        """Tied-flow fusing then the closed-form EM update (``input_layer.py:663-724``); nothing
+        for a layer whose update ran inside the backward (``fused_em``)."""
+        if self.__dict__.pop("_em_done", False):
+            return None
         if self._used_external_params:
             return None
         if self.param_flows is None:

@@ -10,6 +10,10 @@ a step is one graph launch.  Every kernel of the step clears buffers with its ow
 The step is recorded with the circuit's state at capture time: parameter flows start from zero
 each step (``flows_memory = 0``), the batch size and the input buffer are fixed, and changing the
 circuit (``pc.to``, new parameters tensor) needs a new graph.
+
+Input layers that allow it run their EM update inside the flow backward
+(``InputLayer.backward(fused_em=...)``, ``pjamd_input_backward_em``): the parameter flows are
+still stored, and the step's results are those of backward followed by ``mini_batch_em``.
 """
 from __future__ import annotations
 
@@ -26,9 +30,10 @@ class TrainStepGraph:
     log-likelihood tensor ``[B, num_roots]`` of that step (overwritten by the next replay)."""
 
     def __init__(self, pc, x_example: torch.Tensor, lr: float = 0.1, pseudocount: float = 0.1,
-                 precision: Optional[str] = None, keep_zero_params: bool = False):
+                 precision: Optional[str] = None, keep_zero_params: bool = False, fuse_em: bool = True):
         assert x_example.is_cuda and x_example.dim() == 2
         self.pc = pc
+        self.fuse_em = fuse_em
         self.lr, self.pseudocount, self.keep_zero = float(lr), float(pseudocount), keep_zero_params
         self.precision = precision
         self.x = x_example.detach().clone().contiguous()
@@ -41,14 +46,23 @@ class TrainStepGraph:
             self.lls = self._run()
         torch.cuda.synchronize()
 
-    def _run(self) -> torch.Tensor:
-        pc = self.pc
+    @staticmethod
+    def run_step(pc, x: torch.Tensor, lr: float = 0.1, pseudocount: float = 0.1, precision: Optional[str] = None,
+                 keep_zero_params: bool = False, fuse_em: bool = True) -> torch.Tensor:
+        """One eager training step, the sequence the graph records: clear the parameter flows,
+        forward, flow backward (with the input layers' EM inside it where ``fuse_em`` allows),
+        mini-batch EM."""
         with torch.no_grad():
             pc.init_param_flows(flows_memory=0.0)
-            lls = pc.forward(self.x, precision=self.precision)
-            pc.backward(self.x, flows_memory=0.0, precision=self.precision)
-            pc.mini_batch_em(step_size=self.lr, pseudocount=self.pseudocount, keep_zero_params=self.keep_zero)
+            lls = pc.forward(x, precision=precision)
+            pc.backward(x, flows_memory=0.0, precision=precision,
+                        fused_em=(lr, pseudocount, keep_zero_params) if fuse_em else None)
+            pc.mini_batch_em(step_size=lr, pseudocount=pseudocount, keep_zero_params=keep_zero_params)
         return lls
+
+    def _run(self) -> torch.Tensor:
+        return self.run_step(self.pc, self.x, self.lr, self.pseudocount, self.precision, self.keep_zero,
+                             self.fuse_em)
 
     def __call__(self, x: Optional[torch.Tensor] = None) -> torch.Tensor:
         if x is not None:

@@ -380,6 +380,50 @@ def test_train_step_graph_replays(name, device):
             assert U.rel_err(a.params.detach().cpu().numpy(), b.params.detach().cpu().numpy()) <= t_p
 
 
+@pytest.mark.parametrize("name", ["hclt_cat_bs64", "hclt_cat_L256", "hclt_bern16"])
+@pytest.mark.parametrize("B,keep_zero,pcount", [(64, False, 0.1), (300, True, 0.5), (2048, False, 0.0)])
+def test_fused_input_em_matches_two_passes(name, B, keep_zero, pcount, device):
+    """The input layers' EM run inside the flow backward (pjamd_input_backward_em, what the
+    training-step graph uses) against backward then mini_batch_em: the same stored parameter
+    flows and updated parameters, up to summation order (the batch sort of the categorical
+    backward places a category's samples in atomic order, so FP32 flow sums can differ in the
+    last bit between any two runs)."""
+    U.say(f"[fused input EM] {name} B={B} keep_zero={keep_zero}")
+    ns, cards, kw = _circuit(name)
+    pcs = [TensorCircuit(ns, precision="bf16", **kw) for _ in range(2)]
+    for pc in pcs:
+        pc.to(device)
+    pcs[1].params.data.copy_(pcs[0].params)
+    for a, b in zip(pcs[1].input_layer_group, pcs[0].input_layer_group):
+        a.params.data.copy_(b.params)
+    if keep_zero:       # some zero parameters to keep
+        for pc in pcs:
+            for layer in pc.input_layer_group:
+                layer.params.data[::7] = 0.0
+    X = torch.from_numpy(U.random_data(cards, B, seed=31)).to(device)
+    fused = [l.fused_em_ok(B, l._output_ind_range[0]) for l in pcs[0].input_layer_group]
+    if name != "hclt_bern16":
+        assert any(fused), "no input layer takes the fused form"
+    else:
+        assert not any(fused)
+    with torch.no_grad():
+        for i, pc in enumerate(pcs):
+            pc.init_param_flows(flows_memory=0.0)
+            pc(X)
+            pc.backward(X, flows_memory=0.0, fused_em=(0.3, pcount, keep_zero) if i == 0 else None)
+            if i == 0:
+                assert [bool(l.__dict__.get("_em_done")) for l in pc.input_layer_group] == fused
+            pc.mini_batch_em(step_size=0.3, pseudocount=pcount, keep_zero_params=keep_zero)
+    for a, b in zip(pcs[0].input_layer_group, pcs[1].input_layer_group):
+        fa, fb = a.param_flows.detach().cpu().numpy(), b.param_flows.detach().cpu().numpy()
+        assert np.abs(fa - fb).max() <= 1e-6 * max(1.0, np.abs(fb).max()), np.abs(fa - fb).max()
+        pa, pb = a.params.detach().cpu().numpy(), b.params.detach().cpu().numpy()
+        assert np.abs(pa - pb).max() <= 1e-6 * max(1.0, np.abs(pb).max()), np.abs(pa - pb).max()
+        if keep_zero and a._kind() == 0:      # Categorical EM keeps zero parameters at zero
+            assert np.all(pa[::7] == 0.0)
+    assert torch.equal(pcs[0].params, pcs[1].params)
+
+
 @pytest.mark.parametrize("name", ["hclt_cat_bs32", "hmm_small", "pd_sparse", "tying_block1"])
 @pytest.mark.parametrize("precision", ["bf16", "fp32_valu"])
 def test_recompute_elements_matches_resident(name, precision, device):
