@@ -25,6 +25,8 @@
 // headers are staged in LDS once, at kernel start.
 #include "pjamd_mfma.h"
 
+#include <type_traits>
+
 using namespace pjamd;
 
 namespace {
@@ -241,10 +243,12 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
         load_a(a, pa, d.a0, w, d.R >> 4, d.K >> 5, lane);
         global_children(d, gs);
     };
-    auto body = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2][2]) {
+    // FULL: a 256 x 256 group (every bound at its maximum: no guards, constant trip counts)
+    auto body = [&](auto full, int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2][2]) {
+        constexpr bool FULL = decltype(full)::value;
         const Hdr d = header(sm, o);
-        const int n32 = d.K >> 5, ntile = d.R >> 4;
-        const bool rok = 2 * g < d.K;
+        const int n32 = FULL ? kSteps : d.K >> 5, ntile = FULL ? kTiles * kWaves : d.R >> 4;
+        const bool rok = FULL || 2 * g < d.K;
         const int k = rok ? 2 * g : 0;
         const int ent = d.cb_off + (k >> d.lcbs) * (1 + d.KC);
         const int off = k & ((1 << d.lcbs) - 1);
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
 #pragma unroll
         for (int r = 0; r < 2; ++r) x[r] = (rok && okq) ? v[r] : ninf4();
         float m[4], mm;
-        column_max(sm, x, q, d.K >> 5, mcol, m, mm);
+        column_max(sm, x, q, FULL ? kWaves : d.K >> 5, mcol, m, mm);
         if (rok) stage_b(sm, x, m, g, q);
         lds_barrier();
         float (*slot)[kRing] = sm.ring[o & 1];
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
 #pragma unroll
             for (int r = 0; r < 2; ++r) *reinterpret_cast<float4*>(em + (erow + r) * B + cq) = v[r];
         }
-        if (2 * g < d.R && okq) {
+        if ((FULL || 2 * g < d.R) && okq) {
             const int p = 2 * g;
             const int64_t row = (int64_t)sm.tab[d.nid_off + (p >> d.lbs)] + (p & ((1 << d.lbs) - 1));
 #pragma unroll
@@ -305,15 +309,21 @@ __global__ __launch_bounds__(kThreads) void chain_fwd_kernel(float* nm, float* e
         }
     };
 
+    auto run_op = [&](int o, T8 (&a)[kTiles][kSteps], float4 (&gs)[2][2]) {
+        const Hdr d = header(sm, o);
+        if (d.K == kMaxRows && d.R == kMaxRows) body(std::true_type{}, o, a, gs);
+        else body(std::false_type{}, o, a, gs);
+    };
+
     T8 a0[kTiles][kSteps], a1[kTiles][kSteps];
     float4 gs0[2][2], gs1[2][2];
     if (nops > 0) prefetch(0, a0, gs0);
     for (int o = 0; o < nops; o += 2) {
         if (o + 1 < nops) prefetch(o + 1, a1, gs1);
-        body(o, a0, gs0);
+        run_op(o, a0, gs0);
         if (o + 1 >= nops) break;
         if (o + 2 < nops) prefetch(o + 2, a0, gs0);
-        body(o + 1, a1, gs1);
+        run_op(o + 1, a1, gs1);
     }
 }
 
@@ -370,6 +380,8 @@ __global__ __launch_bounds__(kThreads) void chain_bwd_kernel(float* nf, const fl
         for (int r = 0; r < 2; ++r) P.mr[r] = *reinterpret_cast<const float4*>(nm + (prow + r) * B + cqc);
         if (src < 0) load_flows(prow, P.gf);
     };
+    // (no 256 x 256 specialization as in the forward: with it the kernel needs more than 256
+    // VGPRs and spills)
     auto body = [&](int o, Pre& P) {
         const Hdr d = header(sm, o);
         const int n32 = d.R >> 5, ntile = d.K >> 4;
