@@ -10,12 +10,15 @@ constexpr int kLdsFloats = 4096; // largest category count with LDS flow histogr
 
 // node_mars[(off + i) * B + b] = log p_i(x[v_i, b]); categories past K have p = 0, negative ones
 // (missing entries) are marginalized: log 1 = 0.
-// kRowThreads threads per group of NT consecutive node rows, 4 consecutive batch columns each
-// (16-byte loads of the categories, 16-byte stores).  A thread issues the category loads of all NT
-// nodes, then all 4 NT parameter gathers (they hit each node's row of kmax floats in L1 / L2),
-// so a workgroup waits for two memory round trips per NT rows instead of per row.
-constexpr int kRowThreads = 128;
-constexpr int kFwdNodes = 4;
+// kFwdRowThreads threads per group of NT consecutive node rows, 4 consecutive batch columns each
+// per pass (16-byte loads of the categories, 16-byte stores).  A thread issues the category loads
+// of all NT nodes, then all 4 NT parameter gathers (they hit each node's row of kmax floats in
+// L1 / L2), so a workgroup waits for two memory round trips per NT rows instead of per row.
+// Measured at 200704 nodes x 512 columns (ms): 128 threads x 4 nodes 0.178, x 2 0.179, x 8 0.199;
+// 64 x 2 0.165, 64 x 4 0.168, 64 x 8 0.185; 32 x 2 0.168, 32 x 4 0.171.
+constexpr int kFwdRowThreads = 64;
+constexpr int kFwdNodes = 2;
+constexpr int kRowThreads = 128;   // the LDS-histogram backward's threads per node
 
 // log p of one data entry.  A negative category marks a marginalized (missing) entry, whose
 // probability summed over all values is 1; a category past K has probability 0.  th is the
@@ -33,10 +36,10 @@ __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict_
                                                         const int64_t* __restrict__ s_pids,
                                                         const int* __restrict__ ncats, int64_t num_nodes,
                                                         int64_t node_offset, int64_t B) {
-    const int64_t i0 = ((int64_t)blockIdx.x * (256 / kRowThreads) + threadIdx.x / kRowThreads) * NT;
+    const int64_t i0 = ((int64_t)blockIdx.x * (256 / kFwdRowThreads) + threadIdx.x / kFwdRowThreads) * NT;
     if (i0 >= num_nodes) return;
     const int nn = (num_nodes - i0) < NT ? (int)(num_nodes - i0) : NT;
-    const int sub = threadIdx.x % kRowThreads;
+    const int sub = threadIdx.x % kFwdRowThreads;
     const int* xrow[NT];
     const float* th[NT];
     int K[NT];
@@ -49,7 +52,7 @@ __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict_
         K[n] = (KIND == 0) ? ncats[i] : 2;
         p1[n] = (KIND == 1) ? th[n][0] : 0.0f;
     }
-    for (int64_t b = (int64_t)sub * 4; b < B; b += kRowThreads * 4) {
+    for (int64_t b = (int64_t)sub * 4; b < B; b += kFwdRowThreads * 4) {
         if (b + 4 <= B) {
             int4 x[NT];
 #pragma unroll
@@ -884,7 +887,7 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
         PJAMD_LAUNCH_CHECK();
         return PJAMD_OK;
     }
-    const int64_t blocks = cdiv(num_nodes, (int64_t)(256 / kRowThreads) * kFwdNodes);
+    const int64_t blocks = cdiv(num_nodes, (int64_t)(256 / kFwdRowThreads) * kFwdNodes);
     PJAMD_REQUIRE(blocks < ((int64_t)1 << 31));
     if (kind == 0)
         input_fwd_kernel<0, kFwdNodes><<<(unsigned)blocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids,
