@@ -217,6 +217,49 @@ __global__ __launch_bounds__(256) void sum_fwd_mfma(float* __restrict__ node_mar
 // chip, where the kernel time is its chain of dependent memory round trips): the child codes and
 // the marginals of the marked children are fetched at kernel start, beside the element marginals,
 // instead of after the reduction.
+// Children scatter of the fused product backward (FUSE = 1): this lane's 16 product-row flows
+// v[r] (rows acc_row(r, lane) from roff, column b) go to the same rows of each of the product
+// block's Kc child blocks, as log(flow) - mar for a marked child when the children are modified.
+// In chunks of kScatterChunk child blocks, the chunk's codes and then the marginals of its marked
+// children are all loaded before its first store, so a chunk waits for two memory round trips
+// (one when its codes come in `first`, loaded at kernel start) instead of two per child.
+constexpr int kScatterChunk = 2;
+
+__device__ __forceinline__ void scatter_children(float* child_flows, const float* __restrict__ nmars,
+                                                 const int* __restrict__ crow, int Kc, int64_t roff, int lane,
+                                                 int64_t b, int64_t B, bool mod_children, const float (&v)[16],
+                                                 const int* first = nullptr) {
+    for (int k0 = 0; k0 < Kc; k0 += kScatterChunk) {
+        int code[kScatterChunk];
+#pragma unroll
+        for (int u = 0; u < kScatterChunk; ++u)
+            code[u] = (k0 == 0 && first != nullptr) ? first[u] : (k0 + u < Kc ? crow[k0 + u] : 0);
+        float mv[kScatterChunk][16];
+#pragma unroll
+        for (int u = 0; u < kScatterChunk; ++u) {
+            if ((code[u] & 1) && mod_children) {
+                const int64_t cbase = (int64_t)(code[u] >> 1) + roff;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) mv[u][r] = nmars[(cbase + acc_row(r, lane)) * B + b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kScatterChunk; ++u) {
+            if (code[u] == 0) continue;
+            const int64_t cbase = (int64_t)(code[u] >> 1) + roff;
+            if ((code[u] & 1) && mod_children) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    child_flows[(cbase + acc_row(r, lane)) * B + b] =
+                        (mv[u][r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[u][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
+            }
+        }
+    }
+}
+
 template <typename T, int TM, int TN, bool ONE_BLOCK, int FUSE>
 __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, const float* nflows,
                                                     const float* __restrict__ nmars, const float* __restrict__ emars,
@@ -407,24 +450,8 @@ __global__ __launch_bounds__(256) void sum_ele_mfma(float* __restrict__ eflows, 
                 const float e = epre[r];
                 v[r] = (m == kNegInf || e == kNegInf) ? 0.0f : acc[r] * __expf(e + m);
             }
-            const int* crow = ch_nodes + (int64_t)j * Kc;
-            for (int k = 0; k < Kc; ++k) {
-                const int code = crow[k];
-                if (code == 0) continue;
-                const int64_t cbase = (int64_t)(code >> 1) + m0 + wm * 32;
-                if ((code & 1) && mod_children) {
-                    float mv[16];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) mv[r] = nmars[(cbase + acc_row(r, lane)) * B + b];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r)
-                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
-                            (mv[r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[r];
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
-                }
-            }
+            scatter_children(child_flows, nmars, ch_nodes + (int64_t)j * Kc, Kc, m0 + wm * 32, lane, b, B, mod_children,
+                             v);
         } else {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -869,6 +896,13 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
     for (int r = 0; r < 16; ++r) epre[r] = (b < B) ? emars[(obase + acc_row(r, lane)) * B + b] : kNegInf;
     int cpre[FUSE == 2 ? kPreChildren : 1];
     float mpre[FUSE == 2 ? kPreChildren : 1][16];
+    // FUSE = 1: the first chunk of child codes (uniform), fetched beside the reduction's first loads
+    int cfirst[kScatterChunk];
+    if (FUSE == 1) {
+        const int* crow = ch_nodes + (int64_t)j * Kc;
+#pragma unroll
+        for (int u = 0; u < kScatterChunk; ++u) cfirst[u] = u < Kc ? crow[u] : 0;
+    }
     if (FUSE == 2) {
         const int* crow = ch_nodes + (int64_t)j * Kc;
         const int64_t bc = b < B ? b : B - 1;   // clamped column: unconditional loads
@@ -939,24 +973,8 @@ __global__ __launch_bounds__(256) void sum_ele_vec(float* __restrict__ eflows, c
                 }
             }
         } else if (FUSE) {
-            const int* crow = ch_nodes + (int64_t)j * Kc;
-            for (int k = 0; k < Kc; ++k) {
-                const int code = crow[k];
-                if (code == 0) continue;
-                const int64_t cbase = (int64_t)(code >> 1) + m0 + wm * 32;
-                if ((code & 1) && mod_children) {
-                    float mv[16];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) mv[r] = nmars[(cbase + acc_row(r, lane)) * B + b];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r)
-                        child_flows[(cbase + acc_row(r, lane)) * B + b] =
-                            (mv[r] == kNegInf) ? kNegInf : __logf(v[r]) - mv[r];
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) child_flows[(cbase + acc_row(r, lane)) * B + b] = v[r];
-                }
-            }
+            scatter_children(child_flows, nmars, ch_nodes + (int64_t)j * Kc, Kc, m0 + wm * 32, lane, b, B, mod_children,
+                             v, FUSE == 1 ? cfirst : nullptr);
         } else {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
