@@ -196,6 +196,41 @@ def test_headline_hclt784_L256_bs256_bf16(device):
     _em_matches(pc, cpu, 0.1, 0.1, 2e-6)
 
 
+def test_headline_training_step_as_benchmarked(device):
+    """The headline step as bench.py runs it (chain runs on, the input layers' EM inside the
+    backward, TrainStepGraph.run_step) against the FP64 oracle's forward, flow backward and EM on
+    the same batch and starting parameters: every LL column, the stored sum and input parameter
+    flows, and the updated parameters."""
+    U.say("[headline step] HCLT-784, 256 latents, block size 256, batch 512, BF16, chains, fused input EM")
+    ns = _hclt784(device, 256)
+    pc = juice.compile(ns)
+    pc.to(device)
+    pc.use_chains = True
+    x = _images(512, seed=13).astype(np.int64)
+    assert pc._chain_plan("bf16", 512) is not None, "no chain run planned"
+    assert all(l.fused_em_ok(512, l._output_ind_range[0]) for l in pc.input_layer_group)
+    cpu = CompiledCPU(pc, np.float64)          # the starting parameters
+    xt = torch.from_numpy(x).to(device)
+    step, pcount = 0.1, 0.1
+    lls = juice.optim.TrainStepGraph.run_step(pc, xt, step, pcount, "bf16").detach().cpu().numpy()[:, 0]
+    torch.cuda.synchronize()
+    assert all(not l.__dict__.get("_em_done", False) for l in pc.input_layer_group)   # consumed by the EM
+    ref = cpu.forward(x)[0]
+    mean, mx = _ll_err_per_var(lls, ref, x.shape[1])
+    U.say(f"  |dLL|/var over 512 samples: mean {mean:.2e}, max {mx:.2e}")
+    assert mx <= 1e-3, mx
+    cpu.backward()
+    e_sum, e_in = _sum_pflows_close(pc, cpu, 2e-2), _input_pflows_close(pc, cpu)
+    U.say(f"  parameter flows rel. err: sum {e_sum:.2e}, input {e_in:.2e}")
+    assert e_sum <= 2e-2 and e_in <= 2e-2, (e_sum, e_in)
+    cpu.mini_batch_em(step_size=step, pseudocount=pcount)
+    worst = float(np.abs(pc.params.detach().cpu().numpy() - cpu.params).max())
+    for layer, d in zip(pc.input_layer_group, cpu.inputs):
+        worst = max(worst, float(np.abs(layer.params.detach().cpu().numpy() - d["params"][:layer.params.numel()]).max()))
+    U.say(f"  EM parameters max abs err: {worst:.2e}")
+    assert worst <= 1e-3, worst
+
+
 @pytest.mark.parametrize("precision", ["bf16", "fp16"])
 def test_config3_pd_28x28_L256(precision, device):
     U.say(f"[config 3] PD 28x28, 256 latents, batch 1024, {precision}")
