@@ -2,6 +2,8 @@
 // mini-batch EM and sampling.  Data is [num_vars, B] int32; node rows are [num_nodes, B].
 #include "pjamd_common.h"
 
+#include <atomic>
+
 using namespace pjamd;
 
 namespace {
@@ -319,11 +321,13 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
     }
 }
 
-// The same sums when the whole batch fits one chunk (B <= SB), for the kGroupNodes consecutive
-// nodes of a workgroup taken as tiles of TILE nodes (TILE divides kSortTile, so a tile lies inside
-// one uniform tile of the caller's promise): a variable's counting sort is done once for all the
-// group's tiles of that variable (the tiles of one variable are consecutive), each tile's flow
-// rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
+// The same sums when the whole batch fits one chunk (B <= SB), over tiles of TILE consecutive
+// nodes (TILE divides kSortTile, so a tile lies inside one uniform tile of the caller's promise).
+// Workgroup w takes the contiguous tile range [w T / G, (w + 1) T / G) of the T tiles (G
+// workgroups: as many as the chip keeps resident, so every CU gets the same share and no partial
+// last wave of workgroups trails the launch).  A variable's counting sort is done once per
+// workgroup for all its tiles of that variable (the tiles of one variable are consecutive), each
+// tile's flow rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
 // staging takes TILE * SB * 4 bytes of LDS: 16 KiB (8 x 512 columns, 2 x 2048 columns) or 32 KiB
 // (8 x 1024 columns); the smaller forms keep more workgroups resident.
 //
@@ -331,7 +335,6 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
 // place from them as pjamd_input_em would (the node's whole histogram is in the workgroup): the
 // tile's parameters are loaded beside its flow rows, the flow sums reduced per node over the
 // workgroup, and the new parameters stored.  Needs every node to own its slots and parameters.
-constexpr int kGroupNodes = 64;
 
 // EM step of the fused backward (pjamd_input_backward_em); unused by the plain backward.
 struct CatEm {
@@ -342,17 +345,16 @@ struct CatEm {
 };
 
 // One tile's loads of cat_bwd_group_kernel, issued a tile ahead: its flow rows (thread share of
-// NF float4) and, with EM, its parameters.  Branch-free: a tile past the group's last one
-// reloads the group's first (never used); entries past the tile are clamped to real ones.
+// NF float4) and, with EM, its parameters.  Branch-free: the caller passes a real tile (past its
+// range's last tile, the range's first, never used); entries past the tile are clamped to real
+// ones.
 typedef float f32x4 __attribute__((ext_vector_type(4)));   // a register vector (float4 copies lower to memcpy)
 
 template <int TILE, int NF, bool EM, int KPT>
 __device__ __forceinline__ void cat_group_prefetch(f32x4 (&fr)[NF], float (&thn)[EM ? KPT : 1][EM ? TILE : 1],
                                                    const float* __restrict__ node_flows, const int* __restrict__ ncats,
-                                                   int64_t g0, int tq, int64_t num_nodes, int64_t node_offset,
+                                                   int64_t p0, int64_t num_nodes, int64_t node_offset,
                                                    int64_t B, int nq, int tid, const CatEm& em) {
-    const int64_t pt = g0 + (int64_t)tq * TILE;
-    const int64_t p0 = (tq < kGroupNodes / TILE && pt < num_nodes) ? pt : g0;
     const int nh = (int)((num_nodes - p0) < TILE ? (num_nodes - p0) : TILE);
 #pragma unroll
     for (int u = 0; u < NF; ++u) {
@@ -375,8 +377,10 @@ __device__ __forceinline__ void cat_group_prefetch(f32x4 (&fr)[NF], float (&thn)
     }
 }
 
+// Waves per SIMD the register allocation aims at: 5 for the EM form with one category per thread
+// (96 registers, no spills; the compiler's own choice, 102, leaves 4); no constraint otherwise.
 template <typename PF, int TILE, int SB, bool EM = false, int KPT = kSortK / 256>
-__global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pflows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((EM && KPT == 1) ? 5 : 1, 8))) void cat_bwd_group_kernel(PF* __restrict__ pflows,
                                                             const float* __restrict__ node_flows,
                                                             const int* __restrict__ data, const int* __restrict__ vids,
                                                             const int64_t* __restrict__ s_pfids,
@@ -389,22 +393,25 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
     __shared__ __attribute__((aligned(16))) float fl[TILE][SB];
     __shared__ double emred[EM ? 4 : 1][TILE];
     __shared__ double eminv[EM ? TILE : 1];
-    static_assert(kSortTile % TILE == 0 && kGroupNodes % TILE == 0 && SB <= 32767, "tile and batch limits");
+    static_assert(kSortTile % TILE == 0 && SB <= 32767, "tile and batch limits");
     static_assert(KPT >= 1 && KPT * 256 <= kSortK, "categories per thread");   // the caller keeps K <= 256 KPT
     const int tid = threadIdx.x;
     const int nb = (int)B;                                  // B <= SB
     const int nq = nb / 4;
-    const int64_t g0 = (int64_t)blockIdx.x * kGroupNodes;
+    const int64_t ntiles = (num_nodes + TILE - 1) / TILE;
+    const int64_t t0 = (int64_t)blockIdx.x * ntiles / gridDim.x;
+    const int64_t t1 = ((int64_t)blockIdx.x + 1) * ntiles / gridDim.x;
+    if (t0 >= t1) return;
     // Software pipeline: a tile's flow rows (and, with EM, its parameters) are loaded into
     // registers while the previous tile is summed and updated, then copied into LDS.
     constexpr int NF = TILE * SB / 1024;                    // float4 of a tile's flow rows per thread
     f32x4 fr[NF];
     float thn[EM ? KPT : 1][EM ? TILE : 1];
-    cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, g0, 0, num_nodes, node_offset, B, nq, tid, em);
+    cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, t0 * TILE, num_nodes, node_offset, B, nq, tid,
+                                          em);
     int cur_v = -1, K = 0;
-    for (int tt = 0; tt < kGroupNodes / TILE; ++tt) {
-        const int64_t i0 = g0 + (int64_t)tt * TILE;
-        if (i0 >= num_nodes) break;
+    for (int64_t tt = t0; tt < t1; ++tt) {
+        const int64_t i0 = tt * TILE;
         const int n_here = (int)((num_nodes - i0) < TILE ? (num_nodes - i0) : TILE);
         const int v = vids[i0];
         const int k_tile = ncats[i0];
@@ -501,8 +508,8 @@ __global__ __launch_bounds__(256) void cat_bwd_group_kernel(PF* __restrict__ pfl
                     if (r < n_here) pflows[s_pfids[i0 + r] + k] = old[r] + (PF)sums[r];
             }
         }
-        cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, g0, tt + 1, num_nodes, node_offset, B, nq,
-                                              tid, em);
+        cat_group_prefetch<TILE, NF, EM, KPT>(fr, thn, node_flows, ncats, (tt + 1 < t1 ? tt + 1 : t0) * TILE,
+                                              num_nodes, node_offset, B, nq, tid, em);
         if constexpr (EM) {
             // per node: the flow total (or, keeping zeros, the pseudocounted flows of the nonzero
             // parameters) over its K categories, as pjamd_input_em sums them, from the stored values
@@ -899,6 +906,34 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
     return PJAMD_OK;
 }
 
+// Launches cat_bwd_group_kernel over ceil(num_nodes / TILE) tiles with min(tiles, resident)
+// workgroups, resident = the kernel's occupancy per CU (looked up once per instance) x the CUs.
+template <typename PF, int TILE, int SB, bool EM = false, int KPT = kSortK / 256>
+static int launch_group(PF* pflows, const float* node_flows, const int* data, const int* vids, const int64_t* s_pfids,
+                        const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B, int pf_mode,
+                        const CatEm& em, hipStream_t stream) {
+    static std::atomic<int> occupancy{0};
+    int occ = occupancy.load(std::memory_order_relaxed);
+    hipError_t e;
+    if (occ == 0) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, cat_bwd_group_kernel<PF, TILE, SB, EM, KPT>, 256, 0);
+        if (e != hipSuccess) return (int)e;
+        occ = occ > 0 ? occ : 1;
+        occupancy.store(occ, std::memory_order_relaxed);
+    }
+    int dev = 0, cus = 0;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return (int)e;
+    const int64_t tiles = cdiv(num_nodes, (int64_t)TILE), workers = (int64_t)occ * (cus > 0 ? cus : 1);
+    const unsigned grid = (unsigned)(tiles < workers ? tiles : workers);
+    cat_bwd_group_kernel<PF, TILE, SB, EM, KPT><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
+                                                                          num_nodes, node_offset, B, pf_mode, em);
+    PJAMD_LAUNCH_CHECK();
+    return PJAMD_OK;
+}
+
 template <typename PF>
 static int input_backward_impl(int kind, PF* pflows, const float* node_flows, const int* data, const int* vids,
                                const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset,
@@ -908,16 +943,15 @@ static int input_backward_impl(int kind, PF* pflows, const float* node_flows, co
                                                                     node_offset, B, pf_mode);
     } else if (sorted_tiles && kmax <= kSortK && B <= 2048 && B % 4 == 0 &&
                ((((uintptr_t)node_flows) | ((uintptr_t)data)) % 16 == 0) && (node_offset * B) % 4 == 0) {
-        const unsigned grid = (unsigned)cdiv(num_nodes, kGroupNodes);
-        if (B <= 512)   // 16 KiB of staged flows: twice the resident workgroups of the 1024-column form
-            cat_bwd_group_kernel<PF, 8, 512><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
-                                                                        num_nodes, node_offset, B, pf_mode);
-        else if (B <= 1024)
-            cat_bwd_group_kernel<PF, 8, 1024><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
-                                                                        num_nodes, node_offset, B, pf_mode);
-        else
-            cat_bwd_group_kernel<PF, 2, 2048><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
-                                                                        num_nodes, node_offset, B, pf_mode);
+        // 512 columns: 16 KiB of staged flows, twice the resident workgroups of the 1024-column form
+        if (B <= 512)
+            return launch_group<PF, 8, 512>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B,
+                                            pf_mode, CatEm{}, stream);
+        if (B <= 1024)
+            return launch_group<PF, 8, 1024>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B,
+                                             pf_mode, CatEm{}, stream);
+        return launch_group<PF, 2, 2048>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B,
+                                         pf_mode, CatEm{}, stream);
     } else if (sorted_tiles && kmax <= kSortK) {
         cat_bwd_sorted_kernel<PF><<<(unsigned)cdiv(num_nodes, kSortTile), 256, 0, stream>>>(
             pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, pf_mode);
@@ -1012,18 +1046,14 @@ template <typename PF, int KPT>
 static int input_backward_em_impl(PF* pflows, const float* node_flows, const int* data, const int* vids,
                                   const int64_t* s_pfids, const int* ncats, int64_t num_nodes, int64_t node_offset,
                                   int64_t B, const CatEm& em, hipStream_t stream) {
-    const unsigned grid = (unsigned)cdiv(num_nodes, kGroupNodes);
     if (B <= 512)
-        cat_bwd_group_kernel<PF, 8, 512, true, KPT><<<grid, 256, 0, stream>>>(
-            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
-    else if (B <= 1024)
-        cat_bwd_group_kernel<PF, 8, 1024, true, KPT><<<grid, 256, 0, stream>>>(
-            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
-    else
-        cat_bwd_group_kernel<PF, 2, 2048, true, KPT><<<grid, 256, 0, stream>>>(
-            pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset, B, kPfOverwrite, em);
-    PJAMD_LAUNCH_CHECK();
-    return PJAMD_OK;
+        return launch_group<PF, 8, 512, true, KPT>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
+                                                   node_offset, B, kPfOverwrite, em, stream);
+    if (B <= 1024)
+        return launch_group<PF, 8, 1024, true, KPT>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes,
+                                                    node_offset, B, kPfOverwrite, em, stream);
+    return launch_group<PF, 2, 2048, true, KPT>(pflows, node_flows, data, vids, s_pfids, ncats, num_nodes, node_offset,
+                                                B, kPfOverwrite, em, stream);
 }
 
 PJAMD_EXPORT int pjamd_input_backward_em(void* pflows, const float* node_flows, int pf64, const int* data,
