@@ -41,9 +41,9 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
 // or a multiple of 64, and at most kRegChildren children per lane: 256 children at block size
 // >= 64, 512 at 32) the flows and parameters stay in registers between the normaliser and the
 // update, so each is read from memory once; otherwise the update re-reads them.
-constexpr int kEmWaves = 8;
-constexpr int kRegChildren = 32;
-constexpr int kEmChunk = 16;      // children per lane per round trip when they do not fit registers
+constexpr int kEmWaves = 16;
+constexpr int kRegChildren = 16;
+constexpr int kEmChunk = 8;       // children per lane per round trip when they do not fit registers
 
 template <typename PF>
 __global__ __launch_bounds__(64 * kEmWaves) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
