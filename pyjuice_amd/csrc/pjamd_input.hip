@@ -324,8 +324,8 @@ __global__ __launch_bounds__(256) void cat_bwd_sorted_kernel(PF* __restrict__ pf
 // The same sums when the whole batch fits one chunk (B <= SB), over tiles of TILE consecutive
 // nodes (TILE divides kSortTile, so a tile lies inside one uniform tile of the caller's promise).
 // Workgroup w takes the contiguous tile range [w T / G, (w + 1) T / G) of the T tiles (G
-// workgroups: as many as the chip keeps resident, so every CU gets the same share and no partial
-// last wave of workgroups trails the launch).  A variable's counting sort is done once per
+// workgroups: a whole multiple of as many as the chip keeps resident, so every slot gets the same
+// share and no partial last wave of workgroups trails the launch).  A variable's counting sort is done once per
 // workgroup for all its tiles of that variable (the tiles of one variable are consecutive), each
 // tile's flow rows are staged with 16-byte loads, and each (node, category) sum is written once.  The flow
 // staging takes TILE * SB * 4 bytes of LDS: 16 KiB (8 x 512 columns, 2 x 2048 columns) or 32 KiB
@@ -906,8 +906,9 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
     return PJAMD_OK;
 }
 
-// Launches cat_bwd_group_kernel over ceil(num_nodes / TILE) tiles with min(tiles, resident)
-// workgroups, resident = the kernel's occupancy per CU (looked up once per instance) x the CUs.
+// Launches cat_bwd_group_kernel over ceil(num_nodes / TILE) tiles with min(tiles, kPerSlot x
+// resident) workgroups, resident = the kernel's occupancy per CU (looked up once per instance) x
+// the CUs.
 template <typename PF, int TILE, int SB, bool EM = false, int KPT = kSortK / 256>
 static int launch_group(PF* pflows, const float* node_flows, const int* data, const int* vids, const int64_t* s_pfids,
                         const int* ncats, int64_t num_nodes, int64_t node_offset, int64_t B, int pf_mode,
@@ -926,7 +927,13 @@ static int launch_group(PF* pflows, const float* node_flows, const int* data, co
     if (e != hipSuccess) return (int)e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return (int)e;
-    const int64_t tiles = cdiv(num_nodes, (int64_t)TILE), workers = (int64_t)occ * (cus > 0 ? cus : 1);
+    // Workgroups per resident slot.  512 columns: one (balanced ranges; 287 us for the EM form at
+    // HCLT-784 L=256, against 293 / 299 us with 4 / 8).  More columns: a tile's time follows its
+    // variable's category skew (one thread sums a category's samples), so static ranges leave
+    // the heavy ones trailing and 8 per slot let the dispatcher even them out (HCLT-784 L=512,
+    // 2048 columns: 1466 / 1324 / 1281 / 1266 us with 1 / 2 / 4 / 8).
+    constexpr int kPerSlot = SB <= 512 ? 1 : 8;
+    const int64_t tiles = cdiv(num_nodes, (int64_t)TILE), workers = (int64_t)occ * (cus > 0 ? cus : 1) * kPerSlot;
     const unsigned grid = (unsigned)(tiles < workers ? tiles : workers);
     cat_bwd_group_kernel<PF, TILE, SB, EM, KPT><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
                                                                           num_nodes, node_offset, B, pf_mode, em);
