@@ -927,12 +927,13 @@ static int launch_group(PF* pflows, const float* node_flows, const int* data, co
     if (e != hipSuccess) return (int)e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return (int)e;
-    // Workgroups per resident slot.  512 columns: one (balanced ranges; 287 us for the EM form at
-    // HCLT-784 L=256, against 293 / 299 us with 4 / 8).  More columns: a tile's time follows its
-    // variable's category skew (one thread sums a category's samples), so static ranges leave
-    // the heavy ones trailing and 8 per slot let the dispatcher even them out (HCLT-784 L=512,
-    // 2048 columns: 1466 / 1324 / 1281 / 1266 us with 1 / 2 / 4 / 8).
-    constexpr int kPerSlot = SB <= 512 ? 1 : 8;
+    // Workgroups per resident slot.  Up to 1024 columns: one (balanced ranges; the EM form at
+    // HCLT-784 L=256, 512 columns: 287 us, against 293 / 299 us with 4 / 8; PD 28x28 L=256, 1024
+    // columns: 0.42 ms, against 0.46 ms with 8).  At 2048 columns a tile's time follows its
+    // variable's category skew (one thread sums a category's samples), so static ranges leave the
+    // heavy ones trailing and 8 per slot let the dispatcher even them out (HCLT-784 L=512: 1466 /
+    // 1324 / 1281 / 1266 us with 1 / 2 / 4 / 8).
+    constexpr int kPerSlot = SB <= 1024 ? 1 : 8;
     const int64_t tiles = cdiv(num_nodes, (int64_t)TILE), workers = (int64_t)occ * (cus > 0 ? cus : 1) * kPerSlot;
     const unsigned grid = (unsigned)(tiles < workers ? tiles : workers);
     cat_bwd_group_kernel<PF, TILE, SB, EM, KPT><<<grid, 256, 0, stream>>>(pflows, node_flows, data, vids, s_pfids, ncats,
