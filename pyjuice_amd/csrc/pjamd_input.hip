@@ -101,6 +101,69 @@ __global__ __launch_bounds__(256) void input_fwd_kernel(const float* __restrict_
     }
 }
 
+// Categorical forward with each node's log-parameters staged in LDS (every category count <=
+// kFwdLdsK, B % 4 == 0): a wavefront owns NT consecutive nodes, loads their parameter rows with
+// coalesced loads issued beside its first category loads (no load waits for another), takes
+// each parameter's log once instead of once per sample, and reads log p from its own LDS rows.
+// There is no workgroup barrier: a wavefront reads only the rows it wrote, after its LDS writes
+// have completed (lgkmcnt 0).
+constexpr int kFwdLdsK = 256;
+
+template <int NT>
+__global__ __launch_bounds__(256) void input_fwd_lds_kernel(const float* __restrict__ params,
+                                                            float* __restrict__ node_mars,
+                                                            const int* __restrict__ data, const int* __restrict__ vids,
+                                                            const int64_t* __restrict__ s_pids,
+                                                            const int* __restrict__ ncats, int64_t num_nodes,
+                                                            int64_t node_offset, int64_t B) {
+    __shared__ float lt[4][NT][kFwdLdsK];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + w) * NT;
+    if (i0 >= num_nodes) return;   // the whole wavefront
+    const int nn = (num_nodes - i0) < NT ? (int)(num_nodes - i0) : NT;
+    const int* xrow[NT];
+    int K[NT];
+    float pv[NT][kFwdLdsK / 64];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int64_t i = i0 + (n < nn ? n : 0);    // rows past the layer repeat row i0, never stored
+        xrow[n] = data + (int64_t)vids[i] * B;
+        K[n] = ncats[i];
+        const float* th = params + s_pids[i];
+#pragma unroll
+        for (int u = 0; u < kFwdLdsK / 64; ++u) {
+            const int k = lane + 64 * u;
+            pv[n][u] = th[k < K[n] ? k : 0];
+        }
+    }
+    int64_t b = 4 * (int64_t)lane;
+    int4 xc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) xc[n] = b < B ? *reinterpret_cast<const int4*>(xrow[n] + b) : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int u = 0; u < kFwdLdsK / 64; ++u) lt[w][n][lane + 64 * u] = __logf(pv[n][u]);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    for (; b < B; b += 256) {
+        const int64_t bn = b + 256;
+        int4 xn[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            xn[n] = bn < B ? *reinterpret_cast<const int4*>(xrow[n] + bn) : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const int k = K[n] < kFwdLdsK ? K[n] : kFwdLdsK;   // the caller promises K <= kmax <= kFwdLdsK
+            const float* row = lt[w][n];
+            auto lp = [&](int x) { return x < 0 ? 0.0f : (x < k ? row[x] : kNegInf); };
+            const float4 r = make_float4(lp(xc[n].x), lp(xc[n].y), lp(xc[n].z), lp(xc[n].w));
+            if (n < nn) *reinterpret_cast<float4*>(node_mars + (node_offset + i0 + n) * B + b) = r;
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) xc[n] = xn[n];
+    }
+}
+
 // Forward, generic form (any B and alignment): one thread per (node, batch column).
 __global__ __launch_bounds__(256) void cat_fwd_global_kernel(const float* __restrict__ params,
                                                              float* __restrict__ node_mars,
@@ -891,6 +954,14 @@ PJAMD_EXPORT int pjamd_input_forward(int kind, const float* params, float* node_
         dim3 grid((unsigned)num_nodes, cdiv(B, 256));
         cat_fwd_global_kernel<<<grid, 256, 0, stream>>>(params, node_mars, data, vids, s_pids, ncats, node_offset, B,
                                                         kind);
+        PJAMD_LAUNCH_CHECK();
+        return PJAMD_OK;
+    }
+    if (kind == 0 && kmax <= kFwdLdsK) {
+        const int64_t wblocks = cdiv(num_nodes, (int64_t)4 * kFwdNodes);
+        PJAMD_REQUIRE(wblocks < ((int64_t)1 << 31));
+        input_fwd_lds_kernel<kFwdNodes><<<(unsigned)wblocks, 256, 0, stream>>>(params, node_mars, data, vids, s_pids,
+                                                                               ncats, num_nodes, node_offset, B);
         PJAMD_LAUNCH_CHECK();
         return PJAMD_OK;
     }
