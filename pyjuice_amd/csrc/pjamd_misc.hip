@@ -35,27 +35,22 @@ __global__ __launch_bounds__(256) void to_vb_kernel(int* __restrict__ out, const
 
 // Mini-batch EM of the sum parameters.  Record = [par_start, pflow_start, node_start,
 // block_size, n_children]; the parameter of child c of node i sits at par_start + c * block_size
-// + i (same for flows).  A workgroup of kEmWaves wavefronts handles (record, up to 64 nodes): lanes
-// run along the nodes (ti = min(bs, 64) nodes, G = 64 / ti lanes per node) and the wavefronts
-// split the children.  When every lane owns at most kRegChildren children (a block size dividing
-// or a multiple of 64, and at most kRegChildren children per lane: 256 children at block size
+// + i (same for flows).  A workgroup of kEmWaves wavefronts handles (record, up to 64 nodes).
+// Scalar path (any tile): lanes run along the nodes (ti nodes, G = 64 / ti lanes per node) and
+// the wavefronts split the children.  When every lane owns at most kRegChildren children (a tile
+// of nodes dividing 64, and at most kRegChildren children per lane: 256 children at block size
 // >= 64, 512 at 32) the flows and parameters stay in registers between the normaliser and the
-// update, so each is read from memory once; otherwise the update re-reads them.
+// update, so each is read from memory once; otherwise the update re-reads them.  The 16-byte
+// path below takes the tiles whose rows allow it.
 constexpr int kEmWaves = 16;
 constexpr int kRegChildren = 16;
 constexpr int kEmChunk = 8;       // children per lane per round trip when they do not fit registers
 
 template <typename PF>
-__global__ __launch_bounds__(64 * kEmWaves) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
-                                                     const int* __restrict__ recs, int n_recs, double step, double pc,
-                                                     bool keep_zero) {
-    __shared__ double part[kEmWaves][64];
-    const int rec = blockIdx.x;
-    const int* R = recs + (int64_t)rec * 5;
-    const int64_t par0 = R[0], pf0 = R[1];
-    const int bs = R[3], nch = R[4];
-    if (blockIdx.y * 64 >= bs) return;
-    const int ti = bs < 64 ? bs : 64;       // nodes of this workgroup
+__device__ __forceinline__ void sum_em_scalar(float* __restrict__ params, const PF* __restrict__ pflows, int64_t par0,
+                                              int64_t pf0, int bs, int nch, int tn, double step, double pc,
+                                              bool keep_zero, double (&part)[kEmWaves][64]) {
+    const int ti = tn;                      // nodes of this workgroup: min(64, bs - 64 blockIdx.y)
     const int G = 64 / ti;                  // lanes per node inside a wavefront
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.y * 64 + (lane % ti);
@@ -140,6 +135,165 @@ __global__ __launch_bounds__(64 * kEmWaves) void sum_em_kernel(float* __restrict
             }
         }
     }
+}
+
+// The same update with 16-byte accesses, for a tile of tn = min(64, bs - i0) nodes where tn is a
+// power of two and every row start is 16-byte aligned: a lane owns 4 consecutive nodes (quad q)
+// of kVecChildren children (lanes of one quad in a wavefront, then the wavefronts, split the
+// children), loading a float4 of parameters and 16 bytes of flows per child and row; the
+// normaliser is reduced over the lanes of a quad (shuffles), then over the wavefronts (LDS).
+// A quarter of the scalar path's memory instructions for the same bytes.
+constexpr int kVecChildren = 4;
+
+template <typename PF>
+struct Flow4 {   // four consecutive flows of one child, loaded as 16-byte vectors
+    double v[4];
+    __device__ __forceinline__ void load(const PF* p) {
+        if constexpr (sizeof(PF) == 8) {
+            const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+        } else {
+            const float4 a = *reinterpret_cast<const float4*>(p);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        }
+    }
+};
+
+template <typename PF>
+__device__ __forceinline__ void sum_em_vec(float* __restrict__ params, const PF* __restrict__ pflows, int64_t par0,
+                                           int64_t pf0, int bs, int nch, int tn, double step, double pc,
+                                           bool keep_zero, double (&part)[kEmWaves][64], double* den) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = blockIdx.y * 64;
+    const int QT = tn / 4;                 // node quads of the tile (1 .. 16, a power of two)
+    const int GW = 64 / QT;                // lanes per quad in a wavefront
+    const int q = lane % QT;
+    const int g = lane / QT + GW * w;      // this lane's first child
+    const int stride = GW * kEmWaves;
+    const bool in_regs = nch <= stride * kVecChildren;
+    const double npc = pc / (double)nch;
+    const int64_t col = i0 + 4 * q;
+    auto contrib = [&](double f, float t) { return keep_zero ? ((t < 1e-12f) ? 0.0 : f + npc) : f; };
+
+    float4 tr[kVecChildren];
+    Flow4<PF> fr[kVecChildren];
+    double cum[4] = {0.0, 0.0, 0.0, 0.0};
+    if (in_regs) {
+#pragma unroll
+        for (int u = 0; u < kVecChildren; ++u) {
+            const int c = g + u * stride;
+            const int cc = c < nch ? c : g < nch ? g : 0;   // clamped to a real child, masked below
+            const int64_t off = (int64_t)cc * bs + col;
+            tr[u] = *reinterpret_cast<const float4*>(params + par0 + off);
+            fr[u].load(pflows + pf0 + off);
+        }
+#pragma unroll
+        for (int u = 0; u < kVecChildren; ++u) {
+            if (g + u * stride >= nch) continue;
+            cum[0] += contrib(fr[u].v[0], tr[u].x);
+            cum[1] += contrib(fr[u].v[1], tr[u].y);
+            cum[2] += contrib(fr[u].v[2], tr[u].z);
+            cum[3] += contrib(fr[u].v[3], tr[u].w);
+        }
+    } else {
+        for (int c0 = g; c0 < nch; c0 += kVecChildren * stride) {
+            float4 t[kVecChildren];
+            Flow4<PF> f[kVecChildren];
+#pragma unroll
+            for (int u = 0; u < kVecChildren; ++u) {
+                const int c = c0 + u * stride;
+                const int64_t off = (int64_t)(c < nch ? c : c0) * bs + col;
+                t[u] = *reinterpret_cast<const float4*>(params + par0 + off);
+                f[u].load(pflows + pf0 + off);
+            }
+#pragma unroll
+            for (int u = 0; u < kVecChildren; ++u) {
+                if (c0 + u * stride >= nch) continue;
+                cum[0] += contrib(f[u].v[0], t[u].x);
+                cum[1] += contrib(f[u].v[1], t[u].y);
+                cum[2] += contrib(f[u].v[2], t[u].z);
+                cum[3] += contrib(f[u].v[3], t[u].w);
+            }
+        }
+    }
+    // over the lanes of a quad (lane bits >= log2 QT), then over the wavefronts
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        for (int o = QT; o < 64; o <<= 1) cum[i] += __shfl_xor(cum[i], o, 64);
+    if (lane < QT) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[w][4 * q + i] = cum[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < tn) {
+        double tot = 0.0;
+#pragma unroll
+        for (int v = 0; v < kEmWaves; ++v) tot += part[v][threadIdx.x];
+        den[threadIdx.x] = keep_zero ? tot : tot + pc;
+    }
+    __syncthreads();
+    double dn[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dn[i] = den[4 * q + i];
+    auto upd = [&](double f, float t, double d) {
+        const double th = (double)t;
+        double nv = (1.0 - step) * th + step * ((f + npc) / d);
+        if (keep_zero && th < 1e-12) nv = 0.0;
+        return (float)nv;
+    };
+    if (in_regs) {
+#pragma unroll
+        for (int u = 0; u < kVecChildren; ++u) {
+            const int c = g + u * stride;
+            if (c >= nch) continue;
+            const float4 o = make_float4(upd(fr[u].v[0], tr[u].x, dn[0]), upd(fr[u].v[1], tr[u].y, dn[1]),
+                                         upd(fr[u].v[2], tr[u].z, dn[2]), upd(fr[u].v[3], tr[u].w, dn[3]));
+            *reinterpret_cast<float4*>(params + par0 + (int64_t)c * bs + col) = o;
+        }
+    } else {
+        for (int c0 = g; c0 < nch; c0 += kVecChildren * stride) {
+            float4 t[kVecChildren];
+            Flow4<PF> f[kVecChildren];
+#pragma unroll
+            for (int u = 0; u < kVecChildren; ++u) {
+                const int c = c0 + u * stride;
+                const int64_t off = (int64_t)(c < nch ? c : c0) * bs + col;
+                t[u] = *reinterpret_cast<const float4*>(params + par0 + off);
+                f[u].load(pflows + pf0 + off);
+            }
+#pragma unroll
+            for (int u = 0; u < kVecChildren; ++u) {
+                const int c = c0 + u * stride;
+                if (c >= nch) continue;
+                const float4 o = make_float4(upd(f[u].v[0], t[u].x, dn[0]), upd(f[u].v[1], t[u].y, dn[1]),
+                                             upd(f[u].v[2], t[u].z, dn[2]), upd(f[u].v[3], t[u].w, dn[3]));
+                *reinterpret_cast<float4*>(params + par0 + (int64_t)c * bs + col) = o;
+            }
+        }
+    }
+}
+
+// One workgroup per (record, tile of 64 nodes): the 16-byte path where the tile allows it
+// (vec_ptrs: both base pointers 16-byte aligned), the scalar path otherwise.
+template <typename PF>
+__global__ __launch_bounds__(64 * kEmWaves) void sum_em_kernel(float* __restrict__ params, const PF* __restrict__ pflows,
+                                                     const int* __restrict__ recs, int n_recs, double step, double pc,
+                                                     bool keep_zero, bool vec_ptrs) {
+    __shared__ double part[kEmWaves][64];
+    __shared__ double den[64];
+    const int rec = blockIdx.x;
+    const int* R = recs + (int64_t)rec * 5;
+    const int64_t par0 = R[0], pf0 = R[1];
+    const int bs = R[3], nch = R[4];
+    if (blockIdx.y * 64 >= bs || nch <= 0) return;   // uniform over the workgroup
+    const int tn = bs - (int)blockIdx.y * 64 < 64 ? bs - (int)blockIdx.y * 64 : 64;
+    constexpr int kPfAlign = 16 / (int)sizeof(PF);
+    const bool vec = vec_ptrs && bs % 4 == 0 && tn >= 4 && (tn & (tn - 1)) == 0 && par0 % 4 == 0 &&
+                     pf0 % kPfAlign == 0;
+    if (vec)
+        sum_em_vec<PF>(params, pflows, par0, pf0, bs, nch, tn, step, pc, keep_zero, part, den);
+    else
+        sum_em_scalar<PF>(params, pflows, par0, pf0, bs, nch, tn, step, pc, keep_zero, part);
 }
 
 // pf[target + j] = sum over copies c of pf[copy_c + j] (the copies include the target range).
@@ -354,12 +508,13 @@ PJAMD_EXPORT int pjamd_sum_em(float* params, const void* pflows, int pf64, const
     PJAMD_REQUIRE(n_recs >= 0 && max_bs >= 1);
     if (n_recs == 0) return PJAMD_OK;
     dim3 grid((unsigned)n_recs, cdiv(max_bs, 64));
+    const bool vec_ptrs = ((((uintptr_t)params) | ((uintptr_t)pflows)) % 16) == 0;
     if (pf64)
         sum_em_kernel<double><<<grid, 64 * kEmWaves, 0, stream>>>(params, (const double*)pflows, recs, n_recs, step,
-                                                                  pseudocount, keep_zero != 0);
+                                                                  pseudocount, keep_zero != 0, vec_ptrs);
     else
         sum_em_kernel<float><<<grid, 64 * kEmWaves, 0, stream>>>(params, (const float*)pflows, recs, n_recs, step,
-                                                                 pseudocount, keep_zero != 0);
+                                                                 pseudocount, keep_zero != 0, vec_ptrs);
     PJAMD_LAUNCH_CHECK();
     return PJAMD_OK;
 }
