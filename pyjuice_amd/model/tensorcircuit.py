@@ -12,6 +12,7 @@ Parameter flows are accumulated across mini-batches in ``pflow_dtype`` (FP64 by 
 """
 from __future__ import annotations
 
+import weakref
 from functools import partial
 from typing import Callable, Optional, Sequence, Tuple, Union
 
@@ -139,6 +140,7 @@ class TensorCircuit(nn.Module):
         self.fuse_products_max_tiles = 256
         self._prod_fusion = {"groups": set(), "tables": dict(), "tiles": dict()}
         self._data_vb, self._data_marginalized = None, False
+        self._data_src = None   # (weak reference to the forward's input tensor, its version)
 
     # ------------------------------------------------------------------ parameter-flow buffer
 
@@ -795,6 +797,9 @@ class TensorCircuit(nn.Module):
             if input_layer_fn is None:
                 data = self._to_vb(inputs, missing_mask, batch_first)
                 self._data_vb, self._data_marginalized = data, missing_mask is not None
+                # the backward reuses this transpose for the same, unmodified input tensor
+                self._data_src = ((weakref.ref(inputs), inputs._version) if missing_mask is None and batch_first
+                                  else None)
                 kw = {k: v for k, v in kwargs.items() if k not in ("missing_mask", "_batch_first")}
                 for layer in self.input_layer_group:
                     layer(data, self.node_mars, **kw)
@@ -969,8 +974,12 @@ class TensorCircuit(nn.Module):
                 self._pf_fresh = False
 
             if input_layer_fn is None:
+                src = self._data_src
                 if inputs is None:
                     data, marg = self._data_vb, self._data_marginalized
+                elif (src is not None and missing_mask is None and batch_first and src[0]() is inputs
+                      and src[1] == inputs._version and self._data_vb is not None):
+                    data, marg = self._data_vb, False
                 else:
                     data, marg = self._to_vb(inputs, missing_mask, batch_first), missing_mask is not None
                 kw = {k: v for k, v in kwargs.items() if k not in ("missing_mask", "_batch_first")}
