@@ -175,3 +175,35 @@ def test_sum_layer_callbacks(name, device):
     assert [c for k, c in calls if k == "pre"] == order
     assert [c for k, c in calls if k == "post"] == order
     assert U.rel_err(pc.param_flows.detach().cpu().numpy(), want.cpu().numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["hclt_cat_bs64", "hclt_bern16"])
+def test_backward_reuses_forward_transpose_only_for_the_same_unmodified_inputs(name, device):
+    """The backward takes the forward's [V, B] data transpose when it is handed the very input
+    tensor of the forward, unmodified; a different tensor, or the same one written in place, is
+    transposed again.  Each case is compared with a backward that transposes its own copy."""
+    U.say(f"[transpose reuse] {name}")
+    ns, cards, kw = _circuit(name)
+    pc = TensorCircuit(ns, precision="bf16", **kw)
+    pc.to(device)
+    x1 = torch.from_numpy(U.random_data(cards, 64, seed=5)).to(device)
+    x2 = torch.from_numpy(U.random_data(cards, 64, seed=6)).to(device)
+
+    def input_pflows(x_fwd, x_bwd):
+        pc(x_fwd)
+        pc.backward(x_bwd, flows_memory=0.0)
+        return [l.param_flows.detach().clone() for l in pc.input_layer_group]
+
+    def close(a, b):
+        return all(torch.allclose(u, v, rtol=1e-6, atol=1e-6) for u, v in zip(a, b))
+
+    same = input_pflows(x1, x1)
+    assert close(same, input_pflows(x1, x1.clone()))
+    # x1 overwritten with x2's values after the forward: the backward must see x2
+    xm = x1.clone()
+    pc(xm)
+    xm.copy_(x2)
+    pc.backward(xm, flows_memory=0.0)
+    written = [l.param_flows.detach().clone() for l in pc.input_layer_group]
+    assert close(written, input_pflows(x1, x2.clone()))
+    assert not close(written, same)
