@@ -97,8 +97,9 @@ int pjamd_data_mask(int* data, const uint8_t* mask, int mode, int64_t B, int64_t
 
 /* node_mars[node_offset + n][b] = log p_n(data[vids[n]][b]) for the num_nodes nodes of one
  * input layer; s_pids[n] is node n's first parameter, ncats[n] its category count and kmax the
- * largest category count of the layer (Categorical; ignored for Bernoulli).  A negative data
- * value is a marginalized entry (log p = 0); a category >= ncats[n] has p = 0.
+ * largest category count of the layer (Categorical, kmax >= every ncats[n]: up to 256 the
+ * parameter rows are staged in LDS; ignored for Bernoulli).  A negative data value is a
+ * marginalized entry (log p = 0); a category >= ncats[n] has p = 0.
  * Replaces InputLayer.forward (src/pyjuice/layer/input_layer.py:283-404) with
  * Categorical.fw_mar_fn (categorical.py:83-92) / Bernoulli.fw_mar_fn (bernoulli.py:65-71). */
 int pjamd_input_forward(int kind, const float* params, float* node_mars, const int* data, const int* vids,
